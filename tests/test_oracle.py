@@ -1,0 +1,244 @@
+"""Pins the CPU oracle: golden vectors written by the reference's own Fortran
+(tests/golden/make_golden.py) must be reproduced bit for bit, and the
+mathematics pins what the fixtures leave open (closed forms, invariants,
+brute force on tiny inputs)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from hierarquicos_amd import ics
+
+
+def _xyz(d):
+    return d["m"], d["x"], d["y"], d["z"]
+
+
+# ---------------------------------------------------------------- golden pins
+def test_direct_matches_reference_bitwise(golden):
+    d = golden("plummer_n1000_seed3.npz")
+    F = O.direct_forces(*_xyz(d), 1.0, float(d["eps2"]))
+    assert np.array_equal(F, d["F_direct"])
+
+
+@pytest.mark.parametrize("method", [10, 11, 12, 20, 21, 22, 30, 31, 32])
+def test_tree_methods_match_reference_bitwise(golden, method):
+    d = golden("plummer_n1000_seed3.npz")
+    m, x, y, z = _xyz(d)
+    t = O.Tree(m, method, 64).init(x, y, z)
+    assert t.nnodes == int(d[f"nnodes_{method}"])
+    th2, e2 = float(d["theta2"]), float(d["eps2"])
+    if method < 20:
+        F = t.bh_forces(th2, 1.0, e2)
+    elif method < 30:
+        F = t.dehnen(th2, e2, 1.0)
+    else:
+        F = t.forces_groups(th2, 1.0, e2)
+    assert np.array_equal(F, d[f"F_{method}"])
+
+
+def test_tree_store_matches_reference_bitwise(golden):
+    d = golden("plummer_n1000_seed3.npz")
+    m, x, y, z = _xyz(d)
+    e = O.Tree(m, 12, 64).init(x, y, z).export()
+    for k in ("cx", "cy", "cz", "half", "L2", "mass", "qx", "qy", "qz", "particle", "type",
+              "depth", "child", "quad", "oct", "khi", "klo"):
+        assert np.array_equal(e[k], d["tree12_" + k]), k
+
+
+@pytest.mark.parametrize("method", [11, 12, 21, 31, 32])
+def test_hernquist_deep_tree_bitwise(golden, method):
+    d = golden("hernquist_n4096_seed5.npz")
+    m, x, y, z = _xyz(d)
+    t = O.Tree(m, method, 64).init(x, y, z)
+    th2, e2 = float(d["theta2"]), float(d["eps2"])
+    if method < 20:
+        F = t.bh_forces(th2, 1.0, e2)
+    elif method < 30:
+        F = t.dehnen(th2, e2, 1.0)
+    else:
+        F = t.forces_groups(th2, 1.0, e2)
+    assert np.array_equal(F, d[f"F_{method}"])
+    if method == 11:
+        e = t.export()
+        for k in ("depth", "type", "particle", "khi", "klo", "mass"):
+            assert np.array_equal(e[k], d["tree11_" + k]), k
+
+
+def test_morton_matches_reference_bitwise(golden):
+    d = golden("plummer_n1000_seed3.npz")
+    m, x, y, z = _xyz(d)
+    q = np.stack([x, y, z], axis=1)
+    M = O.Morton(m, q, 0, 1.2, True)
+    e = M.export()
+    assert M.L == int(d["morton_L"])
+    assert M.nnodes == int(d["morton_nnodes"])
+    assert np.array_equal(e["perm"], d["morton_perm"])
+    assert np.array_equal(e["keys"], d["morton_keys_sorted"])
+    assert np.array_equal(M.forces(1.0, float(d["eps2"]), float(d["theta2"]), True),
+                          d["F_morton"])
+    keys, L = O.morton_keys(m, q, 0)
+    assert L == M.L and np.array_equal(np.sort(keys), e["keys"])
+
+
+def test_morton_ten_points(golden):
+    d = golden("points10.npz")
+    M = O.Morton(d["m"], d["q"], 0, 1.2, True)
+    e = M.export()
+    assert M.L == int(d["morton_L"])
+    for k in ("perm", "prefix", "level", "first", "last", "parent", "nchild", "child", "next",
+              "size2", "mass", "qcm", "quad"):
+        assert np.array_equal(e[k], d["morton_" + k]), k
+    assert np.array_equal(M.forces(1.0, 1e-4, 0.01, True), d["F_morton"])
+    q = d["q"]
+    for method in (11, 12):
+        t = O.Tree(d["m"], method).init(q[:, 0], q[:, 1], q[:, 2])
+        assert np.array_equal(t.bh_forces(0.01, 1.0, 1e-4), d[f"F_{method}"])
+
+
+def test_edge_cases(golden):
+    d = golden("edge_cases.npz")
+    for method in (10, 11, 12):
+        t = O.Tree(d["two_m"], method).init(d["two_x"], d["two_y"], d["two_z"])
+        assert np.array_equal(t.bh_forces(0.25, 1.0, 1e-4), d[f"two_F_{method}"])
+        t = O.Tree(d["dup_m"], method).init(d["dup_x"], d["dup_y"], d["dup_z"])
+        assert t.nnodes == int(d[f"dup_nnodes_{method}"])
+        F = t.bh_forces(0.25, 1.0, 1e-4)
+        assert np.array_equal(F, d[f"dup_F_{method}"])
+
+
+# ------------------------------------------------------------ mathematics pins
+def test_direct_two_body_closed_form():
+    m = np.array([2.0, 3.0])
+    x, y, z = np.array([0.0, 3.0]), np.array([0.0, 4.0]), np.array([0.0, 0.0])
+    F = O.direct_forces(m, x, y, z, G=1.5, eps2=0.0)
+    # |F| = G m1 m2 / r^2 along the separation, r = 5
+    expect = 1.5 * 6.0 / 25.0 * np.array([3.0, 4.0, 0.0]) / 5.0
+    assert np.allclose(F[0], expect, rtol=1e-15)
+    assert np.allclose(F[1], -expect, rtol=1e-15)
+
+
+def test_direct_momentum_and_gather_form():
+    m, q, _ = ics.plummer(300, seed=1)
+    x, y, z = q.T.copy()
+    F = O.direct_forces(m, x, y, z, 1.0, 1e-4)
+    assert np.abs(F.sum(axis=0)).max() < 1e-14 * np.abs(F).max() * 300
+    Ft = O.direct_targets(m, x, y, z, np.arange(300), 1.0, 1e-4)
+    assert np.allclose(Ft, F, rtol=1e-12, atol=1e-14 * np.abs(F).max())
+
+
+def test_bh_theta_zero_is_direct():
+    # theta = 0 opens every cell: only leaves interact (published 3.6e-15,
+    # fortran/out/forces_time_morton_parallel_20C_N10K.txt:1902)
+    m, q, _ = ics.plummer(500, seed=2)
+    x, y, z = q.T.copy()
+    Fd = O.direct_forces(m, x, y, z, 1.0, 1e-4)
+    for method in (10, 11, 12):
+        F = O.Tree(m, method).init(x, y, z).bh_forces(0.0, 1.0, 1e-4)
+        assert np.abs(F - Fd).max() <= 1e-12 * np.abs(Fd).max()
+    for method in (30, 31, 32):
+        F = O.Tree(m, method).init(x, y, z).forces_groups(0.0, 1.0, 1e-4)
+        assert np.abs(F - Fd).max() <= 1e-12 * np.abs(Fd).max()
+
+
+def test_moments_equal_brute_force_p2m():
+    # M2M translations must reproduce direct P2M over each cell's particles
+    # (the brute-force recipe of fortran/octree_debug.f90:791-847).
+    m, q, _ = ics.plummer(400, seed=4)
+    x, y, z = q.T.copy()
+    t = O.Tree(m, 12).init(x, y, z)
+    e = t.export()
+    nn = t.nnodes
+    # particles under each node
+    members = [[] for _ in range(nn)]
+    for k in range(nn - 1, -1, -1):
+        if e["type"][k] == 1:
+            members[k] = [e["particle"][k]]
+        else:
+            members[k] = sum((members[c] for c in e["child"][k] if c >= 0), [])
+    for k in range(nn):
+        if e["type"][k] != 2:
+            continue
+        idx = np.array(members[k])
+        mm = m[idx]
+        com = (mm[:, None] * q[idx]).sum(0) / mm.sum()
+        dd = q[idx] - com
+        Q = [np.sum(mm * dd[:, a] * dd[:, b]) for a, b in
+             ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+        Ot = [np.sum(mm * dd[:, i] * dd[:, j] * dd[:, j]) for i, j in
+              ((0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2))]
+        Ot.append(np.sum(mm * dd[:, 0] * dd[:, 1] * dd[:, 2]))
+        r2 = (dd * dd).sum(1)
+        aux = [np.sum(mm * dd[:, c] * r2) for c in range(3)]
+        scale = mm.sum() * np.max(np.abs(dd)) ** 2 + 1e-300
+        assert np.allclose(e["mass"][k], mm.sum(), rtol=1e-13)
+        assert np.allclose([e["qx"][k], e["qy"][k], e["qz"][k]], com, rtol=0, atol=1e-13)
+        assert np.allclose(e["quad"][k], Q, rtol=0, atol=1e-12 * scale)
+        s3 = scale * np.max(np.abs(dd)) + 1e-300
+        assert np.allclose(e["oct"][k][:10], Ot, rtol=0, atol=1e-11 * s3)
+        assert np.allclose(e["oct"][k][11:14], aux, rtol=0, atol=1e-11 * s3)
+
+
+def test_multipole_order_convergence():
+    # Particle-cell error falls with multipole order, as in
+    # benchmarks/multipoles_error/out/multipoles_error.txt (d=641, eps=0:
+    # 3.2e-7 / 2.1e-9 / 1.3e-11 for monopole / quadrupole / octupole).
+    rng = np.random.default_rng(0)
+    n = 300
+    m = np.full(n + 1, 1.0 / n)
+    q = np.zeros((n + 1, 3))
+    q[1:] = rng.standard_normal((n, 3)) * 0.3
+    q[0] = [16.0, 10.0, -6.0]  # the cluster fills one root octant, accepted whole
+    x, y, z = q.T.copy()
+    Fd = O.direct_targets(m, x, y, z, [0], 1.0, 0.0)[0]
+    errs = []
+    for method in (10, 11, 12):
+        F, c = O.Tree(m, method).init(x, y, z).bh_forces(0.5, 1.0, 0.0, targets=[0],
+                                                          counts=True)
+        assert c.sum() == 1  # one particle-cell interaction
+        errs.append(np.linalg.norm(F[0] - Fd) / np.linalg.norm(Fd))
+    assert errs[0] > 5 * errs[1] and errs[1] > 5 * errs[2] and errs[2] > 0
+
+
+def test_morton_key_interleave_brute_force():
+    # cell centres ((i+1/2)/2^L) quantise to i; the key interleaves bits MSB
+    # first with x the high bit of each octal digit (morton_octree.f90:205-212)
+    L = 3
+    for ix in range(8):
+        for iy in range(8):
+            for iz in range(8):
+                q3 = (np.array([ix, iy, iz]) + 0.5) / 8.0
+                k = O.morton_key(q3, L)
+                expect = 0
+                for b in range(L - 1, -1, -1):
+                    expect = expect * 8 + ((ix >> b) & 1) * 4 + ((iy >> b) & 1) * 2 + ((iz >> b) & 1)
+                assert k == expect
+    assert O.morton_key(np.array([1.0, 1.0, 1.0]), 4) == 8 ** 4 - 1  # capped at 2^L - 1
+
+
+def test_argsort_is_numpy_stable():
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 50, size=2000)
+    assert np.array_equal(O.argsort_stable(keys), np.argsort(keys, kind="stable"))
+
+
+def test_dehnen_conserves_momentum():
+    m, q, _ = ics.plummer(800, seed=6)
+    x, y, z = q.T.copy()
+    for method in (20, 21):
+        F = O.Tree(m, method).init(x, y, z).dehnen(0.36, 1e-4, 1.0)
+        assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max() * 800
+
+
+def test_tree_invariants_and_capacity():
+    m, q, _ = ics.plummer(2000, seed=7)
+    x, y, z = q.T.copy()
+    e = O.Tree(m, 11).init(x, y, z).export()
+    leaves = e["particle"][e["type"] == 1]
+    assert np.array_equal(np.sort(leaves), np.arange(2000))  # each particle in one leaf
+    for k in range(len(e["type"])):
+        for c in e["child"][k]:
+            if c >= 0:
+                assert e["depth"][c] == e["depth"][k] + 1
+                assert np.isclose(e["half"][c], e["half"][k] / 2)
+                for a in ("cx", "cy", "cz"):
+                    assert abs(e[a][c] - e[a][k]) <= e["half"][k] * (1 + 1e-15)
