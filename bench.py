@@ -1,0 +1,189 @@
+"""Benchmark: particles/s per force evaluation at N = 1,048,576 (Plummer,
+a = 1, r <= 10a, equal masses, G = 1, seed 0), octupole, theta = 0.5,
+eps = 1e-3, mixed precision, with the RMS relative force error against the
+FP64 direct sum on a 4096-particle sample.
+
+A step is one force evaluation from device-resident FP64 positions to
+device-resident FP64 forces: root cube, octant keys, radix sort, octree from
+sorted keys, upward P2M/M2M pass (all of hq_tree_build) and the bunched
+Barnes (1990) list walk with FP32 interactions (hq_tree_forces_bunched).
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu-baseline]
+
+Prints one JSON line.  Single GPU only: --gpus above 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense vector FP32 (published)
+FP64_PEAK_TFLOPS = 78.6
+# flops per interaction as written in the reference's grouped expressions
+# (octree_default.f90 :1978-1987 P2P = monopole, :2009-2027 quadrupole,
+# :2032-2082 octupole)
+FLOPS_P2P = 20
+FLOPS_M2P = {1: 20, 4: 61, 8: 201}
+
+
+def cpu_baseline(args):
+    """Reference Fortran (or the C port) on the host cores, bounded sample."""
+    env = dict(os.environ)
+    env.setdefault("OMP_STACKSIZE", "64M")
+    env.setdefault("OMP_PROC_BIND", "close")
+    threads = int(env.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env["OMP_NUM_THREADS"] = str(threads)
+    cmd = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), "--n", str(args.n),
+           "--seed", str(args.seed), "--theta", str(args.theta), "--eps", str(args.eps),
+           "--method", "12", "--sample", str(args.cpu_sample), "--threads", str(threads)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return {"value": None, "unit": "particles/s", "cores": threads, "kind": "reference",
+                "sample": "failed: " + r.stderr[-300:]}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    for k in ("t_build_s", "t_walk_sample_s", "checksum"):
+        out.pop(k, None)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--eps", type=float, default=1e-3)
+    ap.add_argument("--method", type=int, default=12)
+    ap.add_argument("--sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip FP64 paths and counts")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the force path here runs on a single MI355X; "
+              "see DESIGN.md for how it would shard across GPUs.", file=sys.stderr)
+        sys.exit(2)
+
+    from hierarquicos_amd import DeviceArray, OctreeType, direct_forces, ics
+    from hierarquicos_amd.device import sync
+
+    N = args.n
+    m, q, _ = ics.plummer(N, a=1.0, rmax=10.0, seed=args.seed)
+    base = None if args.no_cpu_baseline else cpu_baseline(args)   # before the GPU is touched
+
+    th2, e2 = args.theta ** 2, args.eps ** 2
+    dm = DeviceArray.from_numpy(m)
+    dx, dy, dz = (DeviceArray.from_numpy(np.ascontiguousarray(q[:, k])) for k in range(3))
+    dF = DeviceArray((3 * N,))
+    tree = OctreeType().pre_init(dm, args.method, 64, precision="mixed")
+
+    def step():
+        tree.init(dx, dy, dz)
+        tree.forces_bunched(th2, 1.0, e2, "mixed", out=dF)
+
+    for _ in range(args.warmup):
+        step()
+    sync()
+    evals, lists, builds = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        inf = tree.info()          # device (HIP event) times of this step's kernels
+        evals.append(inf["ms_eval"])
+        lists.append(inf["ms_lists"])
+        builds.append(inf["ms_build"])
+    sync()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = N / (ms_step * 1e-3)
+    info = tree.info()
+
+    # accuracy against the FP64 direct sum on a sample of targets
+    rng = np.random.default_rng(1234)
+    sample = np.sort(rng.choice(N, args.sample, replace=False))
+    Fd = direct_forces(dm, dx, dy, dz, 1.0, e2, targets=sample)
+    Fm = dF.numpy().reshape(N, 3)[sample]
+
+    def rel(F):
+        r = np.linalg.norm(F - Fd, axis=1) / np.linalg.norm(Fd, axis=1)
+        return float(np.sqrt(np.mean(r * r))), float(r.max())
+    rms, mx = rel(Fm)
+
+    extras = {}
+    flops_alg = None
+    if not args.no_extras:
+        # reference per-target BH counts (leaf size 1, same theta) from the FP64
+        # walk of methods 10-12 = bh_forces_over_p, and the plain FP64 paths
+        t64 = OctreeType().pre_init(dm, args.method, 64, precision="fp64")
+        dF64 = DeviceArray((3 * N,))
+        t64.init(dx, dy, dz)
+        sync()
+        a = time.perf_counter()
+        t64.init(dx, dy, dz)
+        t64.forces_all(th2, 1.0, e2, out=dF64)
+        sync()
+        ms_bh64 = (time.perf_counter() - a) * 1e3
+        i64 = t64.info()
+        flops_alg = FLOPS_P2P * i64["n_p2p"] + FLOPS_M2P[i64["multipole"]] * i64["n_m2p"]
+        rms_bh64, mx_bh64 = rel(dF64.numpy().reshape(N, 3)[sample])
+        a = time.perf_counter()
+        t64.init(dx, dy, dz)
+        t64.forces_bunched(th2, 1.0, e2, "fp64", out=dF64)
+        sync()
+        ms_bun64 = (time.perf_counter() - a) * 1e3
+        rms_b64, mx_b64 = rel(dF64.numpy().reshape(N, 3)[sample])
+        extras = {
+            "fp64_bunched": {"ms_per_eval": ms_bun64, "particles_per_s": N / (ms_bun64 * 1e-3),
+                             "rms_rel_err": rms_b64, "max_rel_err": mx_b64,
+                             "mixed_speedup": ms_bun64 / ms_step},
+            "fp64_reference_bh": {"ms_per_eval": ms_bh64, "particles_per_s": N / (ms_bh64 * 1e-3),
+                                  "rms_rel_err": rms_bh64, "max_rel_err": mx_bh64,
+                                  "n_p2p": i64["n_p2p"], "n_m2p": i64["n_m2p"],
+                                  "walk_ms": i64["ms_walk"]},
+        }
+
+    eval_ms = float(np.mean(evals))
+    roof = None
+    if flops_alg is not None:
+        achieved = flops_alg / (eval_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "compute_unit": "VALU (FP32 FMA/rsqrt; no matrix ops)",
+                "kernel": "k_eval_mixed<8>",
+                "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                "flops_per_launch": flops_alg, "kernel_ms": eval_ms,
+                "executed_interactions": {"cells": info["list_cells"],
+                                          "particles": info["list_parts"]}}
+    line = {
+        "metric": "particles/s per force eval at N=1M, rms rel err <=1e-4 vs FP64 direct",
+        "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Plummer sphere)",
+        "config": {"workload": f"plummer_n{N}_octupole_theta{args.theta}_eps{args.eps}",
+                   "N": N, "seed": args.seed, "theta": args.theta, "eps": args.eps,
+                   "order": {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method],
+                   "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
+                   "walk": "bunched Barnes-1990 lists, 64 particles per bunch"},
+        "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
+        "phases_ms": {"build": float(np.mean(builds)), "lists": float(np.mean(lists)),
+                      "eval": eval_ms, "keys": info["ms_keys"], "sort": info["ms_sort"],
+                      "nodes": info["ms_nodes"], "upward": info["ms_upward"]},
+        "nnodes": info["nnodes"],
+        "roofline": roof,
+        "cpu_baseline": base,
+    }
+    line.update(extras)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,13 @@
+"""hierarquicos_amd -- MI355X (gfx950) hierarchical gravity force path.
+
+GPU implementation of the Barnes-Hut / grouped / Dehnen octree force
+evaluators and the direct sum of Potalej/hierarquicos behind the reference's
+own API surface (the f2py ``api`` module and the OctreeType methods).
+"""
+from . import api, ics  # noqa: F401
+from ._lib import HQError, lib  # noqa: F401
+from .device import DeviceArray, direct_forces, morton_keys  # noqa: F401
+from .octree import OctreeType  # noqa: F401
+
+__all__ = ["api", "ics", "OctreeType", "DeviceArray", "direct_forces", "morton_keys", "lib",
+           "HQError"]

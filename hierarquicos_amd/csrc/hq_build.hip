@@ -1,0 +1,742 @@
+// hq_build.hip -- octree construction on the GPU from sorted octant-path keys.
+//
+// Reproduces the topology of the reference insertion octree
+// (fortran/octree_default.f90 init :320-387, add :547-580, add_to_subnode
+// :515-545, allocate_subnode :481-513) and its moments (incorporate_particles
+// :582-628, evaluate_multipole :744-882) bit for bit, without pointer
+// chasing:
+//   1. k_bbox_*      root cube: bbox centre, side 1.2 * extent (:345-347, :424-443)
+//   2. k_keys        per particle, the octant path of 25 levels computed with
+//                    the reference's floating-point centre recurrence
+//                    (c + sign*h/2, x >= cx, y < cy, z < cz), so boundary
+//                    cases fall exactly as in the reference
+//   3. radix sort    stable LSD radix sort (8-bit digits, wavefront ballot
+//                    ranking) of the 63-bit high key word; equal high words
+//                    are then ordered by the 12-bit low word (levels 22-25)
+//   4. k_node_*      nodes from the sorted keys: with D_i the number of
+//                    common levels of particles i and i+1, particle i starts
+//                    the nodes of depths D_{i-1}+1 .. max(D_{i-1},D_i)+1 (the
+//                    last one its leaf).  Two ballot passes place every node
+//                    in level order; children of a node are contiguous.
+//   5. k_upward      level-synchronous M2M from depth 25 to 0, children summed
+//                    in the reference's child order (octant index 1..8), with
+//                    contraction off, so mass/COM/quad/oct match bit for bit
+//   6. k_topdown     parent and "next" (skip) pointers for stackless walks
+#include <math.h>
+
+#include <algorithm>
+#include <utility>
+
+#include "hq_tree.h"
+
+namespace hq {
+
+// geom buffer layout (doubles)
+enum { G_CX = 0, G_CY, G_CZ, G_SIDE, G_HALF, G_XMIN, G_XMAX, G_YMIN, G_YMAX, G_ZMIN, G_ZMAX,
+       G_L2 = 16, G_HALFD = 48, G_SIZE = 80 };
+
+// ------------------------------------------------------------------ bbox
+__global__ __launch_bounds__(256) void k_bbox_partial(const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z, int64_t n,
+                                                      double* __restrict__ part,
+                                                      int64_t* __restrict__ status) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool bad = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            bad |= !isfinite(v[c]);
+            mn[c] = fmin(mn[c], v[c]);
+            mx[c] = fmax(mx[c], v[c]);
+        }
+    }
+    __shared__ double sh[4][6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[c] = fmin(mn[c], __shfl_xor(mn[c], o, 64));
+            mx[c] = fmax(mx[c], __shfl_xor(mx[c], o, 64));
+        }
+    }
+    if (__any(bad) && __lane_id() == 0) atomicOr((unsigned long long*)&status[ST_FLAGS], 1ull);
+    int w = threadIdx.x >> 6;
+    if (__lane_id() == 0)
+        for (int c = 0; c < 3; ++c) { sh[w][2 * c] = mn[c]; sh[w][2 * c + 1] = mx[c]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = sh[0][threadIdx.x];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+            v = (threadIdx.x & 1) ? fmax(v, sh[k][threadIdx.x]) : fmin(v, sh[k][threadIdx.x]);
+        part[blockIdx.x * 6 + threadIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ part, int nb,
+                                                   double* __restrict__ g) {
+#pragma clang fp contract(off)
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nb; b += 64)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], part[6 * b + 2 * c]);
+            mx[c] = fmax(mx[c], part[6 * b + 2 * c + 1]);
+        }
+    for (int c = 0; c < 3; ++c)
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[c] = fmin(mn[c], __shfl_xor(mn[c], o, 64));
+            mx[c] = fmax(mx[c], __shfl_xor(mx[c], o, 64));
+        }
+    if (threadIdx.x == 0) {
+        // node_size_center (octree_default.f90:424-443) and the root add_node (:345-347)
+        g[G_CX] = 0.5 * (mn[0] + mx[0]);
+        g[G_CY] = 0.5 * (mn[1] + mx[1]);
+        g[G_CZ] = 0.5 * (mn[2] + mx[2]);
+        double ext = mx[0] - mn[0];
+        ext = fmax(ext, mx[1] - mn[1]);
+        ext = fmax(ext, mx[2] - mn[2]);
+        double side = 1.2 * ext;
+        g[G_SIDE] = side;
+        g[G_HALF] = side / 2.0;
+        g[G_XMIN] = mn[0]; g[G_XMAX] = mx[0];
+        g[G_YMIN] = mn[1]; g[G_YMAX] = mx[1];
+        g[G_ZMIN] = mn[2]; g[G_ZMAX] = mx[2];
+    }
+}
+
+// A depth-d node has side ldexp(side, -d): add_node gets the parent's
+// halfside as its side (:508), and L2 = side^2 (:475).
+__global__ void k_geom_tables(double* __restrict__ g) {
+#pragma clang fp contract(off)
+    int d = threadIdx.x;
+    if (d <= MAX_DEPTH) {
+        double s = ldexp(g[G_SIDE], -d);
+        g[G_L2 + d] = s * s;
+        g[G_HALFD + d] = s / 2.0;
+    }
+}
+
+// ------------------------------------------------------------------ keys
+// One particle per thread: the reference decides the octant with x >= cx,
+// y < cy, z < cz (:527-531) and computes the child centre as c + sign*h/2
+// from the parent's halfside h (:495-505).  The digit stored here is
+// 4*xb + 2*yb + zb with xb = (x >= cx), yb = !(y < cy), zb = !(z < cz).
+__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ z, int64_t n,
+                                              const double* __restrict__ g,
+                                              uint64_t* __restrict__ khi,
+                                              uint32_t* __restrict__ klo,
+                                              int* __restrict__ idx) {
+#pragma clang fp contract(off)
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double px = x[p], py = y[p], pz = z[p];
+    double cx = g[G_CX], cy = g[G_CY], cz = g[G_CZ], h = g[G_HALF];
+    uint64_t hi = 0;
+    uint32_t lo = 0;
+#pragma unroll
+    for (int d = 1; d <= MAX_DEPTH; ++d) {
+        int xb = px >= cx;
+        int yb = !(py < cy);
+        int zb = !(pz < cz);
+        double hh = h / 2.0;
+        cx = cx + (xb ? hh : -hh);
+        cy = cy + (yb ? hh : -hh);
+        cz = cz + (zb ? hh : -hh);
+        h = hh;
+        uint32_t dig = (uint32_t)(4 * xb + 2 * yb + zb);
+        if (d <= HI_LEVELS) hi = (hi << 3) | dig;
+        else lo = (lo << 3) | dig;
+    }
+    khi[p] = hi;
+    klo[p] = lo;
+    idx[p] = (int)p;
+}
+
+// ------------------------------------------------------------------ radix sort
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per workgroup
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys,
+                                                        int64_t n, int shift,
+                                                        uint32_t* __restrict__ hist, int nblk) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        int64_t i = base + it * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: wave w of the workgroup owns the contiguous slice
+// [w*1024, (w+1)*1024) of the tile and walks it 64 keys at a time; keys of
+// equal digit inside one 64-key step are ranked by ballot (lanes in index
+// order), across steps by per-wave running counters, across waves by a
+// prefix over the per-wave totals.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
+    const uint64_t* __restrict__ kin, const int* __restrict__ vin, uint64_t* __restrict__ kout,
+    int* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ hist, int nblk) {
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    __shared__ uint32_t goff[256];
+    const int w = threadIdx.x >> 6;
+    const int lane = __lane_id();
+    for (int d = threadIdx.x; d < 256 * (RS_THREADS / 64); d += RS_THREADS) (&wcnt[0][0])[d] = 0;
+    goff[threadIdx.x] = hist[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (RS_ITEMS * 64);
+    uint64_t key[RS_ITEMS];
+    int val[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        int64_t i = base + it * 64 + lane;
+        bool ok = i < n;
+        key[it] = ok ? kin[i] : ~0ull;
+        val[it] = ok ? vin[i] : 0;
+        uint32_t dg = (uint32_t)((key[it] >> shift) & 255u);
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((dg >> b) & 1u);
+            peers &= ((dg >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t before = ok ? wcnt[w][dg] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        rank[it] = before + (uint32_t)__popcll(peers & lt);
+        bool leader = ok && (peers & lt) == 0;
+        if (leader) wcnt[w][dg] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        int d = threadIdx.x;  // one digit per thread
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < RS_THREADS / 64; ++k) {
+            uint32_t c = wcnt[k][d];
+            wcnt[k][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        int64_t i = base + it * 64 + lane;
+        if (i < n) {
+            uint32_t dg = (uint32_t)((key[it] >> shift) & 255u);
+            int64_t pos = (int64_t)goff[dg] + wcnt[w][dg] + rank[it];
+            kout[pos] = key[it];
+            vout[pos] = val[it];
+        }
+    }
+}
+
+// Runs of equal high words (particles sharing levels 1..21) are ordered by
+// the low word (levels 22..25), stably.  Such runs are rare and short.
+__global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __restrict__ klo,
+                          int* __restrict__ idx, int64_t n, int64_t* __restrict__ status) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i + 1 >= n) return;
+    if (khi[i] != khi[i + 1]) return;
+    if (i > 0 && khi[i - 1] == khi[i]) return;
+    int64_t j = i + 1;
+    while (j + 1 < n && khi[j + 1] == khi[i]) ++j;
+    if (j - i + 1 > 4096) {
+        atomicOr((unsigned long long*)&status[ST_FLAGS], 4ull);
+        return;
+    }
+    for (int64_t a = i + 1; a <= j; ++a) {
+        int v = idx[a];
+        uint32_t kv = klo[v];
+        int64_t b = a - 1;
+        while (b >= i && klo[idx[b]] > kv) {
+            idx[b + 1] = idx[b];
+            --b;
+        }
+        idx[b + 1] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather(const int* __restrict__ idx,
+                                                const double* __restrict__ x,
+                                                const double* __restrict__ y,
+                                                const double* __restrict__ z,
+                                                const double* __restrict__ m,
+                                                const uint32_t* __restrict__ klo, int64_t n,
+                                                double* __restrict__ sx, double* __restrict__ sy,
+                                                double* __restrict__ sz, double* __restrict__ sm,
+                                                uint32_t* __restrict__ sklo) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int p = idx[i];
+    sx[i] = x[p];
+    sy[i] = y[p];
+    sz[i] = z[p];
+    sm[i] = m[p];
+    sklo[i] = klo[p];
+}
+
+// D_i = number of common leading levels of the keys of sorted particles i, i+1.
+__global__ __launch_bounds__(256) void k_lcp(const uint64_t* __restrict__ khi,
+                                             const uint32_t* __restrict__ klo, int64_t n,
+                                             int8_t* __restrict__ D) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = -1;
+    if (i + 1 < n) {
+        uint64_t a = khi[i] ^ khi[i + 1];
+        if (a) {
+            d = (__clzll((long long)a) - 1) / 3;
+        } else {
+            uint32_t u = klo[i] ^ klo[i + 1];
+            d = u ? HI_LEVELS + (__clz((int)u) - 20) / 3 : MAX_DEPTH;
+        }
+    }
+    D[i] = (int8_t)d;
+}
+
+// ------------------------------------------------------------------ nodes
+struct PartRange {
+    int lo, e, hi;  // node depths lo..e start at this particle; leaf depth hi
+};
+
+__device__ __forceinline__ PartRange part_range(const int8_t* __restrict__ D, int64_t i,
+                                                int64_t n) {
+    PartRange r;
+    int dm = i > 0 ? (int)D[i - 1] : -1;
+    int dp = (int)D[i];  // D[n-1] = -1
+    r.lo = dm + 1;
+    r.hi = (dm > dp ? dm : dp) + 1;
+    r.e = r.hi <= MAX_DEPTH ? r.hi : dp;
+    return r;
+}
+
+constexpr int NB_THREADS = 256;
+constexpr int NDEP = MAX_DEPTH + 1;  // depths 0..25
+
+__global__ __launch_bounds__(NB_THREADS) void k_node_count(const int8_t* __restrict__ D,
+                                                           int64_t n,
+                                                           uint32_t* __restrict__ bcnt, int nblk,
+                                                           int64_t* __restrict__ status) {
+    __shared__ uint32_t wc[NB_THREADS / 64][NDEP];
+    int64_t i = blockIdx.x * (int64_t)NB_THREADS + threadIdx.x;
+    bool ok = i < n;
+    PartRange r = ok ? part_range(D, i, n) : PartRange{1, 0, 0};
+    int w = threadIdx.x >> 6;
+    for (int d = 0; d < NDEP; ++d) {
+        uint64_t msk = __ballot(ok && r.lo <= d && d <= r.e);
+        if (__lane_id() == 0) wc[w][d] = (uint32_t)__popcll(msk);
+    }
+    uint64_t lostm = __ballot(ok && r.hi > MAX_DEPTH);
+    if (__lane_id() == 0 && lostm) atomicAdd((unsigned long long*)&status[ST_LOST],
+                                             (unsigned long long)__popcll(lostm));
+    __syncthreads();
+    if (threadIdx.x < NDEP) {
+        uint32_t s = 0;
+        for (int k = 0; k < NB_THREADS / 64; ++k) s += wc[k][threadIdx.x];
+        bcnt[(int64_t)threadIdx.x * nblk + blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(NB_THREADS) void k_node_emit(
+    const int8_t* __restrict__ D, const uint64_t* __restrict__ khi,
+    const uint32_t* __restrict__ klo, const double* __restrict__ g, int64_t n,
+    const uint32_t* __restrict__ boff, int nblk, int64_t cap, uint8_t* __restrict__ ndepth,
+    int8_t* __restrict__ ntype, int* __restrict__ nfirst, int* __restrict__ ncstart,
+    uint8_t* __restrict__ ndigit, double* __restrict__ ncx, double* __restrict__ ncy,
+    double* __restrict__ ncz, uint64_t* __restrict__ nkhi, uint32_t* __restrict__ nklo,
+    int64_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t wpre[NB_THREADS / 64][NDEP + 1];
+    __shared__ uint32_t base[NDEP + 1];
+    int64_t i = blockIdx.x * (int64_t)NB_THREADS + threadIdx.x;
+    bool ok = i < n;
+    PartRange r = ok ? part_range(D, i, n) : PartRange{1, 0, 0};
+    int w = threadIdx.x >> 6;
+    for (int d = 0; d < NDEP; ++d) {
+        uint64_t msk = __ballot(ok && r.lo <= d && d <= r.e);
+        if (__lane_id() == 0) wpre[w][d] = (uint32_t)__popcll(msk);
+    }
+    if (__lane_id() == 0) wpre[w][NDEP] = 0;
+    __syncthreads();
+    const uint32_t total = boff[(int64_t)NDEP * nblk];
+    if (threadIdx.x <= NDEP) {
+        int d = threadIdx.x;
+        uint32_t run = 0;
+        for (int k = 0; k < NB_THREADS / 64; ++k) {
+            uint32_t c = wpre[k][d];
+            wpre[k][d] = run;
+            run += c;
+        }
+        base[d] = d < NDEP ? boff[(int64_t)d * nblk + blockIdx.x] : total;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        status[ST_NNODES] = total;
+        if ((int64_t)total > cap) atomicOr((unsigned long long*)&status[ST_FLAGS], 2ull);
+    }
+    const uint64_t lt = lanemask_lt();
+    uint64_t hi_key = ok ? khi[i] : 0;
+    uint32_t lo_key = ok ? klo[i] : 0;
+    double cx = g[G_CX], cy = g[G_CY], cz = g[G_CZ], h = g[G_HALF];
+    uint64_t mcur = __ballot(ok && r.lo <= 0 && 0 <= r.e);
+    for (int d = 0; d < NDEP; ++d) {
+        uint64_t mnext = (d + 1 < NDEP) ? __ballot(ok && r.lo <= d + 1 && d + 1 <= r.e) : 0ull;
+        if (ok && r.lo <= d && d <= r.e) {
+            int64_t k = (int64_t)base[d] + wpre[w][d] + __popcll(mcur & lt);
+            int64_t cs = (d + 1 < NDEP) ? (int64_t)base[d + 1] + wpre[w][d + 1] + __popcll(mnext & lt)
+                                        : (int64_t)total;
+            if (k < cap) {
+                ndepth[k] = (uint8_t)d;
+                ntype[k] = (int8_t)(d == r.hi ? 1 : 2);
+                nfirst[k] = (int)i;
+                ncstart[k] = (int)cs;
+                uint32_t dig = 0;
+                if (d >= 1 && d <= HI_LEVELS) dig = (uint32_t)(hi_key >> (3 * (HI_LEVELS - d))) & 7u;
+                else if (d > HI_LEVELS) dig = (lo_key >> (3 * (MAX_DEPTH - d))) & 7u;
+                ndigit[k] = (uint8_t)dig;
+                ncx[k] = cx;
+                ncy[k] = cy;
+                ncz[k] = cz;
+                uint64_t kh = d >= HI_LEVELS ? hi_key
+                                             : (d == 0 ? 0ull : hi_key & (~0ull << (3 * (HI_LEVELS - d))));
+                uint32_t kl = d <= HI_LEVELS ? 0u : (lo_key & (0xFFFu << (3 * (MAX_DEPTH - d))) & 0xFFFu);
+                nkhi[k] = kh;
+                nklo[k] = kl;
+            }
+        }
+        // descend one level along this particle's own path (same recurrence as k_keys)
+        if (d + 1 < NDEP) {
+            int dn = d + 1;
+            uint32_t dig = dn <= HI_LEVELS ? (uint32_t)(hi_key >> (3 * (HI_LEVELS - dn))) & 7u
+                                           : (lo_key >> (3 * (MAX_DEPTH - dn))) & 7u;
+            double hh = h / 2.0;
+            cx = cx + ((dig & 4u) ? hh : -hh);
+            cy = cy + ((dig & 2u) ? hh : -hh);
+            cz = cz + ((dig & 1u) ? hh : -hh);
+            h = hh;
+        }
+        mcur = mnext;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (int64_t)total <= cap) ncstart[total] = (int)total;
+}
+
+__global__ void k_levels(const uint32_t* __restrict__ boff, int nblk, int64_t* __restrict__ status) {
+    int d = threadIdx.x;
+    if (d <= NDEP) status[ST_LEVEL0 + d] = d < NDEP ? boff[(int64_t)d * nblk] : boff[(int64_t)NDEP * nblk];
+}
+
+// ------------------------------------------------------------------ moments
+// reference child slot (0-based octant index, :527-531) of a digit 4xb+2yb+zb
+__device__ __forceinline__ int ref_slot(uint32_t dig) {
+    return (int)((dig >> 2) & 1u) + 2 * (int)(1u - ((dig >> 1) & 1u)) + 4 * (int)(1u - (dig & 1u));
+}
+
+__global__ __launch_bounds__(128) void k_upward(
+    int64_t a, int64_t b, int depth, int multipole, const int8_t* __restrict__ ntype,
+    const int* __restrict__ nfirst, const int* __restrict__ ncstart,
+    const uint8_t* __restrict__ ndigit, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz, const double* __restrict__ sm,
+    double* __restrict__ nmass, double* __restrict__ ncomx, double* __restrict__ ncomy,
+    double* __restrict__ ncomz, double* __restrict__ nquad, double* __restrict__ noct,
+    int* __restrict__ ncount, int64_t nn) {
+#pragma clang fp contract(off)
+    int64_t k = a + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= b) return;
+    double Q[6] = {0, 0, 0, 0, 0, 0};
+    double O[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double mass = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
+    int count = 0;
+    if (ntype[k] == 1) {
+        int p = nfirst[k];
+        mass = sm[p];
+        qx = sx[p];
+        qy = sy[p];
+        qz = sz[p];
+        count = 1;
+    } else if (depth == MAX_DEPTH) {
+        // max-depth twig: evaluate_multipole skips it (:772-774, stays 0);
+        // incorporate_particles divides 0/0 (:614-616)
+        if (multipole == 1) {
+            qx = qx / mass;
+            qy = qy / mass;
+            qz = qz / mass;
+        }
+    } else {
+        int slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+        int cs = ncstart[k], ce = ncstart[k + 1];
+        for (int c = cs; c < ce; ++c) slot[ref_slot(ndigit[c])] = c;
+        for (int r = 0; r < 8; ++r) {
+            int c = slot[r];
+            if (c < 0) continue;
+            double mc = nmass[c];
+            mass = mass + mc;
+            qx = qx + mc * ncomx[c];
+            qy = qy + mc * ncomy[c];
+            qz = qz + mc * ncomz[c];
+            count += ncount[c];
+        }
+        qx = qx / mass;
+        qy = qy / mass;
+        qz = qz / mass;
+        if (multipole >= 4) {
+            const bool oct = multipole > 4;
+            for (int r = 0; r < 8; ++r) {
+                int c = slot[r];
+                if (c < 0) continue;
+                double pm = nmass[c];
+                double dxi = ncomx[c] - qx;
+                double dyi = ncomy[c] - qy;
+                double dzi = ncomz[c] - qz;
+                bool twig = ntype[c] == 2;
+                double Qc[6];
+                for (int j = 0; j < 6; ++j) Qc[j] = nquad[(int64_t)j * nn + c];
+                if (twig) {
+                    for (int j = 0; j < 6; ++j) Q[j] = Q[j] + Qc[j];
+                    if (oct)
+                        for (int j = 0; j < 14; ++j) O[j] = O[j] + noct[(int64_t)j * nn + c];
+                }
+                Q[0] = Q[0] + pm * (dxi * dxi);
+                Q[1] = Q[1] + pm * (dyi * dyi);
+                Q[2] = Q[2] + pm * (dzi * dzi);
+                Q[3] = Q[3] + pm * dxi * dyi;
+                Q[4] = Q[4] + pm * dxi * dzi;
+                Q[5] = Q[5] + pm * dyi * dzi;
+                if (oct) {
+                    O[0] = O[0] + pm * dxi * (dxi * dxi);
+                    O[1] = O[1] + pm * dxi * (dyi * dyi);
+                    O[2] = O[2] + pm * dxi * (dzi * dzi);
+                    O[3] = O[3] + pm * dyi * (dxi * dxi);
+                    O[4] = O[4] + pm * dyi * (dyi * dyi);
+                    O[5] = O[5] + pm * dyi * (dzi * dzi);
+                    O[6] = O[6] + pm * dzi * (dxi * dxi);
+                    O[7] = O[7] + pm * dzi * (dyi * dyi);
+                    O[8] = O[8] + pm * dzi * (dzi * dzi);
+                    O[9] = O[9] + pm * dxi * dyi * dzi;
+                    if (twig) {
+                        O[0] = O[0] + 3.0 * dxi * Qc[0];
+                        O[1] = O[1] + 2.0 * dyi * Qc[3] + dxi * Qc[1];
+                        O[2] = O[2] + 2.0 * dzi * Qc[4] + dxi * Qc[2];
+                        O[3] = O[3] + 2.0 * dxi * Qc[3] + dyi * Qc[0];
+                        O[4] = O[4] + 3.0 * dyi * Qc[1];
+                        O[5] = O[5] + 2.0 * dzi * Qc[5] + dyi * Qc[2];
+                        O[6] = O[6] + 2.0 * dxi * Qc[4] + dzi * Qc[0];
+                        O[7] = O[7] + 2.0 * dyi * Qc[5] + dzi * Qc[1];
+                        O[8] = O[8] + 3.0 * dzi * Qc[2];
+                        O[9] = O[9] + dxi * Qc[5] + dyi * Qc[4] + dzi * Qc[3];
+                    }
+                }
+            }
+            if (oct) {
+                double s = 0.0;
+                for (int j = 0; j < 10; ++j) s = s + O[j];
+                O[10] = s;
+                O[11] = (O[0] + O[1]) + O[2];
+                O[12] = (O[3] + O[4]) + O[5];
+                O[13] = (O[6] + O[7]) + O[8];
+            }
+        }
+    }
+    nmass[k] = mass;
+    ncomx[k] = qx;
+    ncomy[k] = qy;
+    ncomz[k] = qz;
+    ncount[k] = count;
+    if (multipole >= 4)
+        for (int j = 0; j < 6; ++j) nquad[(int64_t)j * nn + k] = Q[j];
+    if (multipole > 4)
+        for (int j = 0; j < 14; ++j) noct[(int64_t)j * nn + k] = O[j];
+}
+
+// parent and next (skip) pointers, one parent level per launch, top-down
+__global__ __launch_bounds__(128) void k_topdown(int64_t a, int64_t b,
+                                                 const int* __restrict__ ncstart,
+                                                 int* __restrict__ nnext,
+                                                 int* __restrict__ nparent) {
+    int64_t k = a + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= b) return;
+    if (k == 0) {
+        nnext[0] = END;
+        nparent[0] = -1;
+    }
+    int cs = ncstart[k], ce = ncstart[k + 1];
+    int nx = nnext[k];
+    for (int c = cs; c < ce; ++c) {
+        nparent[c] = (int)k;
+        nnext[c] = c + 1 < ce ? c + 1 : nx;
+    }
+}
+
+// ------------------------------------------------------------------ driver
+int read_status(hq_tree* t) {
+    HQ_CHECK_HIP(hipMemcpyAsync(t->hstatus, t->status.p, ST_SIZE * sizeof(int64_t),
+                                hipMemcpyDeviceToHost, t->stream));
+    HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    return HQ_OK;
+}
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
+// keys; the result is left in keys / vals (ktmp / vtmp are scratch).
+int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
+                     DevBuf& hist, hipStream_t s) {
+    if (n <= 1) return HQ_OK;
+    const int nblk = grid_for(n, RS_TILE);
+    int rc;
+    if ((rc = hist.ensure(((size_t)256 * nblk + 1) * sizeof(uint32_t)))) return rc;
+    uint64_t *ka = keys, *kb = ktmp;
+    int *va = vals, *vb = vtmp;
+    const int passes = (bits + 7) / 8;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist.as<uint32_t>(), nblk);
+        k_scan_excl<<<1, 1024, 0, s>>>(hist.as<uint32_t>(), (int64_t)256 * nblk);
+        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist.as<uint32_t>(),
+                                                 nblk);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        HQ_CHECK_HIP(hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        HQ_CHECK_HIP(hipMemcpyAsync(vals, va, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
+    return HQ_OK;
+}
+
+static int radix_sort(hq_tree* t) {
+    int rc;
+    if ((rc = t->khi2.ensure(t->n * sizeof(uint64_t)))) return rc;
+    if ((rc = t->idx2.ensure(t->n * sizeof(int)))) return rc;
+    return radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
+                            t->idx2.as<int>(), t->n, 63, t->hist, t->stream);
+}
+
+int tree_build(hq_tree* t) {
+    const int64_t n = t->n;
+    int rc;
+    size_t n8 = n * sizeof(double);
+    if ((rc = t->geom.ensure(G_SIZE * sizeof(double)))) return rc;
+    if ((rc = t->khi.ensure(n * 8)) || (rc = t->klo.ensure(n * 4)) || (rc = t->idx.ensure(n * 4)))
+        return rc;
+    if ((rc = t->sx.ensure(n8)) || (rc = t->sy.ensure(n8)) || (rc = t->sz.ensure(n8)) ||
+        (rc = t->sm.ensure(n8)) || (rc = t->sklo.ensure(n * 4)) || (rc = t->lcp.ensure(n)))
+        return rc;
+    // node capacity of pre_init (:219-223): 3N for N >= 1000, else 10N
+    t->cap = n >= 1000 ? 3 * n : 10 * n;
+    int64_t cap = t->cap;
+    if ((rc = t->ndepth.ensure(cap)) || (rc = t->ntype.ensure(cap)) ||
+        (rc = t->nfirst.ensure(cap * 4)) || (rc = t->ncount.ensure(cap * 4)) ||
+        (rc = t->ncstart.ensure((cap + 1) * 4)) || (rc = t->nnext.ensure(cap * 4)) ||
+        (rc = t->nparent.ensure(cap * 4)) || (rc = t->ndigit.ensure(cap)) ||
+        (rc = t->nmass.ensure(cap * 8)) || (rc = t->ncomx.ensure(cap * 8)) ||
+        (rc = t->ncomy.ensure(cap * 8)) || (rc = t->ncomz.ensure(cap * 8)) ||
+        (rc = t->ncx.ensure(cap * 8)) || (rc = t->ncy.ensure(cap * 8)) ||
+        (rc = t->ncz.ensure(cap * 8)) || (rc = t->nkhi.ensure(cap * 8)) ||
+        (rc = t->nklo.ensure(cap * 4)))
+        return rc;
+    if (t->multipole >= 4 && (rc = t->nquad.ensure(cap * 6 * 8))) return rc;
+    if (t->multipole > 4 && (rc = t->noct.ensure(cap * 14 * 8))) return rc;
+
+    hipStream_t s = t->stream;
+    HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+    HQ_CHECK_HIP(hipEventRecord(t->ev[0], s));
+    // 1. root cube
+    const int nbb = (int)std::min<int64_t>(1024, grid_for(n, 256));
+    if ((rc = t->hist.ensure((size_t)nbb * 6 * sizeof(double)))) return rc;
+    k_bbox_partial<<<nbb, 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(), t->z.as<double>(), n,
+                                       t->hist.as<double>(), t->status.as<int64_t>());
+    k_bbox_final<<<1, 64, 0, s>>>(t->hist.as<double>(), nbb, t->geom.as<double>());
+    k_geom_tables<<<1, 64, 0, s>>>(t->geom.as<double>());
+    // 2. keys
+    k_keys<<<grid_for(n, 256), 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(),
+                                            t->z.as<double>(), n, t->geom.as<double>(),
+                                            t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
+                                            t->idx.as<int>());
+    HQ_CHECK_HIP(hipEventRecord(t->ev[1], s));
+    // 3. sort
+    if ((rc = radix_sort(t))) return rc;
+    k_tie_fix<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
+                                               t->idx.as<int>(), n, t->status.as<int64_t>());
+    k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->x.as<double>(),
+                                              t->y.as<double>(), t->z.as<double>(),
+                                              t->m.as<double>(), t->klo.as<uint32_t>(), n,
+                                              t->sx.as<double>(), t->sy.as<double>(),
+                                              t->sz.as<double>(), t->sm.as<double>(),
+                                              t->sklo.as<uint32_t>());
+    HQ_CHECK_HIP(hipEventRecord(t->ev[2], s));
+    // 4. nodes
+    k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), n,
+                                           t->lcp.as<int8_t>());
+    const int nblk = grid_for(n, NB_THREADS);
+    if ((rc = t->hist.ensure(((size_t)NDEP * nblk + 1) * sizeof(uint32_t)))) return rc;
+    k_node_count<<<nblk, NB_THREADS, 0, s>>>(t->lcp.as<int8_t>(), n, t->hist.as<uint32_t>(), nblk,
+                                             t->status.as<int64_t>());
+    k_scan_excl<<<1, 1024, 0, s>>>(t->hist.as<uint32_t>(), (int64_t)NDEP * nblk);
+    k_node_emit<<<nblk, NB_THREADS, 0, s>>>(
+        t->lcp.as<int8_t>(), t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), t->geom.as<double>(),
+        n, t->hist.as<uint32_t>(), nblk, cap, t->ndepth.as<uint8_t>(), t->ntype.as<int8_t>(),
+        t->nfirst.as<int>(), t->ncstart.as<int>(), t->ndigit.as<uint8_t>(), t->ncx.as<double>(),
+        t->ncy.as<double>(), t->ncz.as<double>(), t->nkhi.as<uint64_t>(), t->nklo.as<uint32_t>(),
+        t->status.as<int64_t>());
+    k_levels<<<1, 64, 0, s>>>(t->hist.as<uint32_t>(), nblk, t->status.as<int64_t>());
+    HQ_CHECK_HIP(hipEventRecord(t->ev[3], s));
+    if ((rc = read_status(t))) return rc;
+    int64_t flags = t->hstatus[ST_FLAGS];
+    if (flags & 1) return fail(HQ_ERR_NONFINITE, "non-finite input position");
+    if (flags & 4) return fail(HQ_ERR_CAPACITY, "more than 4096 particles share 21 levels");
+    t->nn = t->hstatus[ST_NNODES];
+    t->lost = t->hstatus[ST_LOST];
+    if (flags & 2)
+        return fail(HQ_ERR_CAPACITY, "INSUFFICIENT NUMBER OF NODES: %lld (needed %lld)",
+                    (long long)cap, (long long)t->nn);
+    t->most_depth = 0;
+    for (int d = 0; d <= NDEP; ++d) t->level_start[d] = (int)t->hstatus[ST_LEVEL0 + d];
+    for (int d = 0; d < NDEP; ++d)
+        if (t->level_start[d + 1] > t->level_start[d]) t->most_depth = d;
+    // 5. moments, deepest level first
+    for (int d = t->most_depth; d >= 0; --d) {
+        int64_t a = t->level_start[d], b = t->level_start[d + 1];
+        if (b <= a) continue;
+        k_upward<<<grid_for(b - a, 128), 128, 0, s>>>(
+            a, b, d, t->multipole, t->ntype.as<int8_t>(), t->nfirst.as<int>(),
+            t->ncstart.as<int>(), t->ndigit.as<uint8_t>(), t->sx.as<double>(), t->sy.as<double>(),
+            t->sz.as<double>(), t->sm.as<double>(), t->nmass.as<double>(), t->ncomx.as<double>(),
+            t->ncomy.as<double>(), t->ncomz.as<double>(),
+            t->multipole >= 4 ? t->nquad.as<double>() : nullptr,
+            t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), cap);
+    }
+    // 6. parent / next pointers
+    for (int d = 0; d < t->most_depth; ++d) {
+        int64_t a = t->level_start[d], b = t->level_start[d + 1];
+        if (b <= a) continue;
+        k_topdown<<<grid_for(b - a, 128), 128, 0, s>>>(a, b, t->ncstart.as<int>(),
+                                                       t->nnext.as<int>(), t->nparent.as<int>());
+    }
+    if (t->most_depth == 0) {  // a single node (N = 1): root has no parent and no next
+        int endv = END;
+        HQ_CHECK_HIP(hipMemcpyAsync(t->nnext.p, &endv, sizeof(int), hipMemcpyHostToDevice, s));
+        int none = -1;
+        HQ_CHECK_HIP(hipMemcpyAsync(t->nparent.p, &none, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    HQ_CHECK_HIP(hipEventRecord(t->ev[4], s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    HQ_CHECK_HIP(hipGetLastError());
+    hipEventElapsedTime(&t->ms_keys, t->ev[0], t->ev[1]);
+    hipEventElapsedTime(&t->ms_sort, t->ev[1], t->ev[2]);
+    hipEventElapsedTime(&t->ms_nodes, t->ev[2], t->ev[3]);
+    hipEventElapsedTime(&t->ms_upward, t->ev[3], t->ev[4]);
+    hipEventElapsedTime(&t->ms_build, t->ev[0], t->ev[4]);
+    t->built = true;
+    return HQ_OK;
+}
+
+}  // namespace hq

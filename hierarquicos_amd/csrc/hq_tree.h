@@ -1,0 +1,77 @@
+// hq_tree.h -- device-resident octree of the hierarquicos_amd library.
+//
+// The tree is the reference's insertion octree (fortran/octree_default.f90,
+// leaf size 1, max depth 25, root = bbox centre with side 1.2 * extent),
+// rebuilt data-parallel from sorted octant-path keys.  Nodes are stored in
+// level order: all depth-0 nodes, then depth 1, ...; within a depth in tree
+// (sorted key) order, so the children of a node are one contiguous range
+// [child_start[k], child_start[k+1]).
+#pragma once
+#include "hq_internal.h"
+
+struct hq_tree {
+    int64_t n = 0;
+    int method = 12, multipole = 8, group_size = 64, precision = HQ_FP64;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[12] = {};
+    bool have_masses = false, built = false;
+
+    // inputs, original order
+    hq::DevBuf m, x, y, z;
+    // keys and sort
+    hq::DevBuf khi, klo, idx, khi2, idx2, hist, geom, l2tab;
+    // sorted particles (tree order)
+    hq::DevBuf sx, sy, sz, sm, sklo, lcp;
+    // node store (level order)
+    int64_t nn = 0, cap = 0;
+    hq::DevBuf ndepth, ntype, nfirst, ncount, ncstart, nnext, nparent, ndigit;
+    hq::DevBuf nmass, ncomx, ncomy, ncomz, nquad, noct, ncx, ncy, ncz, nkhi, nklo;
+    hq::DevBuf mom32;  // float moment records for the mixed-precision walk
+    int level_start[hq::MAX_DEPTH + 2] = {};
+    int most_depth = 0;
+    int64_t lost = 0;
+    // groups (methods 30-32) and bunches
+    hq::DevBuf groups, gflag;
+    int64_t ngroups = 0;
+    // walk scratch
+    hq::DevBuf fx, fy, fz, counts, clist, plist, lstat;
+    int64_t list_ccap = 0, list_pcap = 0;
+    // status block: device + host mirror
+    hq::DevBuf status;
+    int64_t* hstatus = nullptr;
+    // last timings / diagnostics
+    float ms_keys = 0, ms_sort = 0, ms_nodes = 0, ms_upward = 0, ms_build = 0;
+    float ms_walk = 0, ms_lists = 0, ms_eval = 0;
+    int64_t list_cells = 0, list_parts = 0, n_p2p = 0, n_m2p = 0;
+};
+
+// status block slots (int64)
+enum {
+    ST_FLAGS = 0,      // bit0 non-finite input, bit1 capacity, bit2 tie run too long,
+                       // bit3 list overflow, bit4 stack overflow
+    ST_NNODES = 1,
+    ST_LOST = 2,
+    ST_LEVEL0 = 3,     // level_start[0..26] at ST_LEVEL0 + d
+    ST_LISTC = 32,     // cells in lists (sum)
+    ST_LISTP = 33,     // particles in lists (sum)
+    ST_NEEDC = 34,     // largest cell list needed
+    ST_NEEDP = 35,     // largest particle list needed
+    ST_P2P = 36,
+    ST_M2P = 37,
+    ST_NGROUPS = 38,
+    ST_SIZE = 48
+};
+
+namespace hq {
+int tree_build(hq_tree* t);
+int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2);
+int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int precision, bool bunched);
+int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2);
+int tree_scatter_forces(hq_tree* t, double* F, int layout, int where);
+int read_status(hq_tree* t);
+int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
+                     DevBuf& hist, hipStream_t s);
+int direct_forces(int64_t n, const double* m, const double* x, const double* y, const double* z,
+                  int64_t nt, const int64_t* targets, double G, double eps2, int precision,
+                  double* F, hipStream_t s);
+}  // namespace hq

@@ -1,0 +1,905 @@
+// hq_walk.hip -- force walks over the level-order octree.
+//
+//   k_bh_walk        methods 10/11/12: bh_forces_over_p (octree_default.f90
+//                    :895-1078) for one target per lane, FP64, stackless
+//                    (first child = child_start[k], skip = next[k]); same
+//                    acceptance test and expressions as the reference
+//   k_lists<MODE>    one wavefront per group: the group's interaction lists
+//                    (group_worker :1874-1952), 64 candidate nodes per step,
+//                    lists compacted by ballot, opened children pushed on an
+//                    LDS stack by a wavefront prefix sum.
+//                      MODE_GROUP: the reference's groups (maximal nodes with
+//                        <= group_size bodies), node-cube box, ancestor test
+//                      MODE_BUNCH / MODE_BUNCH32: bunches of <= 64 particles made
+//                        of adjacent sibling subtrees (so a bunch stays inside
+//                        one parent cube), with their tight bounding box; the
+//                        MODE_BUNCH32 lists hold FP32 records recentred on the
+//                        bunch centre (FP64 subtraction, then rounding)
+//   k_eval_fp64      group_worker's per-target sums (:1955-2090), FP64
+//   k_eval_mixed     the same sums in FP32 on recentred coordinates with a
+//                    lean (algebraically identical) multipole evaluation and
+//                    block-compensated (Kahan) accumulation
+//   k_scatter        sorted forces -> caller order and layout
+#include <math.h>
+
+#include <algorithm>
+
+#include "hq_tree.h"
+
+namespace hq {
+
+enum { G_CX = 0, G_CY, G_CZ, G_SIDE, G_HALF, G_L2 = 16, G_HALFD = 48 };
+
+struct NodeView {
+    const double *mass, *cx, *cy, *cz, *quad, *oct, *ncx, *ncy, *ncz;
+    const int *first, *count, *cstart, *next;
+    const int8_t* type;
+    const uint8_t* depth;
+    int64_t stride;  // SoA stride of quad/oct
+};
+
+static NodeView node_view(const hq_tree* t) {
+    NodeView v;
+    v.mass = t->nmass.as<double>();
+    v.cx = t->ncomx.as<double>();
+    v.cy = t->ncomy.as<double>();
+    v.cz = t->ncomz.as<double>();
+    v.quad = t->multipole >= 4 ? t->nquad.as<double>() : nullptr;
+    v.oct = t->multipole > 4 ? t->noct.as<double>() : nullptr;
+    v.ncx = t->ncx.as<double>();
+    v.ncy = t->ncy.as<double>();
+    v.ncz = t->ncz.as<double>();
+    v.first = t->nfirst.as<int>();
+    v.count = t->ncount.as<int>();
+    v.cstart = t->ncstart.as<int>();
+    v.next = t->nnext.as<int>();
+    v.type = t->ntype.as<int8_t>();
+    v.depth = t->ndepth.as<uint8_t>();
+    v.stride = t->cap;
+    return v;
+}
+
+// ------------------------------------------------------------------ BH walk
+__global__ __launch_bounds__(128) void k_bh_walk(NodeView nv, const double* __restrict__ g,
+                                                 const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ sz,
+                                                 const double* __restrict__ sm, int64_t n,
+                                                 int multipole, double theta2, double G,
+                                                 double eps2, double* __restrict__ fx,
+                                                 double* __restrict__ fy, double* __restrict__ fz,
+                                                 int64_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pm = sm[i], px = sx[i], py = sy[i], pz = sz[i];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    long long np2p = 0, nm2p = 0;
+    const int64_t S = nv.stride;
+    int k = 0;
+    while (k != END) {
+        const double mass = nv.mass[k];
+        const bool leaf = nv.type[k] == 1;
+        if (mass == 0.0 || (leaf && nv.first[k] == (int)i)) {
+            k = nv.next[k];
+            continue;
+        }
+        const double dx = nv.cx[k] - px;
+        const double dy = nv.cy[k] - py;
+        const double dz = nv.cz[k] - pz;
+        const double dist2 = dx * dx + dy * dy + dz * dz;
+        const int depth = nv.depth[k];
+        if (leaf || g[G_L2 + depth] <= theta2 * dist2 || depth == MAX_DEPTH) {
+            if (leaf || multipole == 1) {
+                double rinv = 1.0 / sqrt(dist2 + eps2);
+                rinv = rinv * rinv * rinv;
+                const double f = G * pm * mass * rinv;
+                f0 = f0 + f * dx;
+                f1 = f1 + f * dy;
+                f2 = f2 + f * dz;
+                ++np2p;
+            } else {
+                const double invR = 1.0 / sqrt(dist2 + eps2);
+                const double invR2 = 1.0 / (dist2 + eps2);
+                const double invR3 = invR * invR2;
+                const double invR5 = invR3 * invR * invR;
+                const double invR7 = invR5 * invR * invR;
+                const double invR9 = invR7 * invR * invR;
+                const double Mx = dx * invR3 * mass;
+                const double My = dy * invR3 * mass;
+                const double Mz = dz * invR3 * mass;
+                const double mxi2 = nv.quad[0 * S + k], myi2 = nv.quad[1 * S + k],
+                             mzi2 = nv.quad[2 * S + k], mxyi = nv.quad[3 * S + k],
+                             mxzi = nv.quad[4 * S + k], myzi = nv.quad[5 * S + k];
+                const double dx2 = 15.0 * dx * dx * invR2 - 3.0;
+                const double dy2 = 15.0 * dy * dy * invR2 - 3.0;
+                const double dz2 = 15.0 * dz * dz * invR2 - 3.0;
+                const double dxyz = 15.0 * dx * dy * dz * invR2;
+                double Qx = (mxi2 * (dx2 - 6.0) + myi2 * dy2 + mzi2 * dz2) * 0.5 * dx;
+                double Qy = (myi2 * (dy2 - 6.0) + mzi2 * dz2 + mxi2 * dx2) * 0.5 * dy;
+                double Qz = (mzi2 * (dz2 - 6.0) + mxi2 * dx2 + myi2 * dy2) * 0.5 * dz;
+                Qx = Qx + mxyi * dy * dx2 + mxzi * dz * dx2 + myzi * dxyz;
+                Qy = Qy + myzi * dz * dy2 + mxyi * dx * dy2 + mxzi * dxyz;
+                Qz = Qz + mxzi * dx * dz2 + myzi * dy * dz2 + mxyi * dxyz;
+                if (multipole == 4) {
+                    f0 = f0 + G * pm * (Mx + invR5 * Qx);
+                    f1 = f1 + G * pm * (My + invR5 * Qy);
+                    f2 = f2 + G * pm * (Mz + invR5 * Qz);
+                } else {
+                    double O[14];
+#pragma unroll
+                    for (int j = 0; j < 14; ++j) O[j] = nv.oct[j * S + k];
+                    double Ox = -(1.5 * invR5 * O[11]);
+                    double Oy = -(1.5 * invR5 * O[12]);
+                    double Oz = -(1.5 * invR5 * O[13]);
+                    double Oc = 7.5 * invR7 * (dx * O[11] + dy * O[12] + dz * O[13]);
+                    Ox = Ox + dx * Oc;
+                    Oy = Oy + dy * Oc;
+                    Oz = Oz + dz * Oc;
+                    const double ex2 = dx * dx, ey2 = dy * dy, ez2 = dz * dz;
+                    Ox = Ox + 7.5 * invR7 *
+                                  (ex2 * O[0] + ey2 * O[1] + ez2 * O[2] + 2.0 * dx * dy * O[3] +
+                                   2.0 * dx * dz * O[6] + 2.0 * dy * dz * O[9]);
+                    Oy = Oy + 7.5 * invR7 *
+                                  (ex2 * O[3] + ey2 * O[4] + ez2 * O[5] + 2.0 * dx * dy * O[1] +
+                                   2.0 * dx * dz * O[9] + 2.0 * dy * dz * O[7]);
+                    Oz = Oz + 7.5 * invR7 *
+                                  (ex2 * O[6] + ey2 * O[7] + ez2 * O[8] + 2.0 * dx * dy * O[9] +
+                                   2.0 * dx * dz * O[2] + 2.0 * dy * dz * O[5]);
+                    Oc = -(17.5 * invR9 *
+                           (ex2 * dx * O[0] + ey2 * dy * O[4] + ez2 * dz * O[8] +
+                            3.0 * ex2 * dy * O[3] + 3.0 * ex2 * dz * O[6] + 3.0 * ey2 * dx * O[1] +
+                            3.0 * ey2 * dz * O[7] + 3.0 * ez2 * dx * O[2] + 3.0 * ez2 * dy * O[5] +
+                            6.0 * dx * dy * dz * O[9]));
+                    Ox = Ox + dx * Oc;
+                    Oy = Oy + dy * Oc;
+                    Oz = Oz + dz * Oc;
+                    f0 = f0 + G * pm * (Mx + invR5 * Qx + Ox);
+                    f1 = f1 + G * pm * (My + invR5 * Qy + Oy);
+                    f2 = f2 + G * pm * (Mz + invR5 * Qz + Oz);
+                }
+                ++nm2p;
+            }
+            k = nv.next[k];
+        } else {
+            k = nv.cstart[k];
+        }
+    }
+    fx[i] = f0;
+    fy[i] = f1;
+    fz[i] = f2;
+    long long a = wave_sum(np2p), b = wave_sum(nm2p);
+    if (__lane_id() == 0) {
+        atomicAdd((unsigned long long*)&status[ST_P2P], (unsigned long long)a);
+        atomicAdd((unsigned long long*)&status[ST_M2P], (unsigned long long)b);
+    }
+}
+
+// ------------------------------------------------------------------ lists
+enum { MODE_GROUP = 0, MODE_BUNCH = 1, MODE_BUNCH32 = 2 };
+constexpr int SCAP = 4096;   // LDS walk stack (ints) per wavefront
+constexpr int BUNCH = 64;
+constexpr double LIST_GIB_MAX = 48.0;  // device memory allowed for interaction lists
+
+struct ListOut {
+    int* cidx;      // MODE_GROUP / MODE_BUNCH: node indices
+    int* pidx;      // particle indices
+    float4* c32;    // MODE_BUNCH32: {rc.x, rc.y, rc.z, node bits}
+    float4* p32;    // {r.x, r.y, r.z, m}
+    int64_t ccap, pcap;
+    int* lstat;     // [2*g] = cells, [2*g+1] = particles
+};
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restrict__ g,
+                                              const double* __restrict__ sx,
+                                              const double* __restrict__ sy,
+                                              const double* __restrict__ sz,
+                                              const double* __restrict__ sm, int64_t n,
+                                              const int* __restrict__ groups, int ngroups,
+                                              double theta2, ListOut out,
+                                              int64_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    __shared__ int stk[SCAP];
+    const int gi = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (gi >= ngroups) return;
+    const int lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    int gf, gl, gnode = -1;
+    double bmin[3], bmax[3];
+    if (MODE == MODE_GROUP) {
+        gnode = groups[gi];
+        gf = nv.first[gnode];
+        gl = gf + nv.count[gnode] - 1;
+        const double h = g[G_HALFD + nv.depth[gnode]];
+        bmin[0] = nv.ncx[gnode] - h; bmax[0] = nv.ncx[gnode] + h;
+        bmin[1] = nv.ncy[gnode] - h; bmax[1] = nv.ncy[gnode] + h;
+        bmin[2] = nv.ncz[gnode] - h; bmax[2] = nv.ncz[gnode] + h;
+    } else {
+        gf = groups[gi];
+        gl = groups[gi + 1] - 1;
+        int p = gf + lane;
+        bool ok = p <= gl;
+        double v[3] = {ok ? sx[p] : INFINITY, ok ? sy[p] : INFINITY, ok ? sz[p] : INFINITY};
+        double w[3] = {ok ? sx[p] : -INFINITY, ok ? sy[p] : -INFINITY, ok ? sz[p] : -INFINITY};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                v[c] = fmin(v[c], __shfl_xor(v[c], o, 64));
+                w[c] = fmax(w[c], __shfl_xor(w[c], o, 64));
+            }
+            bmin[c] = v[c];
+            bmax[c] = w[c];
+        }
+    }
+    const double bc[3] = {0.5 * (bmin[0] + bmax[0]), 0.5 * (bmin[1] + bmax[1]),
+                          0.5 * (bmin[2] + bmax[2])};
+    int top = 1, nc = 0, np = 0;
+    bool overflow = false;
+    if (lane == 0) stk[0] = 0;
+    __builtin_amdgcn_wave_barrier();
+    while (top > 0) {
+        const int take = top < 64 ? top : 64;
+        const int base = top - take;
+        const int k = lane < take ? stk[base + lane] : -1;
+        top = base;
+        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell
+        if (k >= 0) {
+            const double mass = nv.mass[k];
+            if (k != gnode && mass != 0.0) {
+                const int f = nv.first[k];
+                const int l = f + nv.count[k] - 1;
+                const bool leaf = nv.type[k] == 1;
+                bool anc;
+                if (MODE == MODE_GROUP) anc = f <= gf && l >= gl;
+                else anc = f <= gl && l >= gf;
+                if (anc) {
+                    action = leaf ? 0 : 1;  // a bunch's own leaves are its in-bunch sum
+                } else if (leaf) {
+                    action = 2;
+                } else {
+                    const double cc[3] = {nv.cx[k], nv.cy[k], nv.cz[k]};
+                    double dist2 = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double dd = fmax(fmax(bmin[c] - cc[c], 0.0), cc[c] - bmax[c]);
+                        dist2 = dist2 + dd * dd;
+                    }
+                    const int depth = nv.depth[k];
+                    action = (g[G_L2 + depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                }
+            }
+        }
+        const uint64_t mp = __ballot(action == 2);
+        if (action == 2) {
+            const int64_t pos = np + __popcll(mp & lt);
+            if (pos < out.pcap) {
+                const int p = nv.first[k];
+                const int64_t o = (int64_t)gi * out.pcap + pos;
+                if (MODE == MODE_BUNCH32)
+                    out.p32[o] = make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]),
+                                             (float)(sz[p] - bc[2]), (float)sm[p]);
+                else
+                    out.pidx[o] = p;
+            }
+        }
+        np += __popcll(mp);
+        const uint64_t mc = __ballot(action == 3);
+        if (action == 3) {
+            const int64_t pos = nc + __popcll(mc & lt);
+            if (pos < out.ccap) {
+                const int64_t o = (int64_t)gi * out.ccap + pos;
+                if (MODE == MODE_BUNCH32)
+                    out.c32[o] = make_float4((float)(nv.cx[k] - bc[0]), (float)(nv.cy[k] - bc[1]),
+                                             (float)(nv.cz[k] - bc[2]), __int_as_float(k));
+                else
+                    out.cidx[o] = k;
+            }
+        }
+        nc += __popcll(mc);
+        const int cs = action == 1 ? nv.cstart[k] : 0;
+        const int nch = action == 1 ? nv.cstart[k + 1] - cs : 0;
+        const int incl = wave_incl_scan(nch);
+        const int tot = __shfl(incl, 63, 64);
+        if (top + tot > SCAP) {
+            overflow = true;
+            break;
+        }
+        const int off = top + incl - nch;
+        for (int c = 0; c < nch; ++c) stk[off + c] = cs + c;
+        top += tot;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) {
+        out.lstat[2 * gi] = nc;
+        out.lstat[2 * gi + 1] = np;
+        if (overflow) atomicOr((unsigned long long*)&status[ST_FLAGS], 16ull);
+        if (nc > out.ccap || np > out.pcap) atomicOr((unsigned long long*)&status[ST_FLAGS], 8ull);
+        atomicAdd((unsigned long long*)&status[ST_LISTC], (unsigned long long)nc);
+        atomicAdd((unsigned long long*)&status[ST_LISTP], (unsigned long long)np);
+        atomicMax((long long*)&status[ST_NEEDC], (long long)nc);
+        atomicMax((long long*)&status[ST_NEEDP], (long long)np);
+    }
+}
+
+// ------------------------------------------------------------------ FP64 evaluation
+// group_worker's per-target loops (:1955-2090) with the reference's
+// expressions; one wavefront per group, one target per lane.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_eval_fp64(NodeView nv, const double* __restrict__ sx,
+                                                  const double* __restrict__ sy,
+                                                  const double* __restrict__ sz,
+                                                  const double* __restrict__ sm, int64_t n,
+                                                  const int* __restrict__ groups, int ngroups,
+                                                  int multipole, double G, double eps2,
+                                                  ListOut lists, double* __restrict__ fx,
+                                                  double* __restrict__ fy,
+                                                  double* __restrict__ fz) {
+#pragma clang fp contract(off)
+    const int gi = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (gi >= ngroups) return;
+    int gf, gl;
+    if (MODE == MODE_GROUP) {
+        const int gn = groups[gi];
+        gf = nv.first[gn];
+        gl = gf + nv.count[gn] - 1;
+    } else {
+        gf = groups[gi];
+        gl = groups[gi + 1] - 1;
+    }
+    const int nc = lists.lstat[2 * gi], np = lists.lstat[2 * gi + 1];
+    const int* __restrict__ cl = lists.cidx + (int64_t)gi * lists.ccap;
+    const int* __restrict__ pl = lists.pidx + (int64_t)gi * lists.pcap;
+    const int64_t S = nv.stride;
+    for (int i = gf + (int)__lane_id(); i <= gl; i += 64) {
+        const double xi = sx[i], yi = sy[i], zi = sz[i];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j = gf; j <= gl; ++j) {
+            if (j == i) continue;
+            const double dx = sx[j] - xi, dy = sy[j] - yi, dz = sz[j] - zi;
+            const double dist2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double invr = 1.0 / sqrt(dist2);
+            const double invr3 = invr * invr * invr;
+            const double f = sm[j] * invr3;
+            ax = ax + f * dx;
+            ay = ay + f * dy;
+            az = az + f * dz;
+        }
+        for (int jj = 0; jj < np; ++jj) {
+            const int j = pl[jj];
+            const double dx = sx[j] - xi, dy = sy[j] - yi, dz = sz[j] - zi;
+            const double dist2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double invr = 1.0 / sqrt(dist2);
+            const double invr3 = invr * invr * invr;
+            const double f = sm[j] * invr3;
+            ax = ax + f * dx;
+            ay = ay + f * dy;
+            az = az + f * dz;
+        }
+        for (int jj = 0; jj < nc; ++jj) {
+            const int k = cl[jj];
+            const double dx = nv.cx[k] - xi, dy = nv.cy[k] - yi, dz = nv.cz[k] - zi;
+            const double dist2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double invr = 1.0 / sqrt(dist2);
+            const double invr3 = invr * invr * invr;
+            const double M = nv.mass[k];
+            if (multipole == 1) {
+                const double f = M * invr3;
+                ax = ax + f * dx;
+                ay = ay + f * dy;
+                az = az + f * dz;
+                continue;
+            }
+            const double invr5 = invr3 * invr * invr;
+            const double invr7 = invr5 * invr * invr;
+            const double qxx = nv.quad[0 * S + k], qyy = nv.quad[1 * S + k],
+                         qzz = nv.quad[2 * S + k], qxy = nv.quad[3 * S + k],
+                         qxz = nv.quad[4 * S + k], qyz = nv.quad[5 * S + k];
+            const double qx = qxx * dx + qxy * dy + qxz * dz;
+            const double qy = qxy * dx + qyy * dy + qyz * dz;
+            const double qz = qxz * dx + qyz * dy + qzz * dz;
+            const double dqd = dx * qx + dy * qy + dz * qz;
+            const double trq = qxx + qyy + qzz;
+            const double s = M * invr3 - 1.5 * trq * invr5 + 7.5 * dqd * invr7;
+            ax = ax + s * dx - 3.0 * qx * invr5;
+            ay = ay + s * dy - 3.0 * qy * invr5;
+            az = az + s * dz - 3.0 * qz * invr5;
+            if (multipole == 8) {
+                double c[14];
+#pragma unroll
+                for (int j = 0; j < 14; ++j) c[j] = nv.oct[j * S + k];
+                const double invr9 = invr7 * invr * invr;
+                double oc = 7.5 * invr7 * (dx * c[11] + dy * c[12] + dz * c[13]);
+                double ox = -(1.5 * invr5 * c[11]) + dx * oc;
+                double oy = -(1.5 * invr5 * c[12]) + dy * oc;
+                double oz = -(1.5 * invr5 * c[13]) + dz * oc;
+                ox = ox + 7.5 * invr7 * (dx * dx * c[0] + dy * dy * c[1] + dz * dz * c[2]);
+                ox = ox + 15.0 * invr7 * (dx * dy * c[3] + dx * dz * c[6] + dy * dz * c[9]);
+                oy = oy + 7.5 * invr7 * (dx * dx * c[3] + dy * dy * c[4] + dz * dz * c[5]);
+                oy = oy + 15.0 * invr7 * (dx * dy * c[1] + dx * dz * c[9] + dy * dz * c[7]);
+                oz = oz + 7.5 * invr7 * (dx * dx * c[6] + dy * dy * c[7] + dz * dz * c[8]);
+                oz = oz + 15.0 * invr7 * (dx * dy * c[9] + dx * dz * c[2] + dy * dz * c[5]);
+                oc = dx * dx * dx * c[0] + dy * dy * dy * c[4] + dz * dz * dz * c[8];
+                oc = oc + 3.0 * (dx * dx * dy * c[3] + dx * dx * dz * c[6] + dx * dy * dy * c[1]);
+                oc = oc + 3.0 * (dy * dy * dz * c[7] + dx * dz * dz * c[2] + dy * dz * dz * c[5]);
+                oc = oc + 6.0 * dx * dy * dz * c[9];
+                oc = -(17.5 * invr9 * oc);
+                ox = ox + dx * oc;
+                oy = oy + dy * oc;
+                oz = oz + dz * oc;
+                ax = ax + ox;
+                ay = ay + oy;
+                az = az + oz;
+            }
+        }
+        fx[i] = G * sm[i] * ax;
+        fy[i] = G * sm[i] * ay;
+        fz[i] = G * sm[i] * az;
+    }
+}
+
+// ------------------------------------------------------------------ mixed evaluation
+// Node moments as FP32 records (24 floats): M, Qxx,Qyy,Qzz,Qxy,Qxz,Qyz,
+// -1.5*trQ, T' = -1.5*(Sum m d r^2) (3), O (10, octree_default.f90 :831-843
+// order), pad.  The oct M2P of :2032-2082 is rewritten with
+// V = O:dd (V_x = dx^2 O1 + dy^2 O2 + dz^2 O3 + 2 dx dy O4 + 2 dx dz O7 +
+// 2 dy dz O10, cyclic) and d.V = the 17.5-term bracket, which is the same
+// polynomial grouped for fewer operations.
+constexpr int REC = 24;
+
+__global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeView nv,
+                                               float* __restrict__ rec) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int64_t S = nv.stride;
+    float r[REC];
+#pragma unroll
+    for (int j = 0; j < REC; ++j) r[j] = 0.0f;
+    r[0] = (float)nv.mass[k];
+    if (multipole >= 4) {
+        double tr = 0.0;
+        for (int j = 0; j < 6; ++j) r[1 + j] = (float)nv.quad[j * S + k];
+        tr = nv.quad[0 * S + k] + nv.quad[1 * S + k] + nv.quad[2 * S + k];
+        r[7] = (float)(-1.5 * tr);
+    }
+    if (multipole > 4) {
+        for (int j = 0; j < 3; ++j) r[8 + j] = (float)(-1.5 * nv.oct[(11 + j) * S + k]);
+        for (int j = 0; j < 10; ++j) r[11 + j] = (float)nv.oct[j * S + k];
+    }
+    float4* o = reinterpret_cast<float4*>(rec + k * REC);
+#pragma unroll
+    for (int j = 0; j < REC / 4; ++j) o[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+}
+
+struct Kahan3 {
+    float s[3] = {0.f, 0.f, 0.f}, c[3] = {0.f, 0.f, 0.f};
+    __device__ __forceinline__ void add(const float p[3]) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float y = p[j] - c[j];
+            float t = s[j] + y;
+            c[j] = (t - s[j]) - y;
+            s[j] = t;
+        }
+    }
+};
+
+template <int MP>
+__device__ __forceinline__ void m2p32(const float4 e, const float* __restrict__ r, float tx,
+                                      float ty, float tz, float eps2, float acc[3]) {
+    const float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+    const float s2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    const float ri = __frsqrt_rn(s2);
+    const float ri2 = ri * ri;
+    const float r3 = ri * ri2;
+    float cd = r[0] * r3;  // M / s^3
+    if (MP == 1) {
+        acc[0] = fmaf(cd, dx, acc[0]);
+        acc[1] = fmaf(cd, dy, acc[1]);
+        acc[2] = fmaf(cd, dz, acc[2]);
+        return;
+    }
+    const float r5 = r3 * ri2;
+    const float r7 = r5 * ri2;
+    const float qx = fmaf(r[1], dx, fmaf(r[4], dy, r[5] * dz));
+    const float qy = fmaf(r[4], dx, fmaf(r[2], dy, r[6] * dz));
+    const float qz = fmaf(r[5], dx, fmaf(r[6], dy, r[3] * dz));
+    float dq = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+    cd = fmaf(r[7], r5, cd);  // - 1.5 trQ / s^5
+    float wx = -3.0f * qx, wy = -3.0f * qy, wz = -3.0f * qz;
+    const float c7 = 7.5f * r7;
+    if (MP == 8) {
+        const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+        const float dx2 = dx + dx;
+        const float xy = dx2 * dy, xz = dx2 * dz, yz = (dy + dy) * dz;
+        // V = O:dd with O1..O10 at r[11..20]
+        const float vx = fmaf(xx, r[11], fmaf(yy, r[12], fmaf(zz, r[13], fmaf(xy, r[14], fmaf(xz, r[17], yz * r[20])))));
+        const float vy = fmaf(xx, r[14], fmaf(yy, r[15], fmaf(zz, r[16], fmaf(xy, r[12], fmaf(xz, r[20], yz * r[18])))));
+        const float vz = fmaf(xx, r[17], fmaf(yy, r[18], fmaf(zz, r[19], fmaf(xy, r[20], fmaf(xz, r[13], yz * r[16])))));
+        const float dv = fmaf(dx, vx, fmaf(dy, vy, dz * vz));
+        // T' = -1.5 T at r[8..10]:  d.T = -(d.T')/1.5
+        const float dT = fmaf(dx, r[8], fmaf(dy, r[9], dz * r[10])) * (-2.0f / 3.0f);
+        dq = dq + dT;
+        cd = fmaf(-17.5f * r7 * ri2, dv, cd);
+        wx += r[8];
+        wy += r[9];
+        wz += r[10];
+        acc[0] = fmaf(c7, vx, acc[0]);
+        acc[1] = fmaf(c7, vy, acc[1]);
+        acc[2] = fmaf(c7, vz, acc[2]);
+    }
+    cd = fmaf(c7, dq, cd);
+    acc[0] = fmaf(cd, dx, fmaf(r5, wx, acc[0]));
+    acc[1] = fmaf(cd, dy, fmaf(r5, wy, acc[1]));
+    acc[2] = fmaf(cd, dz, fmaf(r5, wz, acc[2]));
+}
+
+__device__ __forceinline__ void p2p32(const float4 e, float tx, float ty, float tz, float eps2,
+                                      float acc[3]) {
+    const float dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+    const float s2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    const float ri = __frsqrt_rn(s2);
+    const float f = e.w * ri * ri * ri;
+    acc[0] = fmaf(f, dx, acc[0]);
+    acc[1] = fmaf(f, dy, acc[1]);
+    acc[2] = fmaf(f, dz, acc[2]);
+}
+
+constexpr int KBLK = 32;  // interactions per compensated partial sum
+
+template <int MP>
+__global__ __launch_bounds__(64) void k_eval_mixed(const double* __restrict__ sx,
+                                                   const double* __restrict__ sy,
+                                                   const double* __restrict__ sz,
+                                                   const double* __restrict__ sm, int64_t n,
+                                                   const int* __restrict__ bstart,
+                                                   int ngroups, double G, float eps2,
+                                                   const float4* __restrict__ c32,
+                                                   const float4* __restrict__ p32,
+                                                   const int* __restrict__ lstat, int64_t ccap,
+                                                   int64_t pcap, const float* __restrict__ mom,
+                                                   double* __restrict__ fx,
+                                                   double* __restrict__ fy,
+                                                   double* __restrict__ fz) {
+    __shared__ float4 own[BUNCH];
+    const int gi = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (gi >= ngroups) return;
+    const int lane = __lane_id();
+    const int gf = bstart[gi];
+    const int gl = bstart[gi + 1] - 1;
+    const int nb = gl - gf + 1;
+    // bunch centre, as the list builder computed it
+    const int i = gf + (lane < nb ? lane : 0);
+    double v[3] = {sx[i], sy[i], sz[i]}, w[3] = {v[0], v[1], v[2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            v[c] = fmin(v[c], __shfl_xor(v[c], o, 64));
+            w[c] = fmax(w[c], __shfl_xor(w[c], o, 64));
+        }
+    double bc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) bc[c] = 0.5 * (v[c] + w[c]);
+    const float tx = (float)(sx[i] - bc[0]), ty = (float)(sy[i] - bc[1]), tz = (float)(sz[i] - bc[2]);
+    own[lane] = make_float4(tx, ty, tz, lane < nb ? (float)sm[i] : 0.0f);
+    __syncthreads();
+    Kahan3 sum;
+    float acc[3] = {0.f, 0.f, 0.f};
+    // in-bunch particles (self skipped)
+    for (int j = 0; j < nb; ++j) {
+        float4 e = own[j];
+        if (j == lane) e.w = 0.0f, e.x = tx + 1.0f;  // zero mass, non-zero distance
+        p2p32(e, tx, ty, tz, eps2, acc);
+    }
+    sum.add(acc);
+    const int nc = lstat[2 * gi], np = lstat[2 * gi + 1];
+    const float4* __restrict__ pl = p32 + (int64_t)gi * pcap;
+    for (int j0 = 0; j0 < np; j0 += KBLK) {
+        acc[0] = acc[1] = acc[2] = 0.f;
+        const int j1 = std::min(np, j0 + KBLK);
+        for (int j = j0; j < j1; ++j) p2p32(pl[j], tx, ty, tz, eps2, acc);
+        sum.add(acc);
+    }
+    const float4* __restrict__ cl = c32 + (int64_t)gi * ccap;
+    for (int j0 = 0; j0 < nc; j0 += KBLK) {
+        acc[0] = acc[1] = acc[2] = 0.f;
+        const int j1 = std::min(nc, j0 + KBLK);
+        for (int j = j0; j < j1; ++j) {
+            const float4 e = cl[j];
+            const float* __restrict__ r = mom + (int64_t)__float_as_int(e.w) * REC;
+            m2p32<MP>(e, r, tx, ty, tz, eps2, acc);
+        }
+        sum.add(acc);
+    }
+    if (lane < nb) {
+        const double gm = G * sm[i];
+        fx[i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
+        fy[i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
+        fz[i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
+    }
+}
+
+// ------------------------------------------------------------------ scatter
+__global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ idx,
+                                                 const double* __restrict__ fx,
+                                                 const double* __restrict__ fy,
+                                                 const double* __restrict__ fz, int64_t n,
+                                                 int layout, double* __restrict__ F) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t p = idx[i];
+    if (layout == HQ_LAYOUT_3N) {
+        F[3 * p] = fx[i];
+        F[3 * p + 1] = fy[i];
+        F[3 * p + 2] = fz[i];
+    } else {
+        F[p] = fx[i];
+        F[n + p] = fy[i];
+        F[2 * n + p] = fz[i];
+    }
+}
+
+// lost particles (no leaf) still get their walk result, as in the reference;
+// nothing else to do here.
+
+// ------------------------------------------------------------------ drivers
+static int ensure_forces(hq_tree* t) {
+    int rc;
+    size_t b = t->n * sizeof(double);
+    if ((rc = t->fx.ensure(b)) || (rc = t->fy.ensure(b)) || (rc = t->fz.ensure(b))) return rc;
+    return HQ_OK;
+}
+
+int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2) {
+    int rc;
+    if ((rc = ensure_forces(t))) return rc;
+    hipStream_t s = t->stream;
+    HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+    HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
+    k_bh_walk<<<grid_for(t->n, 128), 128, 0, s>>>(
+        node_view(t), t->geom.as<double>(), t->sx.as<double>(), t->sy.as<double>(),
+        t->sz.as<double>(), t->sm.as<double>(), t->n, t->multipole, theta2, G, eps2,
+        t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>(), t->status.as<int64_t>());
+    HQ_CHECK_HIP(hipEventRecord(t->ev[6], s));
+    if ((rc = read_status(t))) return rc;
+    HQ_CHECK_HIP(hipGetLastError());
+    hipEventElapsedTime(&t->ms_walk, t->ev[5], t->ev[6]);
+    t->n_p2p = t->hstatus[ST_P2P];
+    t->n_m2p = t->hstatus[ST_M2P];
+    return HQ_OK;
+}
+
+// groups (methods 30-32): maximal nodes with count <= group_size, or leaves,
+// found in tree order (prepare_groups :1789-1808)
+__global__ __launch_bounds__(256) void k_mark_groups(NodeView nv, const int* __restrict__ nparent,
+                                                     int64_t nn, int gs, int* __restrict__ gmark) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int cnt = nv.count[k];
+    if (cnt == 0) return;
+    if (!(cnt <= gs || nv.type[k] == 1)) return;
+    if (k != 0) {
+        const int p = nparent[k];
+        if (nv.count[p] <= gs) return;  // the parent is (inside) a group already
+    }
+    gmark[nv.first[k]] = (int)k;
+}
+
+// bunches: in every twig with more than `gs` bodies, runs of adjacent children
+// that hold <= gs bodies each are packed greedily into bunches of <= gs bodies;
+// a tree with <= gs bodies is one bunch.  Marks each bunch's first particle.
+__global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, int gs,
+                                                      int* __restrict__ bmark) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int cnt = nv.count[k];
+    if (k == 0 && cnt <= gs) {
+        bmark[nv.first[0]] = nv.first[0];
+        return;
+    }
+    if (nv.type[k] != 2 || cnt <= gs) return;
+    int run = 0;
+    bool prev = false;
+    for (int c = nv.cstart[k]; c < nv.cstart[k + 1]; ++c) {
+        const int cc = nv.count[c];
+        if (cc == 0 || cc > gs) {
+            prev = false;
+            continue;
+        }
+        if (prev && run + cc <= gs) {
+            run += cc;
+        } else {
+            bmark[nv.first[c]] = nv.first[c];
+            run = cc;
+        }
+        prev = true;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_flag_count(const int* __restrict__ gmark, int64_t n,
+                                                    uint32_t* __restrict__ bcnt) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    uint64_t m = __ballot(i < n && gmark[i] >= 0);
+    __shared__ uint32_t wc[4];
+    if (__lane_id() == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(256) void k_flag_write(const int* __restrict__ gmark, int64_t n,
+                                                    const uint32_t* __restrict__ boff,
+                                                    int* __restrict__ out) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool f = i < n && gmark[i] >= 0;
+    uint64_t m = __ballot(f);
+    __shared__ uint32_t wc[4];
+    int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) wc[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t pre = boff[blockIdx.x];
+    for (int k = 0; k < w; ++k) pre += wc[k];
+    if (f) out[pre + __popcll(m & lanemask_lt())] = gmark[i];
+}
+
+int tree_make_groups(hq_tree* t, bool bunched);
+
+int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int precision,
+                       bool bunched) {
+    int rc;
+    if ((rc = ensure_forces(t))) return rc;
+    hipStream_t s = t->stream;
+    int ngroups;
+    if ((rc = tree_make_groups(t, bunched))) return rc;
+    ngroups = (int)t->ngroups;
+    const bool mixed = bunched && precision == HQ_MIXED;
+    if (mixed) {
+        if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
+        k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
+                                                     t->mom32.as<float>());
+    }
+    if ((rc = t->lstat.ensure((size_t)ngroups * 2 * sizeof(int)))) return rc;
+    if (t->list_ccap == 0) t->list_ccap = 4096;
+    if (t->list_pcap == 0) t->list_pcap = 1024;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        const size_t esz = mixed ? sizeof(float4) : sizeof(int);
+        if ((rc = t->clist.ensure((size_t)ngroups * t->list_ccap * esz)) ||
+            (rc = t->plist.ensure((size_t)ngroups * t->list_pcap * esz)))
+            return rc;
+        ListOut lo;
+        lo.cidx = t->clist.as<int>();
+        lo.pidx = t->plist.as<int>();
+        lo.c32 = t->clist.as<float4>();
+        lo.p32 = t->plist.as<float4>();
+        lo.ccap = t->list_ccap;
+        lo.pcap = t->list_pcap;
+        lo.lstat = t->lstat.as<int>();
+        HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+        HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
+        const NodeView nv = node_view(t);
+        if (!bunched)
+            k_lists<MODE_GROUP><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(), t->sx.as<double>(),
+                                                       t->sy.as<double>(), t->sz.as<double>(),
+                                                       t->sm.as<double>(), t->n,
+                                                       t->groups.as<int>(), ngroups, theta2, lo,
+                                                       t->status.as<int64_t>());
+        else if (!mixed)
+            k_lists<MODE_BUNCH><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(), t->sx.as<double>(),
+                                                       t->sy.as<double>(), t->sz.as<double>(),
+                                                       t->sm.as<double>(), t->n, t->groups.as<int>(),
+                                                       ngroups, theta2, lo, t->status.as<int64_t>());
+        else
+            k_lists<MODE_BUNCH32><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(),
+                                                         t->sx.as<double>(), t->sy.as<double>(),
+                                                         t->sz.as<double>(), t->sm.as<double>(),
+                                                         t->n, t->groups.as<int>(), ngroups, theta2,
+                                                         lo, t->status.as<int64_t>());
+        HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
+        if ((rc = read_status(t))) return rc;
+        const int64_t flags = t->hstatus[ST_FLAGS];
+        if (flags & 16) return fail(HQ_ERR_OVERFLOW, "walk stack overflow (more than %d pending nodes)", SCAP);
+        if (flags & 8) {
+            int64_t needc = t->hstatus[ST_NEEDC], needp = t->hstatus[ST_NEEDP];
+            t->list_ccap = std::max<int64_t>(t->list_ccap, (needc + 1023) / 1024 * 1024);
+            t->list_pcap = std::max<int64_t>(t->list_pcap, (needp + 255) / 256 * 256);
+            const double gib = (double)ngroups * (t->list_ccap + t->list_pcap) * esz / (1 << 30);
+            if (gib > LIST_GIB_MAX)
+                return fail(HQ_ERR_OVERFLOW, "interaction lists would need %.1f GiB (largest: %lld "
+                            "cells, %lld particles)", gib, (long long)needc, (long long)needp);
+            continue;  // lists were truncated: rebuild with room for the largest
+        }
+        t->list_cells = t->hstatus[ST_LISTC];
+        t->list_parts = t->hstatus[ST_LISTP];
+        if (!bunched)
+            k_eval_fp64<MODE_GROUP><<<ngroups, 64, 0, s>>>(
+                nv, t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(),
+                t->n, t->groups.as<int>(), ngroups, t->multipole, G, eps2, lo, t->fx.as<double>(),
+                t->fy.as<double>(), t->fz.as<double>());
+        else if (!mixed)
+            k_eval_fp64<MODE_BUNCH><<<ngroups, 64, 0, s>>>(
+                nv, t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(),
+                t->n, t->groups.as<int>(), ngroups, t->multipole, G, eps2, lo, t->fx.as<double>(),
+                t->fy.as<double>(), t->fz.as<double>());
+        else {
+            auto launch = [&](auto kern) {
+                kern<<<ngroups, 64, 0, s>>>(t->sx.as<double>(), t->sy.as<double>(),
+                                            t->sz.as<double>(), t->sm.as<double>(), t->n,
+                                            t->groups.as<int>(), ngroups,
+                                            G, (float)eps2, lo.c32, lo.p32, lo.lstat, lo.ccap,
+                                            lo.pcap, t->mom32.as<float>(), t->fx.as<double>(),
+                                            t->fy.as<double>(), t->fz.as<double>());
+            };
+            if (t->multipole == 1) launch(k_eval_mixed<1>);
+            else if (t->multipole == 4) launch(k_eval_mixed<4>);
+            else launch(k_eval_mixed<8>);
+        }
+        HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+        HQ_CHECK_HIP(hipGetLastError());
+        hipEventElapsedTime(&t->ms_lists, t->ev[5], t->ev[7]);
+        hipEventElapsedTime(&t->ms_eval, t->ev[7], t->ev[8]);
+        t->ms_walk = t->ms_lists + t->ms_eval;
+        return HQ_OK;
+    }
+    return fail(HQ_ERR_OVERFLOW, "interaction lists kept overflowing");
+}
+
+// groups (methods 30-32) or bunches, as a table of node indices (groups) or
+// of first particles followed by n (bunches), in tree order
+int tree_make_groups(hq_tree* t, bool bunched) {
+    int rc;
+    const int64_t n = t->n;
+    hipStream_t s = t->stream;
+    if (t->lost > 0)
+        return fail(HQ_ERR_LOST, "[prepare_groups] body lost: %lld particles at max depth",
+                    (long long)t->lost);
+    if ((rc = t->gflag.ensure(n * sizeof(int))) || (rc = t->groups.ensure((n + 1) * sizeof(int))))
+        return rc;
+    HQ_CHECK_HIP(hipMemsetAsync(t->gflag.p, 0xFF, n * sizeof(int), s));
+    if (bunched)
+        k_mark_bunches<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nn,
+                                                           std::min(t->group_size, BUNCH),
+                                                           t->gflag.as<int>());
+    else
+        k_mark_groups<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nparent.as<int>(),
+                                                           t->nn, t->group_size, t->gflag.as<int>());
+    const int nb = grid_for(n, 256);
+    if ((rc = t->hist.ensure(((size_t)nb + 1) * sizeof(uint32_t)))) return rc;
+    k_flag_count<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>());
+    k_scan_excl<<<1, 1024, 0, s>>>(t->hist.as<uint32_t>(), nb);
+    k_flag_write<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>(),
+                                    t->groups.as<int>());
+    uint32_t total = 0;
+    HQ_CHECK_HIP(hipMemcpyAsync(&total, t->hist.as<uint32_t>() + nb, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    t->ngroups = total;
+    if (bunched) {
+        const int end = (int)n;
+        HQ_CHECK_HIP(hipMemcpyAsync(t->groups.as<int>() + total, &end, sizeof(int),
+                                    hipMemcpyHostToDevice, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    return HQ_OK;
+}
+
+int tree_scatter_forces(hq_tree* t, double* F, int layout, int where) {
+    hipStream_t s = t->stream;
+    int rc;
+    double* dF = F;
+    if (where == HQ_HOST) {
+        if ((rc = t->counts.ensure(3 * t->n * sizeof(double)))) return rc;
+        dF = t->counts.as<double>();
+    }
+    k_scatter<<<grid_for(t->n, 256), 256, 0, s>>>(t->idx.as<int>(), t->fx.as<double>(),
+                                                  t->fy.as<double>(), t->fz.as<double>(), t->n,
+                                                  layout, dF);
+    if (where == HQ_HOST)
+        HQ_CHECK_HIP(hipMemcpyAsync(F, dF, 3 * t->n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    HQ_CHECK_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+}  // namespace hq

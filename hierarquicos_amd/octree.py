@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference's OctreeType (fortran/octree_default.f90).
+
+    tree = OctreeType()
+    tree.pre_init(m, method=12)              # pre_init (:178-282)
+    tree.init(x, y, z)                       # init (:320-387): GPU build
+    F = tree.forces_all(theta2, G, eps2)     # bh_forces_over_p for every p (:895)
+    F = tree.forces_groups(theta2, G, eps2)  # forces_groups (:1811)
+    F = tree.dehnen_eval(theta2, eps2, G)    # dehnen_eval (:1680)
+
+Forces come back as (3, N) float64 arrays, the reference's ``forces(:,p)``
+layout, or are written into a caller-provided device buffer.  Arrays may be
+numpy (host) or DeviceArrays (HBM-resident; no host round trip).  All work
+runs in libhq.so on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .device import DeviceArray
+
+METHODS = (10, 11, 12, 20, 21, 22, 30, 31, 32)
+
+
+def _is_device(a):
+    return isinstance(a, DeviceArray)
+
+
+def _addr(a):
+    """(pointer, where) of a numpy array or a DeviceArray of float64."""
+    if _is_device(a):
+        if a.dtype != np.float64:
+            raise TypeError("device arrays must be float64")
+        return a.ptr, L.HQ_DEVICE
+    return C.c_void_p(a.ctypes.data), L.HQ_HOST
+
+
+class OctreeType:
+    """The OctreeType handle: one GPU tree, one HIP stream."""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        self.N = 0
+        self.method = None
+        self.is_allocated = False
+
+    # -- pre_init / init ------------------------------------------------------
+    def pre_init(self, m, method=10, group_size=64, precision="fp64"):
+        if method not in METHODS:
+            raise ValueError(f"method not identified! ({method})")
+        prec = {"fp64": L.HQ_FP64, "mixed": L.HQ_MIXED}[precision]
+        self.close()
+        self._m = m if _is_device(m) else L.f64(m)
+        self.N = int(self._m.shape[0])
+        self.method = method
+        self.group_size = group_size
+        lib = L.lib()
+        L.check(lib.hq_tree_create(C.byref(self._h), self.N, method, group_size, prec))
+        p, where = _addr(self._m)
+        L.check(lib.hq_tree_set_masses(self._h, p, where))
+        self.is_allocated = True
+        return self
+
+    def init(self, x, y, z):
+        if not self.is_allocated:
+            raise RuntimeError("Not allocated! Run pre_init first!")
+        xs = [a if _is_device(a) else L.f64(a) for a in (x, y, z)]
+        if any(int(a.shape[0]) != self.N for a in xs):
+            raise ValueError("x, y, z must have the length of m")
+        (px, w), (py, _), (pz, _) = (_addr(a) for a in xs)
+        L.check(L.lib().hq_tree_build(self._h, px, py, pz, w))
+        self._cache = None
+        return self
+
+    # -- forces ---------------------------------------------------------------
+    def _run(self, fn, out, layout, *args):
+        if out is None:
+            buf = np.zeros(3 * self.N)
+            L.check(fn(self._h, *args, C.c_void_p(buf.ctypes.data), layout, L.HQ_HOST))
+            if layout == L.HQ_LAYOUT_3N:       # F[3*p + c] -> (3, N) view
+                return buf.reshape(self.N, 3).T
+            return buf.reshape(3, self.N).T    # F[c*N + p] -> (N, 3) Fortran order
+        p, where = _addr(out)
+        L.check(fn(self._h, *args, p, layout, where))
+        return out
+
+    def forces_all(self, theta2, G=1.0, eps2=0.0, out=None):
+        """bh_forces_over_p (:895-1078) for p = 1..N, as the harness loop does
+        (benchmarks/test_forces.f90:199-213); (3, N)."""
+        return self._run(L.lib().hq_tree_forces, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
+
+    def forces(self, p, theta2=0.0, G=1.0, eps2=0.0):
+        """Force on particle p (0-based), the reference's per-particle call.  The
+        GPU evaluates all targets at once; repeated calls with the same
+        parameters reuse that result."""
+        key = (theta2, G, eps2)
+        if getattr(self, "_cache", None) is None or self._cache[0] != key:
+            self._cache = (key, self.forces_all(theta2, G, eps2))
+        return self._cache[1][:, p].copy()
+
+    def forces_groups(self, theta2, G=1.0, eps2=0.0, out=None):
+        if self.method not in (30, 31, 32):
+            raise ValueError("forces_groups needs method 30, 31 or 32")
+        return self._run(L.lib().hq_tree_forces, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
+
+    def dehnen_eval(self, theta2, eps2=0.0, G=1.0, out=None):
+        if self.method not in (20, 21, 22):
+            raise ValueError("dehnen_eval needs method 20, 21 or 22")
+        return self._run(L.lib().hq_tree_forces, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
+
+    def forces_bunched(self, theta2, G=1.0, eps2=0.0, precision="mixed", out=None):
+        """Barnes (1990) shared-list walk over 64-particle bunches in tree order
+        (FP64 or mixed precision); (3, N)."""
+        prec = {"fp64": L.HQ_FP64, "mixed": L.HQ_MIXED}[precision]
+        lib = L.lib()
+
+        def fn(h, th2, g, e2, p, layout, where):
+            return lib.hq_tree_forces_bunched(h, th2, g, e2, p, layout, where, prec)
+        return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
+
+    # -- introspection ----------------------------------------------------------
+    def info(self):
+        inf = L.TreeInfo()
+        L.check(L.lib().hq_tree_info(self._h, C.byref(inf)))
+        return inf.as_dict()
+
+    @property
+    def number_of_nodes(self):
+        return self.info()["nnodes"]
+
+    def export(self):
+        """Node store in level order (see include/hq.h hq_tree_export)."""
+        nn = self.number_of_nodes
+        out = dict(depth=np.zeros(nn, np.int32), type=np.zeros(nn, np.int32),
+                   particle=np.zeros(nn, np.int32), child_start=np.zeros(nn + 1, np.int32),
+                   key_hi=np.zeros(nn, np.uint64), key_lo=np.zeros(nn, np.uint32),
+                   mass=np.zeros(nn), com=np.zeros((nn, 3)), quad=np.zeros((nn, 6)),
+                   oct=np.zeros((nn, 14)), center=np.zeros((nn, 3)))
+        L.check(L.lib().hq_tree_export(
+            self._h, L.ptr(out["depth"], L._i), L.ptr(out["type"], L._i),
+            L.ptr(out["particle"], L._i), L.ptr(out["child_start"], L._i),
+            L.ptr(out["key_hi"], L._u64), L.ptr(out["key_lo"], L._u32), L.ptr(out["mass"]),
+            L.ptr(out["com"]), L.ptr(out["quad"]), L.ptr(out["oct"]), L.ptr(out["center"])))
+        return out
+
+    def order(self):
+        o = np.zeros(self.N, np.int32)
+        L.check(L.lib().hq_tree_order(self._h, L.ptr(o, L._i)))
+        return o
+
+    def close(self):
+        if self._h:
+            L.lib().hq_tree_destroy(self._h)
+            self._h = C.c_void_p()
+        self.is_allocated = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,158 @@
+/*
+ * hq.h -- C ABI of libhq.so, the MI355X (gfx950) hierarchical gravity library.
+ *
+ * Plain C types only.  Every entry point returns an int status (HQ_OK = 0);
+ * hq_last_error() gives the message of the calling thread's last failure.
+ * Arrays are float64 unless stated.  "where" says whether a pointer is host
+ * memory (HQ_HOST) or device memory of the current HIP device (HQ_DEVICE).
+ * Calls on one handle are serialised by the caller; different handles are
+ * independent (one HIP stream each).  No call exits the process.
+ *
+ * Reference interfaces replaced (paths relative to Potalej/hierarquicos):
+ *   f2py module `api` (fortran/api3d.f90, built by fortran/compile.sh:2)
+ *   OctreeType type-bound API (fortran/octree_default.f90:145-170)
+ *   compute_forces_direct (fortran/direct_forces.f90:8)
+ *   morton_tree_type (fortran/morton/morton_octree.f90:49-64)
+ */
+#ifndef HQ_H
+#define HQ_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HQ_OK 0
+#define HQ_ERR_ARGS 1       /* bad argument (size, method, null pointer) */
+#define HQ_ERR_CAPACITY 2   /* node capacity exceeded; the reference STOPs with
+                               status 0 (octree_default.f90:457-460) */
+#define HQ_ERR_LOST 3       /* particles dropped at max depth 25
+                               (octree_default.f90:570-579); fatal only for the
+                               grouped methods, whose reference STOPs ("body
+                               lost", :1758-1760) */
+#define HQ_ERR_HIP 4        /* HIP runtime error */
+#define HQ_ERR_NONFINITE 5  /* NaN or Inf in the input positions */
+#define HQ_ERR_DEPTH 6      /* Morton depth above 21 levels (64-bit keys) */
+#define HQ_ERR_NOMEM 7      /* device allocation failed */
+#define HQ_ERR_OVERFLOW 8   /* an interaction list or walk stack overflowed */
+
+#define HQ_HOST 0
+#define HQ_DEVICE 1
+
+/* precision of the force evaluation */
+#define HQ_FP64 0   /* plain FP64, the reference's arithmetic */
+#define HQ_MIXED 1  /* FP32 interactions on recentred coordinates with
+                       compensated sums; FP64 keys, positions, moments */
+
+/* force layouts */
+#define HQ_LAYOUT_3N 0  /* F[3*p + c], the OctreeType (3,N) layout */
+#define HQ_LAYOUT_N3 1  /* F[c*N + p], the f2py (N,3) Fortran-order layout */
+
+/* ---- f2py `api.api_mod` surface (fortran/api3d.f90) ------------------- */
+
+/* api3d.f90:43-72 compute_forces_direct(m, x, y, z, G, eps) -> F(N,3).
+ * eps is the softening length (squared inside); host arrays; F is (N,3)
+ * column-major. */
+int hq_compute_forces_direct(int64_t n, const double *m, const double *x, const double *y,
+                             const double *z, double G, double eps, double *F);
+
+/* api3d.f90:20-41 compute_forces(m, x, y, z, theta, G, eps, quad) -> F(N,3).
+ * theta is the opening angle and eps the softening length (both squared
+ * inside); method selects 10/11/12 (BH mono/quad/oct), 20/21/22 (Dehnen) or
+ * 30/31/32 (grouped BH).  The legacy call passed theta unsquared into a
+ * routine that expects theta^2: pass legacy_theta2 = 1 to treat theta as
+ * theta^2 the way that call did. */
+int hq_compute_forces(int64_t n, const double *m, const double *x, const double *y,
+                      const double *z, double theta, double G, double eps, int method,
+                      int legacy_theta2, double *F);
+
+/* api3d.f90:9-18 generate_tree(m, x, y, z, quad): builds and discards. */
+int hq_generate_tree(int64_t n, const double *m, const double *x, const double *y,
+                     const double *z, int method);
+
+/* ---- OctreeType handle (octree_default.f90) ---------------------------- */
+typedef struct hq_tree hq_tree;
+
+/* pre_init(m, method, group_size) (:178-282).  group_size <= 0 gives 64.
+ * precision: HQ_FP64 or HQ_MIXED. */
+int hq_tree_create(hq_tree **out, int64_t n, int method, int group_size, int precision);
+int hq_tree_set_masses(hq_tree *t, const double *m, int where);
+/* init(x, y, z) (:320-387): keys, radix sort, octree from sorted keys,
+ * upward P2M/M2M pass, groups. */
+int hq_tree_build(hq_tree *t, const double *x, const double *y, const double *z, int where);
+/* Forces over all particles with theta^2, G, eps^2 as the reference takes
+ * them: bh_forces_over_p (:895) for methods 10-12, dehnen_eval (:1680) for
+ * 20-22, forces_groups (:1811) for 30-32.  layout: HQ_LAYOUT_3N or _N3. */
+int hq_tree_forces(hq_tree *t, double theta2, double G, double eps2, double *F, int layout,
+                   int where);
+/* Mixed-precision bunch walk of the same tree: contiguous 64-particle
+ * bunches in tree order with their tight bounding boxes under the grouped
+ * (Barnes 1990) acceptance test of octree_default.f90:1927-1937. */
+int hq_tree_forces_bunched(hq_tree *t, double theta2, double G, double eps2, double *F,
+                           int layout, int where, int precision);
+int hq_tree_destroy(hq_tree *t);
+
+typedef struct {
+    int64_t n;
+    int64_t nnodes;        /* number_of_nodes */
+    int64_t ngroups;       /* groups (methods 30-32) */
+    int64_t lost;          /* particles dropped at max depth */
+    int most_depth;        /* deepest node */
+    int multipole;         /* 1, 4, 8 */
+    int64_t list_cells;    /* cells in the last walk's lists (diagnostic) */
+    int64_t list_parts;    /* particles in the last walk's lists (diagnostic) */
+    int64_t n_p2p;         /* per-target BH counts of the last methods 10-12 walk */
+    int64_t n_m2p;
+    double ms_build;       /* device time of the last build (HIP events) */
+    double ms_keys, ms_sort, ms_nodes, ms_upward;
+    double ms_walk;        /* device time of the last force walk kernel(s) */
+    double ms_lists;       /* list building part of the last grouped walk */
+    double ms_eval;        /* evaluation part of the last grouped walk */
+} hq_tree_info_t;
+int hq_tree_info(const hq_tree *t, hq_tree_info_t *info);
+
+/* Node store in level order (depth-major, tree order within a depth) copied
+ * to host arrays of length nnodes (child_start: nnodes+1).  Any pointer may
+ * be NULL.  key_hi/key_lo hold the node's octant path, 3 bits per level with
+ * digit 4*(x>=cx) + 2*(y>=cy) + (z>=cz): levels 1..21 in key_hi (level 1 in
+ * bits 62..60), levels 22..25 in key_lo (level 22 in bits 11..9). */
+int hq_tree_export(const hq_tree *t, int *depth, int *type, int *particle, int *child_start,
+                   uint64_t *key_hi, uint32_t *key_lo, double *mass, double *com /* 3*nn */,
+                   double *quad /* 6*nn */, double *oct /* 14*nn */, double *center /* 3*nn */);
+/* Tree order of the particles: order[i] = original index of sorted slot i. */
+int hq_tree_order(const hq_tree *t, int *order);
+
+/* ---- direct sum (fortran/direct_forces.f90:8-64) ----------------------- */
+/* Gather form over nt targets (targets == NULL: all n, in order); F is 3*nt
+ * in HQ_LAYOUT_3N.  All pointers device memory; synchronous. */
+int hq_direct_forces_device(int64_t n, const double *m, const double *x, const double *y,
+                            const double *z, int64_t nt, const int64_t *targets, double G,
+                            double eps2, int precision, double *F);
+
+/* ---- Morton tree (fortran/morton/morton_octree.f90) -------------------- */
+/* Keys of generate_nodes (:357-394): normalise, quantise, interleave.
+ * L = 0 chooses the smallest depth with distinct keys (choose_depth
+ * :217-244).  Outputs (host or device per `where`): keys in original
+ * order, the stable sort permutation perm (sorted slot -> original index)
+ * and the depth used. */
+int hq_morton_keys(int64_t n, const double *m, const double *q /* (3,N) */, int L, double ssa,
+                   int64_t *keys, int *perm, int *L_out, int where);
+
+/* ---- device memory plumbing ------------------------------------------- */
+/* No counterpart in the reference (it has no device): thin wrappers of
+ * hipMalloc / hipFree / hipMemcpy / hipDeviceSynchronize so that callers
+ * without a HIP runtime of their own can hold inputs and forces in HBM.
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+int hq_dev_alloc(void **p, int64_t bytes);
+int hq_dev_free(void *p);
+int hq_dev_copy(void *dst, const void *src, int64_t bytes, int kind);
+int hq_dev_sync(void);
+
+const char *hq_last_error(void);
+const char *hq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -74,6 +74,20 @@ SUBROUTINE ref_tree_bh(theta2, G, eps2, F) BIND(C, name="ref_tree_bh")
     !$OMP END PARALLEL DO
 END SUBROUTINE
 
+! bh_forces_over_p for a subset of targets (0-based indices); F is (3,nt)
+SUBROUTINE ref_tree_bh_targets(nt, targets, theta2, G, eps2, F) BIND(C, name="ref_tree_bh_targets")
+    INTEGER(C_INT), VALUE :: nt
+    INTEGER(C_INT64_T), INTENT(IN) :: targets(nt)
+    REAL(C_DOUBLE), VALUE :: theta2, G, eps2
+    REAL(C_DOUBLE), INTENT(OUT) :: F(3, nt)
+    INTEGER :: i
+    !$OMP PARALLEL DO SCHEDULE(DYNAMIC, 64) SHARED(tree, F, targets, theta2, G, eps2) PRIVATE(i)
+    DO i = 1, nt
+        F(:, i) = tree % forces(INT(targets(i)) + 1, theta2, G, eps2)
+    END DO
+    !$OMP END PARALLEL DO
+END SUBROUTINE
+
 ! forces_groups (octree_default.f90:1811); F is (3,n)
 SUBROUTINE ref_tree_groups(theta2, G, eps2, F, ngroups) BIND(C, name="ref_tree_groups")
     REAL(C_DOUBLE), VALUE :: theta2, G, eps2

@@ -1,0 +1,237 @@
+"""GPU parity: libhq.so (through its C ABI) against the reference's golden
+vectors and the CPU oracle.
+
+Bars: keys, permutations and tree topology bit-exact; moments bit-exact
+(same summation order, contraction off); FP64 forces within 1e-10 relative
+per particle of the reference; mixed precision within an RMS relative error
+of 1e-4 against the FP64 direct sum.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from hierarquicos_amd import OctreeType, api, ics
+from hierarquicos_amd.device import morton_keys
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def relmax(F, Fr):
+    """max over particles of |F - Fr| / |Fr| (norms of the 3-vectors)."""
+    F = np.asarray(F).reshape(-1, 3) if np.asarray(F).shape[-1] == 3 else np.asarray(F).T
+    Fr = np.asarray(Fr)
+    num = np.linalg.norm(F - Fr, axis=1)
+    den = np.linalg.norm(Fr, axis=1)
+    return float(np.max(num / np.where(den > 0, den, 1.0)))
+
+
+def gpu_forces(m, x, y, z, method, theta2, eps2, G=1.0, gs=64):
+    t = OctreeType().pre_init(m, method, gs).init(x, y, z)
+    if method < 20:
+        F = t.forces_all(theta2, G, eps2)
+    elif method < 30:
+        F = t.dehnen_eval(theta2, eps2, G)
+    else:
+        F = t.forces_groups(theta2, G, eps2)
+    return t, np.ascontiguousarray(F.T)
+
+
+# ---------------------------------------------------------------- direct sum
+def test_direct_fp64_matches_reference(golden):
+    d = golden("plummer_n1000_seed3.npz")
+    eps = float(np.sqrt(d["eps2"]))
+    F = api.api_mod.compute_forces_direct(d["m"], d["x"], d["y"], d["z"], 1.0, eps)
+    assert F.shape == (1000, 3)
+    assert relmax(F, d["F_direct"]) < 1e-13
+
+
+def test_direct_mixed_error_budget(golden):
+    from hierarquicos_amd import DeviceArray, direct_forces
+    d = golden("plummer_n1000_seed3.npz")
+    F = direct_forces(d["m"], d["x"], d["y"], d["z"], 1.0, float(d["eps2"]), precision="mixed")
+    rms, mx = O.rel_errors(F, d["F_direct"])
+    assert rms < 1e-6 and mx < 1e-5, (rms, mx)
+    # target subset in the gather form, device-resident inputs
+    dm, dx, dy, dz = (DeviceArray.from_numpy(d[k]) for k in ("m", "x", "y", "z"))
+    tg = np.array([0, 17, 999, 500])
+    Ft = direct_forces(dm, dx, dy, dz, 1.0, float(d["eps2"]), targets=tg)
+    assert relmax(Ft, d["F_direct"][tg]) < 1e-13
+
+
+# ---------------------------------------------------------------- tree build
+def _canonical(depth, khi, klo):
+    return [(int(a), int(b), int(c)) for a, b, c in zip(depth, khi, klo)]
+
+
+def test_tree_topology_and_moments_bitexact(golden):
+    d = golden("plummer_n1000_seed3.npz")
+    t = OctreeType().pre_init(d["m"], 12).init(d["x"], d["y"], d["z"])
+    e = t.export()
+    nref = len(d["tree12_depth"])
+    assert len(e["depth"]) == nref == int(d["nnodes_12"])
+    ref_keys = _canonical(d["tree12_depth"], d["tree12_khi"], d["tree12_klo"])
+    gpu_keys = _canonical(e["depth"], e["key_hi"], e["key_lo"])
+    assert len(set(ref_keys)) == nref
+    rpos = {k: i for i, k in enumerate(ref_keys)}
+    perm = np.array([rpos[k] for k in gpu_keys])  # gpu node -> reference node
+    assert np.array_equal(np.sort(perm), np.arange(nref))
+    assert np.array_equal(e["type"], d["tree12_type"][perm])
+    assert np.array_equal(e["particle"], d["tree12_particle"][perm])
+    assert np.array_equal(e["mass"], d["tree12_mass"][perm])
+    com_ref = np.stack([d["tree12_qx"], d["tree12_qy"], d["tree12_qz"]], 1)[perm]
+    assert np.array_equal(e["com"], com_ref)
+    assert np.array_equal(e["quad"], d["tree12_quad"][perm])
+    assert np.array_equal(e["oct"], d["tree12_oct"][perm])
+    cen_ref = np.stack([d["tree12_cx"], d["tree12_cy"], d["tree12_cz"]], 1)[perm]
+    assert np.array_equal(e["center"], cen_ref)
+    # children are a contiguous range in level order and agree with the reference
+    cs = e["child_start"]
+    for k in range(nref):
+        kids = set(perm[cs[k]:cs[k + 1]].tolist())
+        rk = perm[k]
+        assert kids == {c for c in d["tree12_child"][rk] if c >= 0}
+
+
+def test_tree_topology_hernquist_bitexact(golden):
+    d = golden("hernquist_n4096_seed5.npz")
+    t = OctreeType().pre_init(d["m"], 11).init(d["x"], d["y"], d["z"])
+    e = t.export()
+    ref = sorted(zip(_canonical(d["tree11_depth"], d["tree11_khi"], d["tree11_klo"]),
+                     d["tree11_particle"].tolist(), d["tree11_mass"].tolist()))
+    got = sorted(zip(_canonical(e["depth"], e["key_hi"], e["key_lo"]), e["particle"].tolist(),
+                     e["mass"].tolist()))
+    assert ref == got
+    assert t.info()["lost"] == 0
+
+
+# ---------------------------------------------------------------- FP64 forces
+@pytest.mark.parametrize("method", [10, 11, 12, 30, 31, 32])
+def test_fp64_forces_match_reference(golden, method):
+    d = golden("plummer_n1000_seed3.npz")
+    _, F = gpu_forces(d["m"], d["x"], d["y"], d["z"], method, float(d["theta2"]),
+                      float(d["eps2"]))
+    assert relmax(F, d[f"F_{method}"]) < FP64_TOL
+
+
+@pytest.mark.parametrize("method", [11, 12, 31, 32])
+def test_fp64_forces_deep_tree(golden, method):
+    d = golden("hernquist_n4096_seed5.npz")
+    _, F = gpu_forces(d["m"], d["x"], d["y"], d["z"], method, float(d["theta2"]),
+                      float(d["eps2"]))
+    assert relmax(F, d[f"F_{method}"]) < FP64_TOL
+
+
+def test_bh_counts_match_oracle():
+    m, q, _ = ics.plummer(5000, seed=21)
+    x, y, z = q.T.copy()
+    t = OctreeType().pre_init(m, 12).init(x, y, z)
+    F = t.forces_all(0.25, 1.0, 1e-6).T
+    info = t.info()
+    Fo, c = O.Tree(m, 12).init(x, y, z).bh_forces(0.25, 1.0, 1e-6, counts=True)
+    assert relmax(F, Fo) < FP64_TOL
+    assert info["n_p2p"] == int(c[:, 0].sum()) and info["n_m2p"] == int(c[:, 1].sum())
+
+
+def test_edge_cases(golden):
+    d = golden("edge_cases.npz")
+    for method in (10, 11, 12):
+        _, F = gpu_forces(d["two_m"], d["two_x"], d["two_y"], d["two_z"], method, 0.25, 1e-4)
+        assert relmax(F, d[f"two_F_{method}"]) < FP64_TOL
+        t, F = gpu_forces(d["dup_m"], d["dup_x"], d["dup_y"], d["dup_z"], method, 0.25, 1e-4)
+        assert t.number_of_nodes == int(d[f"dup_nnodes_{method}"])
+        assert t.info()["lost"] == 2
+        assert relmax(F, d[f"dup_F_{method}"]) < FP64_TOL
+
+
+def test_ten_points(golden):
+    d = golden("points10.npz")
+    q = d["q"]
+    for method in (11, 12):
+        _, F = gpu_forces(d["m"], q[:, 0], q[:, 1], q[:, 2], method, 0.01, 1e-4)
+        assert relmax(F, d[f"F_{method}"]) < FP64_TOL
+
+
+def test_larger_n_matches_oracle():
+    m, q, _ = ics.plummer(30000, seed=8)
+    x, y, z = q.T.copy()
+    for method in (12, 32):
+        _, F = gpu_forces(m, x, y, z, method, 0.25, 1e-6)
+        t = O.Tree(m, method).init(x, y, z)
+        Fo = t.bh_forces(0.25, 1.0, 1e-6) if method < 30 else t.forces_groups(0.25, 1.0, 1e-6)
+        assert relmax(F, Fo) < FP64_TOL
+
+
+def test_f2py_surface():
+    m, q, _ = ics.plummer(700, seed=9)
+    x, y, z = q.T.copy()
+    F = api.api_mod.compute_forces(m, x, y, z, 0.5, 1.0, 0.01, True)
+    Fo = O.Tree(m, 11).init(x, y, z).bh_forces(0.25, 1.0, 1e-4)
+    assert F.shape == (700, 3) and F.flags.f_contiguous
+    assert relmax(F, Fo) < FP64_TOL
+    # legacy: theta passed straight through as theta^2
+    F2 = api.api_mod.compute_forces(m, x, y, z, 0.25, 1.0, 0.01, True, legacy_theta2=True)
+    assert relmax(F2, Fo) < FP64_TOL
+    api.api_mod.generate_tree(m, x, y, z, True)
+
+
+def test_errors_are_status_codes():
+    from hierarquicos_amd import HQError
+    m = np.full(4, 0.25)
+    x = np.array([0.0, 1.0, np.nan, 2.0])
+    with pytest.raises(HQError) as ei:
+        OctreeType().pre_init(m, 12).init(x, x, x)
+    assert ei.value.code == 5
+    with pytest.raises(ValueError):
+        OctreeType().pre_init(m, 13)
+    # grouped methods stop on particles lost at max depth (prepare_groups)
+    mm = np.full(8, 0.125)
+    xx = np.array([0.0, 1.0, 2.0, 0.5, 0.5, 3.0, 1.5, 2.5])
+    with pytest.raises(HQError) as ei:
+        t = OctreeType().pre_init(mm, 32).init(xx, xx * 0.5, xx * 0.25)
+        t.forces_groups(0.25, 1.0, 1e-4)
+    assert ei.value.code == 3
+
+
+# ---------------------------------------------------------------- Morton keys
+def test_morton_keys_and_permutation_bitexact(golden):
+    d = golden("plummer_n1000_seed3.npz")
+    q = np.stack([d["x"], d["y"], d["z"]], 1)
+    keys, perm, L = morton_keys(d["m"], q, 0)
+    assert L == int(d["morton_L"])
+    assert np.array_equal(perm, d["morton_perm"])
+    assert np.array_equal(keys[perm], d["morton_keys_sorted"])
+    d10 = golden("points10.npz")
+    keys, perm, L = morton_keys(d10["m"], d10["q"], 0)
+    assert L == int(d10["morton_L"]) and np.array_equal(perm, d10["morton_perm"])
+    # fixed depth with ties: stable order
+    keys, perm, L = morton_keys(d["m"], q, 3)
+    ko = O.morton_keys(d["m"], q, 3)[0]
+    assert L == 3 and np.array_equal(keys, ko)
+    assert np.array_equal(perm, np.argsort(ko, kind="stable"))
+
+
+# ---------------------------------------------------------------- mixed precision
+def test_mixed_bunched_error_budget():
+    m, q, _ = ics.plummer(50000, seed=12)
+    x, y, z = q.T.copy()
+    t = OctreeType().pre_init(m, 12).init(x, y, z)
+    Fm = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    F64 = t.forces_bunched(0.25, 1.0, 1e-6, "fp64").T
+    sample = np.random.default_rng(0).choice(50000, 2048, replace=False)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    rms_m, max_m = O.rel_errors(Fm[sample], Fd)
+    rms_d, _ = O.rel_errors(F64[sample], Fd)
+    assert rms_m < 1e-4, (rms_m, max_m)
+    # the FP32 arithmetic adds little on top of the truncation error
+    assert abs(rms_m - rms_d) < 0.05 * rms_d + 1e-6, (rms_m, rms_d)
+
+
+def test_mixed_deterministic():
+    m, q, _ = ics.plummer(20000, seed=13)
+    x, y, z = q.T.copy()
+    t = OctreeType().pre_init(m, 12).init(x, y, z)
+    a = np.array(t.forces_bunched(0.25, 1.0, 1e-6, "mixed"))
+    b = np.array(t.forces_bunched(0.25, 1.0, 1e-6, "mixed"))
+    assert np.array_equal(a, b)
