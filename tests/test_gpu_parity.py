@@ -235,3 +235,26 @@ def test_mixed_deterministic():
     a = np.array(t.forces_bunched(0.25, 1.0, 1e-6, "mixed"))
     b = np.array(t.forces_bunched(0.25, 1.0, 1e-6, "mixed"))
     assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- Dehnen
+@pytest.mark.parametrize("method", [20, 21, 22])
+def test_dehnen_matches_reference(golden, method):
+    d = golden("plummer_n1000_seed3.npz")
+    _, F = gpu_forces(d["m"], d["x"], d["y"], d["z"], method, float(d["theta2"]),
+                      float(d["eps2"]))
+    assert relmax(F, d[f"F_{method}"]) < FP64_TOL
+
+
+def test_dehnen_deep_tree_and_momentum(golden):
+    d = golden("hernquist_n4096_seed5.npz")
+    _, F = gpu_forces(d["m"], d["x"], d["y"], d["z"], 21, float(d["theta2"]), float(d["eps2"]))
+    assert relmax(F, d["F_21"]) < FP64_TOL
+    m, q, _ = ics.plummer(20000, seed=14)
+    x, y, z = q.T.copy()
+    for method in (20, 21, 22):
+        _, F = gpu_forces(m, x, y, z, method, 0.36, 1e-6)
+        Fo = O.Tree(m, method).init(x, y, z).dehnen(0.36, 1e-6, 1.0)
+        assert relmax(F, Fo) < FP64_TOL
+        if method < 22:  # mutual M2L conserves momentum (Marcello's term targets torque)
+            assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max() * len(m)
