@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--eps", type=float, default=1e-3)
     ap.add_argument("--method", type=int, default=12)
+    ap.add_argument("--bunch", type=int, default=64, help="bunch size (<= 64)")
+    ap.add_argument("--super", type=int, default=256, help="super-group size of the bunching")
+    ap.add_argument("--unfused", action="store_true", help="lists and evaluation as two kernels")
     ap.add_argument("--sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -74,6 +77,7 @@ def main():
         sys.exit(2)
 
     from hierarquicos_amd import DeviceArray, OctreeType, direct_forces, ics
+    from hierarquicos_amd import _lib as L
     from hierarquicos_amd.device import sync
 
     N = args.n
@@ -84,11 +88,18 @@ def main():
     dm = DeviceArray.from_numpy(m)
     dx, dy, dz = (DeviceArray.from_numpy(np.ascontiguousarray(q[:, k])) for k in range(3))
     dF = DeviceArray((3 * N,))
-    tree = OctreeType().pre_init(dm, args.method, 64, precision="mixed")
+    tree = OctreeType().pre_init(dm, args.method, args.bunch, precision="mixed")
+    tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused)
 
     def step():
         tree.init(dx, dy, dz)
         tree.forces_bunched(th2, 1.0, e2, "mixed", out=dF)
+
+    # a plain 256 MiB device-to-device copy: the reference point that scales
+    # the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile to bytes
+    probe_a, probe_b = DeviceArray((1 << 25,)), DeviceArray((1 << 25,))
+    L.check(L.lib().hq_dev_copy(probe_b.ptr, probe_a.ptr, probe_a.nbytes, 2))
+    del probe_a, probe_b
 
     for _ in range(args.warmup):
         step()
@@ -123,7 +134,8 @@ def main():
     if not args.no_extras:
         # reference per-target BH counts (leaf size 1, same theta) from the FP64
         # walk of methods 10-12 = bh_forces_over_p, and the plain FP64 paths
-        t64 = OctreeType().pre_init(dm, args.method, 64, precision="fp64")
+        t64 = OctreeType().pre_init(dm, args.method, args.bunch, precision="fp64")
+        t64.tune(bunch=args.bunch, super_size=args.super)
         dF64 = DeviceArray((3 * N,))
         t64.init(dx, dy, dz)
         sync()
@@ -156,12 +168,18 @@ def main():
     if flops_alg is not None:
         achieved = flops_alg / (eval_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "compute_unit": "VALU (FP32 FMA/rsqrt; no matrix ops)",
-                "kernel": "k_eval_mixed<8>",
+                "kernel": "k_eval_mixed<8>" if args.unfused else "k_walk_mixed<8,1536>",
                 "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                 "flops_per_launch": flops_alg, "kernel_ms": eval_ms,
-                "executed_interactions": {"cells": info["list_cells"],
-                                          "particles": info["list_parts"]}}
+                "list_entries": {"cells": info["list_cells"], "particles": info["list_parts"]},
+                "executed_interactions": {"cells": info["exec_cells"],
+                                          "particles": info["exec_parts"]},
+                "executed_tflops": (FLOPS_M2P[info["multipole"]] * info["exec_cells"]
+                                    + FLOPS_P2P * info["exec_parts"]) / (eval_ms * 1e-3) / 1e12,
+                "bunches": info["ngroups"],
+                "longest_lists": {"cells": info["max_cells"], "particles": info["max_parts"],
+                                  "stack": info["max_stack"]}}
     line = {
         "metric": "particles/s per force eval at N=1M, rms rel err <=1e-4 vs FP64 direct",
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
@@ -172,12 +190,17 @@ def main():
                    "N": N, "seed": args.seed, "theta": args.theta, "eps": args.eps,
                    "order": {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method],
                    "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
-                   "walk": "bunched Barnes-1990 lists, 64 particles per bunch"},
+                   "walk": (f"bunched Barnes-1990 lists, <= {args.bunch} particles per bunch, "
+                            f"cut from sibling subtrees packed to <= {args.super}")},
         "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
         "phases_ms": {"build": float(np.mean(builds)), "lists": float(np.mean(lists)),
                       "eval": eval_ms, "keys": info["ms_keys"], "sort": info["ms_sort"],
                       "nodes": info["ms_nodes"], "upward": info["ms_upward"]},
         "nnodes": info["nnodes"],
+        "walk_stats": {"bunches": info["ngroups"], "list_cells": info["list_cells"],
+                       "list_parts": info["list_parts"], "exec_cells": info["exec_cells"],
+                       "exec_parts": info["exec_parts"], "max_cells": info["max_cells"],
+                       "max_parts": info["max_parts"], "max_stack": info["max_stack"]},
         "roofline": roof,
         "cpu_baseline": base,
     }

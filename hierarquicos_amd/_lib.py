@@ -20,6 +20,7 @@ HQ_ERR_NONFINITE, HQ_ERR_DEPTH, HQ_ERR_NOMEM, HQ_ERR_OVERFLOW = 5, 6, 7, 8
 HQ_HOST, HQ_DEVICE = 0, 1
 HQ_FP64, HQ_MIXED = 0, 1
 HQ_LAYOUT_3N, HQ_LAYOUT_N3 = 0, 1
+HQ_TUNE_BUNCH, HQ_TUNE_SUPER, HQ_TUNE_FUSED = 1, 2, 3
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int)
@@ -31,7 +32,7 @@ _vp = C.c_void_p
 EXPORTS = (
     "hq_compute_forces_direct", "hq_compute_forces", "hq_generate_tree", "hq_tree_create",
     "hq_tree_set_masses", "hq_tree_build", "hq_tree_forces", "hq_tree_forces_bunched",
-    "hq_tree_destroy", "hq_tree_info", "hq_tree_export", "hq_tree_order",
+    "hq_tree_destroy", "hq_tree_tune", "hq_tree_info", "hq_tree_export", "hq_tree_order",
     "hq_direct_forces_device", "hq_morton_keys", "hq_dev_alloc", "hq_dev_free", "hq_dev_copy",
     "hq_dev_sync", "hq_last_error", "hq_version")
 
@@ -42,7 +43,9 @@ class TreeInfo(C.Structure):
                 ("list_cells", C.c_int64), ("list_parts", C.c_int64), ("n_p2p", C.c_int64),
                 ("n_m2p", C.c_int64), ("ms_build", C.c_double), ("ms_keys", C.c_double),
                 ("ms_sort", C.c_double), ("ms_nodes", C.c_double), ("ms_upward", C.c_double),
-                ("ms_walk", C.c_double), ("ms_lists", C.c_double), ("ms_eval", C.c_double)]
+                ("ms_walk", C.c_double), ("ms_lists", C.c_double), ("ms_eval", C.c_double),
+                ("exec_cells", C.c_int64), ("exec_parts", C.c_int64), ("max_cells", C.c_int64),
+                ("max_parts", C.c_int64), ("max_stack", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,6 +81,7 @@ def lib():
     L.hq_tree_forces_bunched.argtypes = [_vp, C.c_double, C.c_double, C.c_double, _vp, C.c_int,
                                          C.c_int, C.c_int]
     L.hq_tree_destroy.argtypes = [_vp]
+    L.hq_tree_tune.argtypes = [_vp, C.c_int, C.c_int64]
     L.hq_tree_info.argtypes = [_vp, C.POINTER(TreeInfo)]
     L.hq_tree_export.argtypes = [_vp, _i, _i, _i, _i, _u64, _u32, _d, _d, _d, _d, _d]
     L.hq_tree_order.argtypes = [_vp, _i]

@@ -120,6 +120,22 @@ int hq_tree_destroy(hq_tree* t) {
     return HQ_OK;
 }
 
+int hq_tree_tune(hq_tree* t, int key, int64_t value) {
+    if (!t) return fail(HQ_ERR_ARGS, "null handle");
+    if (key == HQ_TUNE_BUNCH) {
+        if (value < 1 || value > 64) return fail(HQ_ERR_ARGS, "bunch size %lld outside 1..64", (long long)value);
+        t->group_size = (int)value;
+    } else if (key == HQ_TUNE_SUPER) {
+        if (value < 1 || value > (1 << 20)) return fail(HQ_ERR_ARGS, "super-group size %lld out of range", (long long)value);
+        t->super_size = (int)value;
+    } else if (key == HQ_TUNE_FUSED) {
+        t->fused = value != 0;
+    } else {
+        return fail(HQ_ERR_ARGS, "unknown tuning key %d", key);
+    }
+    return HQ_OK;
+}
+
 int hq_tree_set_masses(hq_tree* t, const double* m, int where) {
     if (!t || !m) return fail(HQ_ERR_ARGS, "null argument");
     int rc = stage(t->m, m, t->n, where, t->stream);
@@ -185,6 +201,11 @@ int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {
     info->ms_walk = t->ms_walk;
     info->ms_lists = t->ms_lists;
     info->ms_eval = t->ms_eval;
+    info->exec_cells = t->exec_cells;
+    info->exec_parts = t->exec_parts;
+    info->max_cells = t->max_cells;
+    info->max_parts = t->max_parts;
+    info->max_stack = t->max_stack;
     return HQ_OK;
 }
 

@@ -587,8 +587,15 @@ int read_status(hq_tree* t) {
 
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
 // keys; the result is left in keys / vals (ktmp / vtmp are scratch).
+int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s) {
+    int rc;
+    if ((rc = part.ensure(scan_part_size(len) * sizeof(uint32_t)))) return rc;
+    scan_excl(a, len, part.as<uint32_t>(), s);
+    return HQ_OK;
+}
+
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
-                     DevBuf& hist, hipStream_t s) {
+                     DevBuf& hist, DevBuf& part, hipStream_t s) {
     if (n <= 1) return HQ_OK;
     const int nblk = grid_for(n, RS_TILE);
     int rc;
@@ -599,7 +606,7 @@ int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64
     for (int pass = 0; pass < passes; ++pass) {
         const int shift = 8 * pass;
         k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist.as<uint32_t>(), nblk);
-        k_scan_excl<<<1, 1024, 0, s>>>(hist.as<uint32_t>(), (int64_t)256 * nblk);
+        if ((rc = scan_excl_buf(hist.as<uint32_t>(), (int64_t)256 * nblk, part, s))) return rc;
         k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist.as<uint32_t>(),
                                                  nblk);
         std::swap(ka, kb);
@@ -617,7 +624,7 @@ static int radix_sort(hq_tree* t) {
     if ((rc = t->khi2.ensure(t->n * sizeof(uint64_t)))) return rc;
     if ((rc = t->idx2.ensure(t->n * sizeof(int)))) return rc;
     return radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
-                            t->idx2.as<int>(), t->n, 63, t->hist, t->stream);
+                            t->idx2.as<int>(), t->n, 63, t->hist, t->scanpart, t->stream);
 }
 
 int tree_build(hq_tree* t) {
@@ -680,7 +687,7 @@ int tree_build(hq_tree* t) {
     if ((rc = t->hist.ensure(((size_t)NDEP * nblk + 1) * sizeof(uint32_t)))) return rc;
     k_node_count<<<nblk, NB_THREADS, 0, s>>>(t->lcp.as<int8_t>(), n, t->hist.as<uint32_t>(), nblk,
                                              t->status.as<int64_t>());
-    k_scan_excl<<<1, 1024, 0, s>>>(t->hist.as<uint32_t>(), (int64_t)NDEP * nblk);
+    if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), (int64_t)NDEP * nblk, t->scanpart, s))) return rc;
     k_node_emit<<<nblk, NB_THREADS, 0, s>>>(
         t->lcp.as<int8_t>(), t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), t->geom.as<double>(),
         n, t->hist.as<uint32_t>(), nblk, cap, t->ndepth.as<uint8_t>(), t->ntype.as<int8_t>(),

@@ -386,8 +386,9 @@ int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2) {
         uint32_t* bacc = sums.as<uint32_t>();
         uint32_t* bnext = bacc + nblk + 1;
         k_dual_count<<<nblk, DB, 0, s>>>(v, rmax, theta2, pc->as<int2>(), nf, bacc, bnext, nblk);
-        k_scan_excl<<<1, 1024, 0, s>>>(bacc, nblk);
-        k_scan_excl<<<1, 1024, 0, s>>>(bnext, nblk);
+        if ((rc = scan_excl_buf(bacc, nblk, t->scanpart, s)) ||
+            (rc = scan_excl_buf(bnext, nblk, t->scanpart, s)))
+            return rc;
         HQ_CHECK_HIP(hipMemcpyAsync(&tot[0], bacc + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HQ_CHECK_HIP(hipMemcpyAsync(&tot[1], bnext + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HQ_CHECK_HIP(hipStreamSynchronize(s));

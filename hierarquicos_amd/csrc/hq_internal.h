@@ -91,37 +91,97 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Exclusive scan of a uint32 array of length len in one workgroup; writes the
-// grand total to a[len].
-static __global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ a, int64_t len) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry_s;
-    if (threadIdx.x == 0) carry_s = 0;
+// Device-wide exclusive scan of uint32 a[0..len) with the grand total written
+// to a[len]: per-tile sums (SCAN_TILE contiguous elements per workgroup), a
+// scan of the tile sums in one workgroup, then a per-tile scan with the tile
+// offset.  `part` needs ceil(len / SCAN_TILE) + 1 entries.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+static __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
+    const uint32_t incl = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 63) ws[w] = incl;
     __syncthreads();
-    for (int64_t base = 0; base < len; base += 1024) {
-        int64_t i = base + threadIdx.x;
-        uint32_t v = i < len ? a[i] : 0u;
-        uint32_t incl = wave_incl_scan(v);
-        int w = threadIdx.x >> 6;
-        if (__lane_id() == 63) wsum[w] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t e = 0;
-            for (int k = 0; k < 16; ++k) {  // exclusive wave offsets
-                uint32_t c = wsum[k];
-                wsum[k] = e;
-                e += c;
-            }
-        }
-        __syncthreads();
-        uint32_t carry = carry_s;
-        uint32_t out = carry + wsum[w] + incl - v;
-        if (i < len) a[i] = out;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry_s = out + v;
-        __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        const uint32_t c = ws[k];
+        off += k < w ? c : 0u;
+        tot += c;
     }
-    if (threadIdx.x == 0) a[len] = carry_s;
+    __syncthreads();
+    total = tot;
+    return off + incl - v;
 }
+
+static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t* __restrict__ a,
+                                                                    int64_t len,
+                                                                    uint32_t* __restrict__ part) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t i = base + it * SCAN_THREADS + threadIdx.x;
+        s += i < len ? a[i] : 0u;
+    }
+    uint32_t tot;
+    block_excl_scan(s, ws, tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// single-workgroup scan of a short array (the tile sums); total to a[len]
+static __global__ __launch_bounds__(1024) void k_scan_small(uint32_t* __restrict__ a, int64_t len) {
+    __shared__ uint32_t ws[16];
+    uint32_t carry = 0;
+    for (int64_t b0 = 0; b0 < len; b0 += 1024) {
+        const int64_t i = b0 + threadIdx.x;
+        const uint32_t v = i < len ? a[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, ws, tot);
+        if (i < len) a[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) a[len] = carry;
+}
+
+static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(uint32_t* __restrict__ a,
+                                                                    int64_t len,
+                                                                    const uint32_t* __restrict__ part) {
+    __shared__ uint32_t ws[SCAN_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t i = base + it;
+        v[it] = i < len ? a[i] : 0u;
+        s += v[it];
+    }
+    uint32_t tot;
+    uint32_t run = part[blockIdx.x] + block_excl_scan(s, ws, tot);
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t i = base + it;
+        if (i < len) a[i] = run;
+        run += v[it];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) a[len] = part[gridDim.x];
+}
+
+// Exclusive scan in place; grand total to a[len]; part: scratch (>= tiles + 1)
+static inline void scan_excl(uint32_t* a, int64_t len, uint32_t* part, hipStream_t s) {
+    const int tiles = (int)((len + SCAN_TILE - 1) / SCAN_TILE);
+    if (tiles <= 0) {
+        k_scan_small<<<1, 1024, 0, s>>>(a, 0);
+        return;
+    }
+    k_scan_tiles<<<tiles, SCAN_THREADS, 0, s>>>(a, len, part);
+    k_scan_small<<<1, 1024, 0, s>>>(part, tiles);
+    k_scan_apply<<<tiles, SCAN_THREADS, 0, s>>>(a, len, part);
+}
+
+static inline int64_t scan_part_size(int64_t len) { return (len + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 }  // namespace hq

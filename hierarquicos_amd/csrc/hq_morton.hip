@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_keys_to_orig(int64_t n, const uint64_t*
 int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, int64_t* keys,
                 int* perm, int* L_out, hipStream_t s) {
     int rc;
-    DevBuf par, mm, key, idx, ktmp, vtmp, hist, lmax;
+    DevBuf par, mm, key, idx, ktmp, vtmp, hist, part, lmax;
     if ((rc = par.ensure(16 * sizeof(double))) || (rc = mm.ensure(12 * sizeof(uint64_t))) ||
         (rc = key.ensure(n * 8)) || (rc = idx.ensure(n * 4)) || (rc = ktmp.ensure(n * 8)) ||
         (rc = vtmp.ensure(n * 4)) || (rc = lmax.ensure(sizeof(int))))
@@ -167,7 +167,7 @@ int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, 
     int Lu = L;
     if (L == 0) {
         if ((rc = radix_sort_pairs(key.as<uint64_t>(), idx.as<int>(), ktmp.as<uint64_t>(),
-                                   vtmp.as<int>(), n, 3 * ML, hist, s)))
+                                   vtmp.as<int>(), n, 3 * ML, hist, part, s)))
             return rc;
         k_max_lcp<<<grid_for(n, 256), 256, 0, s>>>(n, key.as<uint64_t>(), lmax.as<int>());
         int mx = 0;
@@ -180,7 +180,7 @@ int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, 
         if (L < 0 || L > ML) return fail(HQ_ERR_DEPTH, "Morton depth %d outside 1..21", L);
         k_shift_keys<<<grid_for(n, 256), 256, 0, s>>>(n, key.as<uint64_t>(), 3 * (ML - L));
         if ((rc = radix_sort_pairs(key.as<uint64_t>(), idx.as<int>(), ktmp.as<uint64_t>(),
-                                   vtmp.as<int>(), n, 3 * L, hist, s)))
+                                   vtmp.as<int>(), n, 3 * L, hist, part, s)))
             return rc;
     }
     k_keys_to_orig<<<grid_for(n, 256), 256, 0, s>>>(n, key.as<uint64_t>(), idx.as<int>(), keys);

@@ -19,7 +19,7 @@ struct hq_tree {
     // inputs, original order
     hq::DevBuf m, x, y, z;
     // keys and sort
-    hq::DevBuf khi, klo, idx, khi2, idx2, hist, geom, l2tab;
+    hq::DevBuf khi, klo, idx, khi2, idx2, hist, geom, l2tab, scanpart;
     // sorted particles (tree order)
     hq::DevBuf sx, sy, sz, sm, sklo, lcp;
     // node store (level order)
@@ -31,7 +31,9 @@ struct hq_tree {
     int most_depth = 0;
     int64_t lost = 0;
     // groups (methods 30-32) and bunches
-    hq::DevBuf groups, gflag;
+    hq::DevBuf groups, gflag, bstart;
+    int super_size = 256;
+    bool fused = true;     // mixed bunched walk as one fused kernel  // bunching: sibling subtrees packed up to this many bodies
     int64_t ngroups = 0;
     // walk scratch
     hq::DevBuf fx, fy, fz, counts, clist, plist, lstat;
@@ -42,7 +44,8 @@ struct hq_tree {
     // last timings / diagnostics
     float ms_keys = 0, ms_sort = 0, ms_nodes = 0, ms_upward = 0, ms_build = 0;
     float ms_walk = 0, ms_lists = 0, ms_eval = 0;
-    int64_t list_cells = 0, list_parts = 0, n_p2p = 0, n_m2p = 0;
+    int64_t list_cells = 0, list_parts = 0, n_p2p = 0, n_m2p = 0, exec_cells = 0, exec_parts = 0,
+            max_cells = 0, max_parts = 0, max_stack = 0;
 };
 
 // status block slots (int64)
@@ -59,6 +62,9 @@ enum {
     ST_P2P = 36,
     ST_M2P = 37,
     ST_NGROUPS = 38,
+    ST_EXECC = 39,     // executed target-cell interactions of the last grouped walk
+    ST_EXECP = 40,     // executed target-particle interactions
+    ST_MAXSTK = 41,    // deepest walk stack of the last grouped walk
     ST_SIZE = 48
 };
 
@@ -70,7 +76,8 @@ int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2);
 int tree_scatter_forces(hq_tree* t, double* F, int layout, int where);
 int read_status(hq_tree* t);
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
-                     DevBuf& hist, hipStream_t s);
+                     DevBuf& hist, DevBuf& part, hipStream_t s);
+int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s);
 int direct_forces(int64_t n, const double* m, const double* x, const double* y, const double* z,
                   int64_t nt, const int64_t* targets, double G, double eps2, int precision,
                   double* F, hipStream_t s);

@@ -180,6 +180,7 @@ enum { MODE_GROUP = 0, MODE_BUNCH = 1, MODE_BUNCH32 = 2 };
 constexpr int SCAP = 4096;   // LDS walk stack (ints) per wavefront
 constexpr int BUNCH = 64;
 constexpr double LIST_GIB_MAX = 48.0;  // device memory allowed for interaction lists
+constexpr int SCAP_FUSED = 1536;       // LDS walk stack of the fused mixed walk
 
 struct ListOut {
     int* cidx;      // MODE_GROUP / MODE_BUNCH: node indices
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
     }
     const double bc[3] = {0.5 * (bmin[0] + bmax[0]), 0.5 * (bmin[1] + bmax[1]),
                           0.5 * (bmin[2] + bmax[2])};
-    int top = 1, nc = 0, np = 0;
+    int top = 1, nc = 0, np = 0, maxtop = 1;
     bool overflow = false;
     if (lane == 0) stk[0] = 0;
     __builtin_amdgcn_wave_barrier();
@@ -309,6 +310,7 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
         const int off = top + incl - nch;
         for (int c = 0; c < nch; ++c) stk[off + c] = cs + c;
         top += tot;
+        maxtop = top > maxtop ? top : maxtop;
         __builtin_amdgcn_wave_barrier();
     }
     if (lane == 0) {
@@ -318,7 +320,12 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
         if (nc > out.ccap || np > out.pcap) atomicOr((unsigned long long*)&status[ST_FLAGS], 8ull);
         atomicAdd((unsigned long long*)&status[ST_LISTC], (unsigned long long)nc);
         atomicAdd((unsigned long long*)&status[ST_LISTP], (unsigned long long)np);
+        const unsigned long long nb = (unsigned long long)(gl - gf + 1);
+        atomicAdd((unsigned long long*)&status[ST_EXECC], (unsigned long long)nc * nb);
+        atomicAdd((unsigned long long*)&status[ST_EXECP],
+                  (unsigned long long)np * nb + nb * (nb - 1));
         atomicMax((long long*)&status[ST_NEEDC], (long long)nc);
+        atomicMax((long long*)&status[ST_MAXSTK], (long long)maxtop);
         atomicMax((long long*)&status[ST_NEEDP], (long long)np);
     }
 }
@@ -595,29 +602,260 @@ __global__ __launch_bounds__(64) void k_eval_mixed(const double* __restrict__ sx
     }
     sum.add(acc);
     const int nc = lstat[2 * gi], np = lstat[2 * gi + 1];
+    // particle list: entries prefetched one ahead
     const float4* __restrict__ pl = p32 + (int64_t)gi * pcap;
-    for (int j0 = 0; j0 < np; j0 += KBLK) {
-        acc[0] = acc[1] = acc[2] = 0.f;
-        const int j1 = std::min(np, j0 + KBLK);
-        for (int j = j0; j < j1; ++j) p2p32(pl[j], tx, ty, tz, eps2, acc);
-        sum.add(acc);
-    }
-    const float4* __restrict__ cl = c32 + (int64_t)gi * ccap;
-    for (int j0 = 0; j0 < nc; j0 += KBLK) {
-        acc[0] = acc[1] = acc[2] = 0.f;
-        const int j1 = std::min(nc, j0 + KBLK);
-        for (int j = j0; j < j1; ++j) {
-            const float4 e = cl[j];
-            const float* __restrict__ r = mom + (int64_t)__float_as_int(e.w) * REC;
-            m2p32<MP>(e, r, tx, ty, tz, eps2, acc);
+    if (np > 0) {
+        float4 e0 = pl[0];
+        for (int j0 = 0; j0 < np; j0 += KBLK) {
+            acc[0] = acc[1] = acc[2] = 0.f;
+            const int j1 = std::min(np, j0 + KBLK);
+            for (int j = j0; j < j1; ++j) {
+                const float4 e1 = pl[j + 1 < np ? j + 1 : j];
+                p2p32(e0, tx, ty, tz, eps2, acc);
+                e0 = e1;
+            }
+            sum.add(acc);
         }
-        sum.add(acc);
+    }
+    // cell list: entries prefetched two ahead, node records one ahead, so the
+    // scalar loads of the next cells overlap this cell's arithmetic
+    const float4* __restrict__ cl = c32 + (int64_t)gi * ccap;
+    if (nc > 0) {
+        float4 e0 = cl[0];
+        float4 e1 = cl[nc > 1 ? 1 : 0];
+        float r0[REC];
+        {
+            const float4* rp = reinterpret_cast<const float4*>(mom + (int64_t)__float_as_int(e0.w) * REC);
+#pragma unroll
+            for (int q = 0; q < REC / 4; ++q) {
+                const float4 v4 = rp[q];
+                r0[4 * q] = v4.x; r0[4 * q + 1] = v4.y; r0[4 * q + 2] = v4.z; r0[4 * q + 3] = v4.w;
+            }
+        }
+        for (int j0 = 0; j0 < nc; j0 += KBLK) {
+            acc[0] = acc[1] = acc[2] = 0.f;
+            const int j1 = std::min(nc, j0 + KBLK);
+            for (int j = j0; j < j1; ++j) {
+                const float4 e2 = cl[j + 2 < nc ? j + 2 : nc - 1];
+                float r1[REC];
+                const float4* rp = reinterpret_cast<const float4*>(mom + (int64_t)__float_as_int(e1.w) * REC);
+#pragma unroll
+                for (int q = 0; q < REC / 4; ++q) {
+                    const float4 v4 = rp[q];
+                    r1[4 * q] = v4.x; r1[4 * q + 1] = v4.y; r1[4 * q + 2] = v4.z; r1[4 * q + 3] = v4.w;
+                }
+                m2p32<MP>(e0, r0, tx, ty, tz, eps2, acc);
+                e0 = e1;
+                e1 = e2;
+#pragma unroll
+                for (int q = 0; q < REC; ++q) r0[q] = r1[q];
+            }
+            sum.add(acc);
+        }
     }
     if (lane < nb) {
         const double gm = G * sm[i];
         fx[i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
         fy[i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
         fz[i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
+    }
+}
+
+// ------------------------------------------------------------------ fused mixed walk
+// One wavefront per bunch walks the tree (as k_lists<MODE_BUNCH32>) and
+// evaluates accepted cells and particles in chunks of 64 as they arrive,
+// from two LDS ring buffers: no interaction lists in global memory, and the
+// latency-bound walk of some wavefronts overlaps the arithmetic of others.
+constexpr int RING = 128;
+
+template <int MP>
+__device__ __forceinline__ void load_rec(const float4* __restrict__ ring, int pos,
+                                         const float* __restrict__ mom, float4& e, float r[REC]) {
+    e = ring[pos & (RING - 1)];
+    const int node = __builtin_amdgcn_readfirstlane(__float_as_int(e.w));
+    const float4* __restrict__ rp = reinterpret_cast<const float4*>(mom + (int64_t)node * REC);
+    constexpr int NQ = MP == 1 ? 1 : (MP == 4 ? 2 : REC / 4);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const float4 v4 = rp[q];
+        r[4 * q] = v4.x;
+        r[4 * q + 1] = v4.y;
+        r[4 * q + 2] = v4.z;
+        r[4 * q + 3] = v4.w;
+    }
+}
+
+template <int MP>
+__device__ __forceinline__ void eval_cells(const float4* __restrict__ ring, int head, int cnt,
+                                           const float* __restrict__ mom, float tx, float ty,
+                                           float tz, float eps2, Kahan3& sum) {
+    float acc[3] = {0.f, 0.f, 0.f};
+    float4 eA, eB;
+    float rA[REC], rB[REC];
+    load_rec<MP>(ring, head, mom, eA, rA);
+    for (int j = 0; j < cnt; j += 2) {
+        const bool two = j + 1 < cnt;
+        if (two) load_rec<MP>(ring, head + j + 1, mom, eB, rB);
+        m2p32<MP>(eA, rA, tx, ty, tz, eps2, acc);
+        if (two) {
+            if (j + 2 < cnt) load_rec<MP>(ring, head + j + 2, mom, eA, rA);
+            m2p32<MP>(eB, rB, tx, ty, tz, eps2, acc);
+        }
+    }
+    sum.add(acc);
+}
+
+__device__ __forceinline__ void eval_parts(const float4* __restrict__ ring, int head, int cnt,
+                                           float tx, float ty, float tz, float eps2, Kahan3& sum) {
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int j = 0; j < cnt; ++j) p2p32(ring[(head + j) & (RING - 1)], tx, ty, tz, eps2, acc);
+    sum.add(acc);
+}
+
+template <int MP, int SC>
+__global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __restrict__ g,
+                                                   const double* __restrict__ sx,
+                                                   const double* __restrict__ sy,
+                                                   const double* __restrict__ sz,
+                                                   const double* __restrict__ sm, int64_t n,
+                                                   const int* __restrict__ bstart, int nbunch,
+                                                   double theta2, double G, float eps2,
+                                                   const float* __restrict__ mom,
+                                                   double* __restrict__ fx, double* __restrict__ fy,
+                                                   double* __restrict__ fz,
+                                                   int64_t* __restrict__ status) {
+    __shared__ int stk[SC];
+    __shared__ float4 cring[RING], pring[RING];
+    __shared__ float4 own[BUNCH];
+    const int gi = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (gi >= nbunch) return;
+    const int lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    const int gf = bstart[gi], gl = bstart[gi + 1] - 1;
+    const int nb = gl - gf + 1;
+    const int i = gf + (lane < nb ? lane : 0);
+    double bmin[3], bmax[3];
+    {
+        double v[3] = {sx[i], sy[i], sz[i]}, w[3] = {v[0], v[1], v[2]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                v[c] = fmin(v[c], __shfl_xor(v[c], o, 64));
+                w[c] = fmax(w[c], __shfl_xor(w[c], o, 64));
+            }
+            bmin[c] = v[c];
+            bmax[c] = w[c];
+        }
+    }
+    const double bc[3] = {0.5 * (bmin[0] + bmax[0]), 0.5 * (bmin[1] + bmax[1]),
+                          0.5 * (bmin[2] + bmax[2])};
+    const float tx = (float)(sx[i] - bc[0]), ty = (float)(sy[i] - bc[1]), tz = (float)(sz[i] - bc[2]);
+    own[lane] = make_float4(tx, ty, tz, lane < nb ? (float)sm[i] : 0.0f);
+    Kahan3 sum;
+    int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0;
+    long long ntot_c = 0, ntot_p = 0;
+    bool overflow = false;
+    if (lane == 0) stk[0] = 0;
+    __builtin_amdgcn_wave_barrier();
+    while (top > 0) {
+        const int take = top < 64 ? top : 64;
+        const int base = top - take;
+        const int k = lane < take ? stk[base + lane] : -1;
+        top = base;
+        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell
+        if (k >= 0) {
+            const double mass = nv.mass[k];
+            if (mass != 0.0) {
+                const int f = nv.first[k];
+                const int l = f + nv.count[k] - 1;
+                const bool leaf = nv.type[k] == 1;
+                if (f <= gl && l >= gf) {
+                    action = leaf ? 0 : 1;
+                } else if (leaf) {
+                    action = 2;
+                } else {
+                    const double cc[3] = {nv.cx[k], nv.cy[k], nv.cz[k]};
+                    double dist2 = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double dd = fmax(fmax(bmin[c] - cc[c], 0.0), cc[c] - bmax[c]);
+                        dist2 = fma(dd, dd, dist2);
+                    }
+                    const int depth = nv.depth[k];
+                    action = (g[G_L2 + depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                }
+            }
+        }
+        const uint64_t mp = __ballot(action == 2);
+        if (action == 2) {
+            const int p = nv.first[k];
+            pring[(phead + pcnt + __popcll(mp & lt)) & (RING - 1)] =
+                make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]), (float)(sz[p] - bc[2]),
+                            (float)sm[p]);
+        }
+        pcnt += __popcll(mp);
+        const uint64_t mc = __ballot(action == 3);
+        if (action == 3)
+            cring[(chead + ccnt + __popcll(mc & lt)) & (RING - 1)] =
+                make_float4((float)(nv.cx[k] - bc[0]), (float)(nv.cy[k] - bc[1]),
+                            (float)(nv.cz[k] - bc[2]), __int_as_float(k));
+        ccnt += __popcll(mc);
+        const int cs = action == 1 ? nv.cstart[k] : 0;
+        const int nch = action == 1 ? nv.cstart[k + 1] - cs : 0;
+        const int incl = wave_incl_scan(nch);
+        const int tot = __shfl(incl, 63, 64);
+        if (top + tot > SC) {
+            overflow = true;
+            break;
+        }
+        const int off = top + incl - nch;
+        for (int c = 0; c < nch; ++c) stk[off + c] = cs + c;
+        top += tot;
+        maxtop = top > maxtop ? top : maxtop;
+        __builtin_amdgcn_wave_barrier();
+        if (ccnt >= 64) {
+            eval_cells<MP>(cring, chead, 64, mom, tx, ty, tz, eps2, sum);
+            chead += 64;
+            ccnt -= 64;
+            ntot_c += 64;
+        }
+        if (pcnt >= 64) {
+            eval_parts(pring, phead, 64, tx, ty, tz, eps2, sum);
+            phead += 64;
+            pcnt -= 64;
+            ntot_p += 64;
+        }
+    }
+    if (!overflow) {
+        if (ccnt > 0) eval_cells<MP>(cring, chead, ccnt, mom, tx, ty, tz, eps2, sum);
+        if (pcnt > 0) eval_parts(pring, phead, pcnt, tx, ty, tz, eps2, sum);
+        ntot_c += ccnt;
+        ntot_p += pcnt;
+        float acc[3] = {0.f, 0.f, 0.f};
+        for (int j = 0; j < nb; ++j) {
+            float4 e = own[j];
+            if (j == lane) e.w = 0.0f, e.x = tx + 1.0f;  // zero mass, non-zero distance
+            p2p32(e, tx, ty, tz, eps2, acc);
+        }
+        sum.add(acc);
+        if (lane < nb) {
+            const double gm = G * sm[i];
+            fx[i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
+            fy[i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
+            fz[i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
+        }
+    }
+    if (lane == 0) {
+        if (overflow) atomicOr((unsigned long long*)&status[ST_FLAGS], 16ull);
+        const unsigned long long unb = (unsigned long long)nb;
+        atomicAdd((unsigned long long*)&status[ST_LISTC], (unsigned long long)ntot_c);
+        atomicAdd((unsigned long long*)&status[ST_LISTP], (unsigned long long)ntot_p);
+        atomicAdd((unsigned long long*)&status[ST_EXECC], (unsigned long long)ntot_c * unb);
+        atomicAdd((unsigned long long*)&status[ST_EXECP], (unsigned long long)ntot_p * unb + unb * (unb - 1));
+        atomicMax((long long*)&status[ST_NEEDC], (long long)ntot_c);
+        atomicMax((long long*)&status[ST_NEEDP], (long long)ntot_p);
+        atomicMax((long long*)&status[ST_MAXSTK], (long long)maxtop);
     }
 }
 
@@ -718,6 +956,28 @@ __global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, i
     }
 }
 
+// split each super-group [sg[i], sg[i+1]) into ceil(c / B) bunches of near-equal
+// size (contiguous in tree order)
+__global__ __launch_bounds__(256) void k_bunch_count(const int* __restrict__ sg, int nsg, int B,
+                                                     uint32_t* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsg) return;
+    const int c = sg[i + 1] - sg[i];
+    cnt[i] = (uint32_t)((c + B - 1) / B);
+}
+
+__global__ __launch_bounds__(256) void k_bunch_emit(const int* __restrict__ sg, int nsg, int B,
+                                                    const uint32_t* __restrict__ off,
+                                                    int* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsg) return;
+    const int c = sg[i + 1] - sg[i];
+    const int nb = (c + B - 1) / B;
+    const int base = c / nb, rem = c % nb;
+    const uint32_t o = off[i];
+    for (int j = 0; j < nb; ++j) out[o + j] = sg[i] + j * base + (j < rem ? j : rem);
+}
+
 __global__ __launch_bounds__(256) void k_flag_count(const int* __restrict__ gmark, int64_t n,
                                                     uint32_t* __restrict__ bcnt) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -758,6 +1018,38 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
         k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
                                                      t->mom32.as<float>());
+        if (t->fused) {
+            HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+            HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
+            auto launch = [&](auto kern) {
+                kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>(), t->sx.as<double>(),
+                                            t->sy.as<double>(), t->sz.as<double>(),
+                                            t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
+                                            theta2, G, (float)eps2, t->mom32.as<float>(),
+                                            t->fx.as<double>(), t->fy.as<double>(),
+                                            t->fz.as<double>(), t->status.as<int64_t>());
+            };
+            if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
+            else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
+            else launch(k_walk_mixed<8, SCAP_FUSED>);
+            HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
+            if ((rc = read_status(t))) return rc;
+            HQ_CHECK_HIP(hipGetLastError());
+            if (!(t->hstatus[ST_FLAGS] & 16)) {
+                hipEventElapsedTime(&t->ms_eval, t->ev[5], t->ev[8]);
+                t->ms_lists = 0.0f;
+                t->ms_walk = t->ms_eval;
+                t->list_cells = t->hstatus[ST_LISTC];
+                t->list_parts = t->hstatus[ST_LISTP];
+                t->exec_cells = t->hstatus[ST_EXECC];
+                t->exec_parts = t->hstatus[ST_EXECP];
+                t->max_cells = t->hstatus[ST_NEEDC];
+                t->max_parts = t->hstatus[ST_NEEDP];
+                t->max_stack = t->hstatus[ST_MAXSTK];
+                return HQ_OK;
+            }
+            // the fused kernel's stack overflowed: redo with the two-kernel walk
+        }
     }
     if ((rc = t->lstat.ensure((size_t)ngroups * 2 * sizeof(int)))) return rc;
     if (t->list_ccap == 0) t->list_ccap = 4096;
@@ -787,13 +1079,13 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         else if (!mixed)
             k_lists<MODE_BUNCH><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(), t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
-                                                       t->sm.as<double>(), t->n, t->groups.as<int>(),
+                                                       t->sm.as<double>(), t->n, t->bstart.as<int>(),
                                                        ngroups, theta2, lo, t->status.as<int64_t>());
         else
             k_lists<MODE_BUNCH32><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(),
                                                          t->sx.as<double>(), t->sy.as<double>(),
                                                          t->sz.as<double>(), t->sm.as<double>(),
-                                                         t->n, t->groups.as<int>(), ngroups, theta2,
+                                                         t->n, t->bstart.as<int>(), ngroups, theta2,
                                                          lo, t->status.as<int64_t>());
         HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
         if ((rc = read_status(t))) return rc;
@@ -811,6 +1103,11 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         }
         t->list_cells = t->hstatus[ST_LISTC];
         t->list_parts = t->hstatus[ST_LISTP];
+        t->exec_cells = t->hstatus[ST_EXECC];
+        t->exec_parts = t->hstatus[ST_EXECP];
+        t->max_cells = t->hstatus[ST_NEEDC];
+        t->max_parts = t->hstatus[ST_NEEDP];
+        t->max_stack = t->hstatus[ST_MAXSTK];
         if (!bunched)
             k_eval_fp64<MODE_GROUP><<<ngroups, 64, 0, s>>>(
                 nv, t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(),
@@ -819,13 +1116,13 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         else if (!mixed)
             k_eval_fp64<MODE_BUNCH><<<ngroups, 64, 0, s>>>(
                 nv, t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(),
-                t->n, t->groups.as<int>(), ngroups, t->multipole, G, eps2, lo, t->fx.as<double>(),
+                t->n, t->bstart.as<int>(), ngroups, t->multipole, G, eps2, lo, t->fx.as<double>(),
                 t->fy.as<double>(), t->fz.as<double>());
         else {
             auto launch = [&](auto kern) {
                 kern<<<ngroups, 64, 0, s>>>(t->sx.as<double>(), t->sy.as<double>(),
                                             t->sz.as<double>(), t->sm.as<double>(), t->n,
-                                            t->groups.as<int>(), ngroups,
+                                            t->bstart.as<int>(), ngroups,
                                             G, (float)eps2, lo.c32, lo.p32, lo.lstat, lo.ccap,
                                             lo.pcap, t->mom32.as<float>(), t->fx.as<double>(),
                                             t->fy.as<double>(), t->fz.as<double>());
@@ -857,9 +1154,10 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     if ((rc = t->gflag.ensure(n * sizeof(int))) || (rc = t->groups.ensure((n + 1) * sizeof(int))))
         return rc;
     HQ_CHECK_HIP(hipMemsetAsync(t->gflag.p, 0xFF, n * sizeof(int), s));
+    const int B = std::min(t->group_size, BUNCH);
+    const int S = std::max(B, t->super_size);
     if (bunched)
-        k_mark_bunches<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nn,
-                                                           std::min(t->group_size, BUNCH),
+        k_mark_bunches<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nn, S,
                                                            t->gflag.as<int>());
     else
         k_mark_groups<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nparent.as<int>(),
@@ -867,7 +1165,7 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     const int nb = grid_for(n, 256);
     if ((rc = t->hist.ensure(((size_t)nb + 1) * sizeof(uint32_t)))) return rc;
     k_flag_count<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>());
-    k_scan_excl<<<1, 1024, 0, s>>>(t->hist.as<uint32_t>(), nb);
+    if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), nb, t->scanpart, s))) return rc;
     k_flag_write<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>(),
                                     t->groups.as<int>());
     uint32_t total = 0;
@@ -876,10 +1174,24 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     HQ_CHECK_HIP(hipStreamSynchronize(s));
     t->ngroups = total;
     if (bunched) {
+        // super-groups -> bunches of <= B particles
         const int end = (int)n;
-        HQ_CHECK_HIP(hipMemcpyAsync(t->groups.as<int>() + total, &end, sizeof(int),
+        const int nsg = (int)total;
+        int* sg = t->groups.as<int>();
+        HQ_CHECK_HIP(hipMemcpyAsync(sg + nsg, &end, sizeof(int), hipMemcpyHostToDevice, s));
+        if ((rc = t->gflag.ensure(((size_t)2 * nsg + 2 + n) * sizeof(int)))) return rc;
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(t->gflag.as<int>());
+        k_bunch_count<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt);
+        if ((rc = scan_excl_buf(cnt, nsg, t->scanpart, s))) return rc;
+        uint32_t nbun = 0;
+        HQ_CHECK_HIP(hipMemcpyAsync(&nbun, cnt + nsg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+        if ((rc = t->bstart.ensure(((size_t)nbun + 1) * sizeof(int)))) return rc;
+        k_bunch_emit<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt, t->bstart.as<int>());
+        HQ_CHECK_HIP(hipMemcpyAsync(t->bstart.as<int>() + nbun, &end, sizeof(int),
                                     hipMemcpyHostToDevice, s));
         HQ_CHECK_HIP(hipStreamSynchronize(s));
+        t->ngroups = nbun;
     }
     return HQ_OK;
 }

@@ -120,6 +120,18 @@ class OctreeType:
             return lib.hq_tree_forces_bunched(h, th2, g, e2, p, layout, where, prec)
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
+    def tune(self, bunch=None, super_size=None, fused=None):
+        """Bunch size (<= 64), super-group size and fused/two-kernel mode of the
+        bunched walk."""
+        lib = L.lib()
+        if fused is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FUSED, int(bool(fused))))
+        if bunch is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_BUNCH, int(bunch)))
+        if super_size is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_SUPER, int(super_size)))
+        return self
+
     # -- introspection ----------------------------------------------------------
     def info(self):
         inf = L.TreeInfo()

@@ -93,6 +93,15 @@ int hq_tree_forces_bunched(hq_tree *t, double theta2, double G, double eps2, dou
                            int layout, int where, int precision);
 int hq_tree_destroy(hq_tree *t);
 
+/* Tuning knobs (no reference counterpart): HQ_TUNE_BUNCH sets the bunch size
+ * of hq_tree_forces_bunched (1..64, default 64; also the group_size of
+ * methods 30-32), HQ_TUNE_SUPER the size up to which sibling subtrees are
+ * packed before being cut into bunches (default 256). */
+#define HQ_TUNE_BUNCH 1
+#define HQ_TUNE_SUPER 2
+#define HQ_TUNE_FUSED 3   /* 1 (default): walk and evaluate in one kernel; 0: two kernels */
+int hq_tree_tune(hq_tree *t, int key, int64_t value);
+
 typedef struct {
     int64_t n;
     int64_t nnodes;        /* number_of_nodes */
@@ -109,6 +118,11 @@ typedef struct {
     double ms_walk;        /* device time of the last force walk kernel(s) */
     double ms_lists;       /* list building part of the last grouped walk */
     double ms_eval;        /* evaluation part of the last grouped walk */
+    int64_t exec_cells;    /* target-cell interactions the last grouped walk evaluated */
+    int64_t exec_parts;    /* target-particle interactions (lists + in-group) */
+    int64_t max_cells;     /* longest cell list of the last grouped walk */
+    int64_t max_parts;     /* longest particle list */
+    int64_t max_stack;     /* deepest walk stack */
 } hq_tree_info_t;
 int hq_tree_info(const hq_tree *t, hq_tree_info_t *info);
 
