@@ -164,13 +164,21 @@ def main():
         }
 
     eval_ms = float(np.mean(evals))
+    kernel = "k_eval_mixed<8>" if args.unfused else "k_walk_mixed<8,1536>"
+    workload = f"plummer_n{N}_octupole_theta{args.theta}_eps{args.eps}"
+    traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
+    tf = os.path.join(REPO, "profiles", f"traffic_k_walk_mixed_{workload.split('_theta')[0]}.json")
+    if os.path.exists(tf) and not args.unfused:
+        rec = json.load(open(tf))
+        if rec.get("workload") == workload and args.method == 12:
+            traffic = rec["hbm_bytes_per_launch"]
     roof = None
     if flops_alg is not None:
         achieved = flops_alg / (eval_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "compute_unit": "VALU (FP32 FMA/rsqrt; no matrix ops)",
-                "kernel": "k_eval_mixed<8>" if args.unfused else "k_walk_mixed<8,1536>",
+                "kernel": kernel,
                 "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                 "flops_per_launch": flops_alg, "kernel_ms": eval_ms,
                 "list_entries": {"cells": info["list_cells"], "particles": info["list_parts"]},
                 "executed_interactions": {"cells": info["exec_cells"],
