@@ -29,7 +29,7 @@ int fail(int code, const char* fmt, ...) {
 }
 
 int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, int64_t* keys,
-                int* perm, int* L_out, hipStream_t s);
+                int* perm, int* L_out, hipStream_t s, double* par_out = nullptr);
 
 static int multipole_of(int method) {
     switch (method) {

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_keys_to_orig(int64_t n, const uint64_t*
 }
 
 int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, int64_t* keys,
-                int* perm, int* L_out, hipStream_t s) {
+                int* perm, int* L_out, hipStream_t s, double* par_out) {
     int rc;
     DevBuf par, mm, key, idx, ktmp, vtmp, hist, part, lmax;
     if ((rc = par.ensure(16 * sizeof(double))) || (rc = mm.ensure(12 * sizeof(uint64_t))) ||
@@ -185,6 +185,8 @@ int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, 
     }
     k_keys_to_orig<<<grid_for(n, 256), 256, 0, s>>>(n, key.as<uint64_t>(), idx.as<int>(), keys);
     HQ_CHECK_HIP(hipMemcpyAsync(perm, idx.p, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (par_out)  // qcm (3), mass, ssa, normalised-box centre (3)
+        HQ_CHECK_HIP(hipMemcpyAsync(par_out, par.p, 8 * sizeof(double), hipMemcpyDeviceToDevice, s));
     HQ_CHECK_HIP(hipStreamSynchronize(s));
     HQ_CHECK_HIP(hipGetLastError());
     *L_out = Lu;

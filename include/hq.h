@@ -163,6 +163,22 @@ int hq_dev_free(void *p);
 int hq_dev_copy(void *dst, const void *src, int64_t bytes, int kind);
 int hq_dev_sync(void);
 
+/* morton_tree_type (morton_octree.f90:49-180): init(N, ms, qs, L, ssa,
+ * use_quadrupole) = create + build; forces_seq/forces_par = forces.  q is
+ * (3,N); L = 0 chooses the depth; forces are returned in original order. */
+typedef struct hq_morton hq_morton;
+int hq_morton_create(hq_morton **out, int64_t n, int L, double ssa, int use_quad);
+int hq_morton_build(hq_morton *t, const double *m, const double *q, int where);
+int hq_morton_forces(hq_morton *t, double G, double eps2, double theta2, int mult, double *F,
+                     int layout, int where);
+int hq_morton_info(const hq_morton *t, int *L, int64_t *nnodes);
+/* Node store in the reference's numbering (level by level); 0-based indices,
+ * first_child = -1 for leaves; any pointer may be NULL. */
+int hq_morton_export(const hq_morton *t, int *perm, int64_t *prefix, int *level, int *first,
+                     int *last, int *parent, int *nchild, int *first_child, int *next,
+                     double *mass, double *qcm /* 3*nn */, double *quad /* 6*nn */);
+int hq_morton_destroy(hq_morton *t);
+
 const char *hq_last_error(void);
 const char *hq_version(void);
 

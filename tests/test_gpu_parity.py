@@ -258,3 +258,52 @@ def test_dehnen_deep_tree_and_momentum(golden):
         assert relmax(F, Fo) < FP64_TOL
         if method < 22:  # mutual M2L conserves momentum (Marcello's term targets torque)
             assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max() * len(m)
+
+
+# ---------------------------------------------------------------- Morton tree
+def test_morton_tree_node_store_matches_reference(golden):
+    from hierarquicos_amd import MortonTreeType
+    d = golden("points10.npz")
+    t = MortonTreeType().init(10, d["m"], d["q"].T, 0, 1.2, True)
+    e = t.export()
+    assert t.depth == int(d["morton_L"])
+    nn = len(d["morton_level"])
+    assert t.nnodes == nn
+    for k in ("perm", "prefix", "level", "first", "last", "parent", "nchild", "next"):
+        assert np.array_equal(e[k], d["morton_" + k]), k
+    child = np.full((nn, 8), -1)
+    for k in range(nn):
+        for j in range(e["nchild"][k]):
+            child[k, j] = e["first_child"][k] + j
+    assert np.array_equal(child, d["morton_child"])
+    # node sums run from the children upward (the reference sums particles):
+    # equal to rounding
+    assert np.allclose(e["mass"], d["morton_mass"], rtol=1e-14, atol=0)
+    assert np.allclose(e["qcm"], d["morton_qcm"], rtol=1e-13, atol=1e-15)
+    assert np.allclose(e["quad"], d["morton_quad"], rtol=1e-12, atol=1e-14)
+    F = t.forces_par(1.0, 1e-4, 0.01).T
+    assert relmax(F, d["F_morton"]) < FP64_TOL
+
+
+def test_morton_tree_forces_match_reference(golden):
+    from hierarquicos_amd import MortonTreeType
+    d = golden("plummer_n1000_seed3.npz")
+    q = np.stack([d["x"], d["y"], d["z"]])
+    t = MortonTreeType().init(1000, d["m"], q, 0, 1.2, True)
+    assert t.depth == int(d["morton_L"]) and t.nnodes == int(d["morton_nnodes"])
+    e = t.export()
+    assert np.array_equal(e["perm"], d["morton_perm"])
+    F = t.forces_seq(1.0, float(d["eps2"]), float(d["theta2"])).T
+    assert relmax(F, d["F_morton"]) < FP64_TOL
+
+
+def test_morton_tree_larger_n_matches_oracle():
+    from hierarquicos_amd import MortonTreeType
+    m, q, _ = ics.plummer(20000, seed=16)
+    for L, mult in ((0, True), (9, True), (9, False)):
+        t = MortonTreeType().init(20000, m, q.T, L, 1.2, True)
+        F = t.forces_par(1.0, 1e-6, 0.3, mult).T
+        Mo = O.Morton(m, q, L, 1.2, True)
+        assert t.depth == Mo.L and t.nnodes == Mo.nnodes
+        Fo = Mo.forces(1.0, 1e-6, 0.3, mult)
+        assert relmax(F, Fo) < FP64_TOL
