@@ -31,6 +31,7 @@ _vp = C.c_void_p
 
 EXPORTS = (
     "hq_compute_forces_direct", "hq_compute_forces", "hq_generate_tree", "hq_tree_create",
+    "hq_tree_create_batch",
     "hq_tree_set_masses", "hq_tree_build", "hq_tree_forces", "hq_tree_forces_bunched",
     "hq_tree_destroy", "hq_tree_tune", "hq_tree_info", "hq_tree_export", "hq_tree_order",
     "hq_direct_forces_device", "hq_morton_keys", "hq_morton_create", "hq_morton_build",
@@ -76,6 +77,7 @@ def lib():
                                     C.c_double, C.c_int, C.c_int, _d]
     L.hq_generate_tree.argtypes = [C.c_int64, _d, _d, _d, _d, C.c_int]
     L.hq_tree_create.argtypes = [C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_int]
+    L.hq_tree_create_batch.argtypes = [C.POINTER(_vp), C.c_int, _i64, C.c_int, C.c_int, C.c_int]
     L.hq_tree_set_masses.argtypes = [_vp, _vp, C.c_int]
     L.hq_tree_build.argtypes = [_vp, _vp, _vp, _vp, C.c_int]
     L.hq_tree_forces.argtypes = [_vp, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int]

@@ -108,6 +108,33 @@ int hq_tree_create(hq_tree** out, int64_t n, int method, int group_size, int pre
     return HQ_OK;
 }
 
+int hq_tree_create_batch(hq_tree** out, int nsys, const int64_t* counts, int method,
+                         int group_size, int precision) {
+    if (!out) return fail(HQ_ERR_ARGS, "null handle pointer");
+    *out = nullptr;
+    if (nsys < 1 || !counts) return fail(HQ_ERR_ARGS, "batch needs at least one system");
+    std::vector<int64_t> off(nsys + 1, 0);
+    for (int k = 0; k < nsys; ++k) {
+        if (counts[k] < 1)
+            return fail(HQ_ERR_ARGS, "system %d of the batch has %lld particles", k, (long long)counts[k]);
+        off[k + 1] = off[k] + counts[k];
+        if (off[k + 1] > (int64_t)1 << 30)
+            return fail(HQ_ERR_ARGS, "batch holds more than 2^30 particles");
+    }
+    hq_tree* t = nullptr;
+    int rc = hq_tree_create(&t, off[nsys], method, group_size, precision);
+    if (rc) return rc;
+    std::unique_ptr<hq_tree, int (*)(hq_tree*)> guard(t, hq_tree_destroy);
+    t->nsys = nsys;
+    t->hsysoff = off;
+    if ((rc = t->sysoff.ensure((nsys + 1) * sizeof(int64_t)))) return rc;
+    HQ_CHECK_HIP(hipMemcpyAsync(t->sysoff.p, off.data(), (nsys + 1) * sizeof(int64_t),
+                                hipMemcpyHostToDevice, t->stream));
+    HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    *out = guard.release();
+    return HQ_OK;
+}
+
 int hq_tree_destroy(hq_tree* t) {
     if (!t) return HQ_OK;
     if (t->stream) hipStreamSynchronize(t->stream);

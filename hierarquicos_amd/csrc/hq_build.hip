@@ -32,8 +32,7 @@
 namespace hq {
 
 // geom buffer layout (doubles)
-enum { G_CX = 0, G_CY, G_CZ, G_SIDE, G_HALF, G_XMIN, G_XMAX, G_YMIN, G_YMAX, G_ZMIN, G_ZMAX,
-       G_L2 = 16, G_HALFD = 48, G_SIZE = 80 };
+enum { G_CX = 0, G_CY, G_CZ, G_SIDE, G_HALF, G_XMIN, G_XMAX, G_YMIN, G_YMAX, G_ZMIN, G_ZMAX };
 
 // ------------------------------------------------------------------ bbox
 __global__ __launch_bounds__(256) void k_bbox_partial(const double* __restrict__ x,
@@ -103,19 +102,80 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
         g[G_XMIN] = mn[0]; g[G_XMAX] = mx[0];
         g[G_YMIN] = mn[1]; g[G_YMAX] = mx[1];
         g[G_ZMIN] = mn[2]; g[G_ZMAX] = mx[2];
+        double* g0 = g + G_SYS;  // system 0 of the per-system table
+        g0[S_CX] = g[G_CX]; g0[S_CY] = g[G_CY]; g0[S_CZ] = g[G_CZ];
+        g0[S_SIDE] = side; g0[S_HALF] = side / 2.0;
+    }
+}
+
+// One workgroup per system of a batch: the system's root cube (same
+// arithmetic as k_bbox_final) into sg[SG * s + ...].
+__global__ __launch_bounds__(256) void k_sys_bbox(const double* __restrict__ x,
+                                                  const double* __restrict__ y,
+                                                  const double* __restrict__ z,
+                                                  const int64_t* __restrict__ off,
+                                                  double* __restrict__ sg,
+                                                  int64_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    const int sys = blockIdx.x;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool bad = false;
+    for (int64_t i = off[sys] + threadIdx.x; i < off[sys + 1]; i += blockDim.x) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int c = 0; c < 3; ++c) {
+            bad |= !isfinite(v[c]);
+            mn[c] = fmin(mn[c], v[c]);
+            mx[c] = fmax(mx[c], v[c]);
+        }
+    }
+    __shared__ double sh[4][6];
+    for (int c = 0; c < 3; ++c)
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[c] = fmin(mn[c], __shfl_xor(mn[c], o, 64));
+            mx[c] = fmax(mx[c], __shfl_xor(mx[c], o, 64));
+        }
+    if (__any(bad) && __lane_id() == 0) atomicOr((unsigned long long*)&status[ST_FLAGS], 1ull);
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0)
+        for (int c = 0; c < 3; ++c) { sh[w][2 * c] = mn[c]; sh[w][2 * c + 1] = mx[c]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k)
+            for (int c = 0; c < 3; ++c) {
+                mn[c] = fmin(mn[c], sh[k][2 * c]);
+                mx[c] = fmax(mx[c], sh[k][2 * c + 1]);
+            }
+        double* g = sg + SG * sys;
+        g[S_CX] = 0.5 * (mn[0] + mx[0]);
+        g[S_CY] = 0.5 * (mn[1] + mx[1]);
+        g[S_CZ] = 0.5 * (mn[2] + mx[2]);
+        double ext = mx[0] - mn[0];
+        ext = fmax(ext, mx[1] - mn[1]);
+        ext = fmax(ext, mx[2] - mn[2]);
+        const double side = 1.2 * ext;
+        g[S_SIDE] = side;
+        g[S_HALF] = side / 2.0;
     }
 }
 
 // A depth-d node has side ldexp(side, -d): add_node gets the parent's
-// halfside as its side (:508), and L2 = side^2 (:475).
-__global__ void k_geom_tables(double* __restrict__ g) {
+// halfside as its side (:508), and L2 = side^2 (:475).  One workgroup per system.
+__global__ void k_geom_tables(double* __restrict__ sg) {
 #pragma clang fp contract(off)
-    int d = threadIdx.x;
+    double* g = sg + SG * blockIdx.x;
+    const int d = threadIdx.x;
     if (d <= MAX_DEPTH) {
-        double s = ldexp(g[G_SIDE], -d);
-        g[G_L2 + d] = s * s;
-        g[G_HALFD + d] = s / 2.0;
+        const double sd = ldexp(g[S_SIDE], -d);
+        g[S_L2 + d] = sd * sd;
+        g[S_HALFD + d] = sd / 2.0;
     }
+}
+
+// system id of every particle of a batch (original order)
+__global__ __launch_bounds__(256) void k_sysid(const int64_t* __restrict__ off,
+                                               int* __restrict__ sysid) {
+    const int sys = blockIdx.x;
+    for (int64_t i = off[sys] + threadIdx.x; i < off[sys + 1]; i += blockDim.x) sysid[i] = sys;
 }
 
 // ------------------------------------------------------------------ keys
@@ -126,15 +186,18 @@ __global__ void k_geom_tables(double* __restrict__ g) {
 __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
                                               const double* __restrict__ y,
                                               const double* __restrict__ z, int64_t n,
-                                              const double* __restrict__ g,
+                                              const double* __restrict__ sg,
+                                              const int* __restrict__ sysid,
                                               uint64_t* __restrict__ khi,
                                               uint32_t* __restrict__ klo,
-                                              int* __restrict__ idx) {
+                                              int* __restrict__ idx,
+                                              uint64_t* __restrict__ khio) {
 #pragma clang fp contract(off)
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     double px = x[p], py = y[p], pz = z[p];
-    double cx = g[G_CX], cy = g[G_CY], cz = g[G_CZ], h = g[G_HALF];
+    const double* g = sg + SG * (sysid ? sysid[p] : 0);
+    double cx = g[S_CX], cy = g[S_CY], cz = g[S_CZ], h = g[S_HALF];
     uint64_t hi = 0;
     uint32_t lo = 0;
 #pragma unroll
@@ -154,6 +217,7 @@ __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
     khi[p] = hi;
     klo[p] = lo;
     idx[p] = (int)p;
+    if (khio) khio[p] = hi;
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -244,13 +308,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 // Runs of equal high words (particles sharing levels 1..21) are ordered by
 // the low word (levels 22..25), stably.  Such runs are rare and short.
 __global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __restrict__ klo,
-                          int* __restrict__ idx, int64_t n, int64_t* __restrict__ status) {
+                          int* __restrict__ idx, const int* __restrict__ sysid, int64_t n,
+                          int64_t* __restrict__ status) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i + 1 >= n) return;
-    if (khi[i] != khi[i + 1]) return;
-    if (i > 0 && khi[i - 1] == khi[i]) return;
+    auto same = [&](int64_t a, int64_t b) {
+        return khi[a] == khi[b] && (!sysid || sysid[idx[a]] == sysid[idx[b]]);
+    };
+    if (!same(i, i + 1)) return;
+    if (i > 0 && same(i - 1, i)) return;
     int64_t j = i + 1;
-    while (j + 1 < n && khi[j + 1] == khi[i]) ++j;
+    while (j + 1 < n && same(j + 1, i)) ++j;
     if (j - i + 1 > 4096) {
         atomicOr((unsigned long long*)&status[ST_FLAGS], 4ull);
         return;
@@ -275,7 +343,9 @@ __global__ __launch_bounds__(256) void k_gather(const int* __restrict__ idx,
                                                 const uint32_t* __restrict__ klo, int64_t n,
                                                 double* __restrict__ sx, double* __restrict__ sy,
                                                 double* __restrict__ sz, double* __restrict__ sm,
-                                                uint32_t* __restrict__ sklo) {
+                                                uint32_t* __restrict__ sklo,
+                                                const int* __restrict__ sysid,
+                                                int* __restrict__ ssys) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     int p = idx[i];
@@ -284,16 +354,34 @@ __global__ __launch_bounds__(256) void k_gather(const int* __restrict__ idx,
     sz[i] = z[p];
     sm[i] = m[p];
     sklo[i] = klo[p];
+    ssys[i] = sysid ? sysid[p] : 0;
+}
+
+// A batch is sorted by octant key, then stably by system: the second sort's
+// key is the system id of each sorted slot, and afterwards the octant keys are
+// read back from the copy k_keys kept in input order.
+__global__ __launch_bounds__(256) void k_syskey(const int* __restrict__ idx,
+                                                const int* __restrict__ sysid, int64_t n,
+                                                uint64_t* __restrict__ key) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) key[i] = (uint64_t)sysid[idx[i]];
+}
+__global__ __launch_bounds__(256) void k_restore_keys(const int* __restrict__ idx,
+                                                      const uint64_t* __restrict__ khio, int64_t n,
+                                                      uint64_t* __restrict__ khi) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) khi[i] = khio[idx[i]];
 }
 
 // D_i = number of common leading levels of the keys of sorted particles i, i+1.
 __global__ __launch_bounds__(256) void k_lcp(const uint64_t* __restrict__ khi,
-                                             const uint32_t* __restrict__ klo, int64_t n,
+                                             const uint32_t* __restrict__ klo,
+                                             const int* __restrict__ ssys, int64_t n,
                                              int8_t* __restrict__ D) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     int d = -1;
-    if (i + 1 < n) {
+    if (i + 1 < n && ssys[i] == ssys[i + 1]) {
         uint64_t a = khi[i] ^ khi[i + 1];
         if (a) {
             d = (__clzll((long long)a) - 1) / 3;
@@ -350,7 +438,8 @@ __global__ __launch_bounds__(NB_THREADS) void k_node_count(const int8_t* __restr
 
 __global__ __launch_bounds__(NB_THREADS) void k_node_emit(
     const int8_t* __restrict__ D, const uint64_t* __restrict__ khi,
-    const uint32_t* __restrict__ klo, const double* __restrict__ g, int64_t n,
+    const uint32_t* __restrict__ klo, const double* __restrict__ sg,
+    const int* __restrict__ ssys, int64_t n,
     const uint32_t* __restrict__ boff, int nblk, int64_t cap, uint8_t* __restrict__ ndepth,
     int8_t* __restrict__ ntype, int* __restrict__ nfirst, int* __restrict__ ncstart,
     uint8_t* __restrict__ ndigit, double* __restrict__ ncx, double* __restrict__ ncy,
@@ -388,7 +477,8 @@ __global__ __launch_bounds__(NB_THREADS) void k_node_emit(
     const uint64_t lt = lanemask_lt();
     uint64_t hi_key = ok ? khi[i] : 0;
     uint32_t lo_key = ok ? klo[i] : 0;
-    double cx = g[G_CX], cy = g[G_CY], cz = g[G_CZ], h = g[G_HALF];
+    const double* g = sg + SG * (ok ? ssys[i] : 0);
+    double cx = g[S_CX], cy = g[S_CY], cz = g[S_CZ], h = g[S_HALF];
     uint64_t mcur = __ballot(ok && r.lo <= 0 && 0 <= r.e);
     for (int d = 0; d < NDEP; ++d) {
         uint64_t mnext = (d + 1 < NDEP) ? __ballot(ok && r.lo <= d + 1 && d + 1 <= r.e) : 0ull;
@@ -565,16 +655,20 @@ __global__ __launch_bounds__(128) void k_topdown(int64_t a, int64_t b,
                                                  int* __restrict__ nparent) {
     int64_t k = a + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= b) return;
-    if (k == 0) {
-        nnext[0] = END;
-        nparent[0] = -1;
-    }
     int cs = ncstart[k], ce = ncstart[k + 1];
     int nx = nnext[k];
     for (int c = cs; c < ce; ++c) {
         nparent[c] = (int)k;
         nnext[c] = c + 1 < ce ? c + 1 : nx;
     }
+}
+
+__global__ __launch_bounds__(256) void k_roots(int64_t nroots, int* __restrict__ nnext,
+                                               int* __restrict__ nparent) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nroots) return;
+    nnext[k] = END;
+    nparent[k] = -1;
 }
 
 // ------------------------------------------------------------------ driver
@@ -629,16 +723,28 @@ static int radix_sort(hq_tree* t) {
 
 int tree_build(hq_tree* t) {
     const int64_t n = t->n;
+    const int nsys = t->nsys;
+    const bool batch = nsys > 1;
     int rc;
     size_t n8 = n * sizeof(double);
-    if ((rc = t->geom.ensure(G_SIZE * sizeof(double)))) return rc;
+    if ((rc = t->geom.ensure((G_SYS + (size_t)SG * nsys) * sizeof(double)))) return rc;
     if ((rc = t->khi.ensure(n * 8)) || (rc = t->klo.ensure(n * 4)) || (rc = t->idx.ensure(n * 4)))
         return rc;
     if ((rc = t->sx.ensure(n8)) || (rc = t->sy.ensure(n8)) || (rc = t->sz.ensure(n8)) ||
-        (rc = t->sm.ensure(n8)) || (rc = t->sklo.ensure(n * 4)) || (rc = t->lcp.ensure(n)))
+        (rc = t->sm.ensure(n8)) || (rc = t->sklo.ensure(n * 4)) || (rc = t->lcp.ensure(n)) ||
+        (rc = t->ssys.ensure(n * 4)))
         return rc;
-    // node capacity of pre_init (:219-223): 3N for N >= 1000, else 10N
-    t->cap = n >= 1000 ? 3 * n : 10 * n;
+    if (batch && ((rc = t->sysid.ensure(n * 4)) || (rc = t->khio.ensure(n * 8)))) return rc;
+    // node capacity of pre_init (:219-223): 3N for N >= 1000, else 10N, per system
+    if (!batch) {
+        t->cap = n >= 1000 ? 3 * n : 10 * n;
+    } else {
+        t->cap = 0;
+        for (int k = 0; k < nsys; ++k) {
+            const int64_t ns = t->hsysoff[k + 1] - t->hsysoff[k];
+            t->cap += ns >= 1000 ? 3 * ns : 10 * ns;
+        }
+    }
     int64_t cap = t->cap;
     if ((rc = t->ndepth.ensure(cap)) || (rc = t->ntype.ensure(cap)) ||
         (rc = t->nfirst.ensure(cap * 4)) || (rc = t->ncount.ensure(cap * 4)) ||
@@ -654,43 +760,66 @@ int tree_build(hq_tree* t) {
     if (t->multipole > 4 && (rc = t->noct.ensure(cap * 14 * 8))) return rc;
 
     hipStream_t s = t->stream;
+    double* sg = t->geom.as<double>() + G_SYS;
+    const int* sysid = batch ? t->sysid.as<int>() : nullptr;
     HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
     HQ_CHECK_HIP(hipEventRecord(t->ev[0], s));
-    // 1. root cube
-    const int nbb = (int)std::min<int64_t>(1024, grid_for(n, 256));
-    if ((rc = t->hist.ensure((size_t)nbb * 6 * sizeof(double)))) return rc;
-    k_bbox_partial<<<nbb, 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(), t->z.as<double>(), n,
-                                       t->hist.as<double>(), t->status.as<int64_t>());
-    k_bbox_final<<<1, 64, 0, s>>>(t->hist.as<double>(), nbb, t->geom.as<double>());
-    k_geom_tables<<<1, 64, 0, s>>>(t->geom.as<double>());
+    // 1. root cube(s)
+    if (!batch) {
+        const int nbb = (int)std::min<int64_t>(1024, grid_for(n, 256));
+        if ((rc = t->hist.ensure((size_t)nbb * 6 * sizeof(double)))) return rc;
+        k_bbox_partial<<<nbb, 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(),
+                                           t->z.as<double>(), n, t->hist.as<double>(),
+                                           t->status.as<int64_t>());
+        k_bbox_final<<<1, 64, 0, s>>>(t->hist.as<double>(), nbb, t->geom.as<double>());
+    } else {
+        k_sysid<<<nsys, 256, 0, s>>>(t->sysoff.as<int64_t>(), t->sysid.as<int>());
+        k_sys_bbox<<<nsys, 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(), t->z.as<double>(),
+                                        t->sysoff.as<int64_t>(), sg, t->status.as<int64_t>());
+    }
+    k_geom_tables<<<nsys, 64, 0, s>>>(sg);
     // 2. keys
     k_keys<<<grid_for(n, 256), 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(),
-                                            t->z.as<double>(), n, t->geom.as<double>(),
+                                            t->z.as<double>(), n, sg, sysid,
                                             t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
-                                            t->idx.as<int>());
+                                            t->idx.as<int>(),
+                                            batch ? t->khio.as<uint64_t>() : nullptr);
     HQ_CHECK_HIP(hipEventRecord(t->ev[1], s));
-    // 3. sort
+    // 3. sort: by octant key; a batch then stably by system
     if ((rc = radix_sort(t))) return rc;
+    if (batch) {
+        int bits = 1;
+        while ((1ll << bits) < nsys) ++bits;
+        k_syskey<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), sysid, n,
+                                                  t->khi.as<uint64_t>());
+        if ((rc = radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(),
+                                   t->khi2.as<uint64_t>(), t->idx2.as<int>(), n, bits, t->hist,
+                                   t->scanpart, s)))
+            return rc;
+        k_restore_keys<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->khio.as<uint64_t>(),
+                                                        n, t->khi.as<uint64_t>());
+    }
     k_tie_fix<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
-                                               t->idx.as<int>(), n, t->status.as<int64_t>());
+                                               t->idx.as<int>(), sysid, n,
+                                               t->status.as<int64_t>());
     k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->x.as<double>(),
                                               t->y.as<double>(), t->z.as<double>(),
                                               t->m.as<double>(), t->klo.as<uint32_t>(), n,
                                               t->sx.as<double>(), t->sy.as<double>(),
                                               t->sz.as<double>(), t->sm.as<double>(),
-                                              t->sklo.as<uint32_t>());
+                                              t->sklo.as<uint32_t>(), sysid, t->ssys.as<int>());
     HQ_CHECK_HIP(hipEventRecord(t->ev[2], s));
     // 4. nodes
-    k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), n,
-                                           t->lcp.as<int8_t>());
+    k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),
+                                           t->ssys.as<int>(), n, t->lcp.as<int8_t>());
     const int nblk = grid_for(n, NB_THREADS);
     if ((rc = t->hist.ensure(((size_t)NDEP * nblk + 1) * sizeof(uint32_t)))) return rc;
     k_node_count<<<nblk, NB_THREADS, 0, s>>>(t->lcp.as<int8_t>(), n, t->hist.as<uint32_t>(), nblk,
                                              t->status.as<int64_t>());
     if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), (int64_t)NDEP * nblk, t->scanpart, s))) return rc;
     k_node_emit<<<nblk, NB_THREADS, 0, s>>>(
-        t->lcp.as<int8_t>(), t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), t->geom.as<double>(),
-        n, t->hist.as<uint32_t>(), nblk, cap, t->ndepth.as<uint8_t>(), t->ntype.as<int8_t>(),
+        t->lcp.as<int8_t>(), t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), sg,
+        t->ssys.as<int>(), n, t->hist.as<uint32_t>(), nblk, cap, t->ndepth.as<uint8_t>(), t->ntype.as<int8_t>(),
         t->nfirst.as<int>(), t->ncstart.as<int>(), t->ndigit.as<uint8_t>(), t->ncx.as<double>(),
         t->ncy.as<double>(), t->ncz.as<double>(), t->nkhi.as<uint64_t>(), t->nklo.as<uint32_t>(),
         t->status.as<int64_t>());
@@ -707,6 +836,9 @@ int tree_build(hq_tree* t) {
                     (long long)cap, (long long)t->nn);
     t->most_depth = 0;
     for (int d = 0; d <= NDEP; ++d) t->level_start[d] = (int)t->hstatus[ST_LEVEL0 + d];
+    if (t->level_start[1] != nsys)
+        return fail(HQ_ERR_ARGS, "batch build produced %d roots for %d systems",
+                    t->level_start[1], nsys);
     for (int d = 0; d < NDEP; ++d)
         if (t->level_start[d + 1] > t->level_start[d]) t->most_depth = d;
     // 5. moments, deepest level first
@@ -721,18 +853,13 @@ int tree_build(hq_tree* t) {
             t->multipole >= 4 ? t->nquad.as<double>() : nullptr,
             t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), cap);
     }
-    // 6. parent / next pointers
+    // 6. parent / next pointers; the roots (one per system) end every walk
+    k_roots<<<grid_for(nsys, 256), 256, 0, s>>>(nsys, t->nnext.as<int>(), t->nparent.as<int>());
     for (int d = 0; d < t->most_depth; ++d) {
         int64_t a = t->level_start[d], b = t->level_start[d + 1];
         if (b <= a) continue;
         k_topdown<<<grid_for(b - a, 128), 128, 0, s>>>(a, b, t->ncstart.as<int>(),
                                                        t->nnext.as<int>(), t->nparent.as<int>());
-    }
-    if (t->most_depth == 0) {  // a single node (N = 1): root has no parent and no next
-        int endv = END;
-        HQ_CHECK_HIP(hipMemcpyAsync(t->nnext.p, &endv, sizeof(int), hipMemcpyHostToDevice, s));
-        int none = -1;
-        HQ_CHECK_HIP(hipMemcpyAsync(t->nparent.p, &none, sizeof(int), hipMemcpyHostToDevice, s));
     }
     HQ_CHECK_HIP(hipEventRecord(t->ev[4], s));
     HQ_CHECK_HIP(hipStreamSynchronize(s));

@@ -22,8 +22,6 @@
 
 namespace hq {
 
-enum { G_L2 = 16, G_HALFD = 48 };
-
 struct DView {
     const double *mass, *cx, *cy, *cz, *quad, *oct;
     const int *first, *cstart;
@@ -343,6 +341,11 @@ __global__ __launch_bounds__(128) void k_down(int64_t a, int64_t b, int multipol
     }
 }
 
+__global__ __launch_bounds__(256) void k_root_pairs(int64_t nsys, int2* __restrict__ pairs) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < nsys) pairs[k] = make_int2((int)k, (int)k);
+}
+
 // ------------------------------------------------------------------ driver
 int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2) {
     int rc;
@@ -368,17 +371,16 @@ int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2) {
     }
     // breadth-first dual walk: frontier in t->clist / t->plist (ping-pong),
     // accepted pairs in t->lstat (grown on demand, contents kept)
-    int64_t nf = 1, nacc = 0;
+    int64_t nf = t->nsys, nacc = 0;  // one (root, root) pair per system
     DevBuf sums;
     DevBuf* pc = &t->clist;
     DevBuf* pn = &t->plist;
     DevBuf& accp = t->lstat;
-    if ((rc = pc->ensure(std::max<int64_t>(1024, 4 * nn) * sizeof(int2))) ||
+    if ((rc = pc->ensure(std::max<int64_t>(std::max<int64_t>(1024, nf), 4 * nn) * sizeof(int2))) ||
         (rc = pn->ensure(std::max<int64_t>(1024, 4 * nn) * sizeof(int2))) ||
         (rc = accp.ensure(std::max<int64_t>(1024, 8 * nn) * sizeof(int2))))
         return rc;
-    const int2 root = make_int2(0, 0);
-    HQ_CHECK_HIP(hipMemcpyAsync(pc->p, &root, sizeof(int2), hipMemcpyHostToDevice, s));
+    k_root_pairs<<<grid_for(nf, 256), 256, 0, s>>>(nf, pc->as<int2>());
     uint32_t tot[2];
     while (nf > 0) {
         const int nblk = grid_for(nf, DB);

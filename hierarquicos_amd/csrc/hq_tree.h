@@ -7,11 +7,19 @@
 // (sorted key) order, so the children of a node are one contiguous range
 // [child_start[k], child_start[k+1]).
 #pragma once
+#include <vector>
+
 #include "hq_internal.h"
 
 struct hq_tree {
     int64_t n = 0;
     int method = 12, multipole = 8, group_size = 64, precision = HQ_FP64;
+    // batch of independent systems: system s owns particles [sysoff[s], sysoff[s+1])
+    // of the input order, gets its own root cube and its own tree; its root is
+    // node s.  nsys == 1 is the reference's single tree.
+    int nsys = 1;
+    std::vector<int64_t> hsysoff;
+    hq::DevBuf sysoff, sysid, ssys, khio;
     hipStream_t stream = nullptr;
     hipEvent_t ev[12] = {};
     bool have_masses = false, built = false;
@@ -47,6 +55,13 @@ struct hq_tree {
     int64_t list_cells = 0, list_parts = 0, n_p2p = 0, n_m2p = 0, exec_cells = 0, exec_parts = 0,
             max_cells = 0, max_parts = 0, max_stack = 0;
 };
+
+// Geometry buffer (doubles): the whole-input bounding box at G_* (single
+// system), then one block of SG doubles per system from G_SYS: root centre,
+// side and half side, and per depth d the opening size L2 = side_d^2 and the
+// half side of a depth-d cube.
+enum { G_SYS = 16 };
+enum { S_CX = 0, S_CY, S_CZ, S_SIDE, S_HALF, S_L2 = 8, S_HALFD = 40, SG = 72 };
 
 // status block slots (int64)
 enum {

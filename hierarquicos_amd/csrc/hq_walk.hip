@@ -28,13 +28,12 @@
 
 namespace hq {
 
-enum { G_CX = 0, G_CY, G_CZ, G_SIDE, G_HALF, G_L2 = 16, G_HALFD = 48 };
-
 struct NodeView {
     const double *mass, *cx, *cy, *cz, *quad, *oct, *ncx, *ncy, *ncz;
     const int *first, *count, *cstart, *next;
     const int8_t* type;
     const uint8_t* depth;
+    const int* sys;  // system of each sorted particle; its tree's root is node sys
     int64_t stride;  // SoA stride of quad/oct
 };
 
@@ -55,6 +54,7 @@ static NodeView node_view(const hq_tree* t) {
     v.next = t->nnext.as<int>();
     v.type = t->ntype.as<int8_t>();
     v.depth = t->ndepth.as<uint8_t>();
+    v.sys = t->ssys.as<int>();
     v.stride = t->cap;
     return v;
 }
@@ -76,7 +76,9 @@ __global__ __launch_bounds__(128) void k_bh_walk(NodeView nv, const double* __re
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     long long np2p = 0, nm2p = 0;
     const int64_t S = nv.stride;
-    int k = 0;
+    const int sys = nv.sys[i];
+    const double* __restrict__ gl2 = g + SG * sys + S_L2;
+    int k = sys;
     while (k != END) {
         const double mass = nv.mass[k];
         const bool leaf = nv.type[k] == 1;
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(128) void k_bh_walk(NodeView nv, const double* __re
         const double dz = nv.cz[k] - pz;
         const double dist2 = dx * dx + dy * dy + dz * dz;
         const int depth = nv.depth[k];
-        if (leaf || g[G_L2 + depth] <= theta2 * dist2 || depth == MAX_DEPTH) {
+        if (leaf || gl2[depth] <= theta2 * dist2 || depth == MAX_DEPTH) {
             if (leaf || multipole == 1) {
                 double rinv = 1.0 / sqrt(dist2 + eps2);
                 rinv = rinv * rinv * rinv;
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
         gnode = groups[gi];
         gf = nv.first[gnode];
         gl = gf + nv.count[gnode] - 1;
-        const double h = g[G_HALFD + nv.depth[gnode]];
+        const double h = g[SG * nv.sys[gf] + S_HALFD + nv.depth[gnode]];
         bmin[0] = nv.ncx[gnode] - h; bmax[0] = nv.ncx[gnode] + h;
         bmin[1] = nv.ncy[gnode] - h; bmax[1] = nv.ncy[gnode] + h;
         bmin[2] = nv.ncz[gnode] - h; bmax[2] = nv.ncz[gnode] + h;
@@ -238,7 +240,9 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
                           0.5 * (bmin[2] + bmax[2])};
     int top = 1, nc = 0, np = 0, maxtop = 1;
     bool overflow = false;
-    if (lane == 0) stk[0] = 0;
+    const int sys = nv.sys[gf];
+    const double* __restrict__ gl2 = g + SG * sys + S_L2;
+    if (lane == 0) stk[0] = sys;
     __builtin_amdgcn_wave_barrier();
     while (top > 0) {
         const int take = top < 64 ? top : 64;
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
                         dist2 = dist2 + dd * dd;
                     }
                     const int depth = nv.depth[k];
-                    action = (g[G_L2 + depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                    action = (gl2[depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
                 }
             }
         }
@@ -756,7 +760,9 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
     int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0;
     long long ntot_c = 0, ntot_p = 0;
     bool overflow = false;
-    if (lane == 0) stk[0] = 0;
+    const int sys = nv.sys[gf];
+    const double* __restrict__ gl2 = g + SG * sys + S_L2;
+    if (lane == 0) stk[0] = sys;
     __builtin_amdgcn_wave_barrier();
     while (top > 0) {
         const int take = top < 64 ? top : 64;
@@ -783,7 +789,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                         dist2 = fma(dd, dd, dist2);
                     }
                     const int depth = nv.depth[k];
-                    action = (g[G_L2 + depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                    action = (gl2[depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
                 }
             }
         }
@@ -897,7 +903,7 @@ int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2) {
     HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
     HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
     k_bh_walk<<<grid_for(t->n, 128), 128, 0, s>>>(
-        node_view(t), t->geom.as<double>(), t->sx.as<double>(), t->sy.as<double>(),
+        node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(), t->sy.as<double>(),
         t->sz.as<double>(), t->sm.as<double>(), t->n, t->multipole, theta2, G, eps2,
         t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>(), t->status.as<int64_t>());
     HQ_CHECK_HIP(hipEventRecord(t->ev[6], s));
@@ -918,10 +924,8 @@ __global__ __launch_bounds__(256) void k_mark_groups(NodeView nv, const int* __r
     const int cnt = nv.count[k];
     if (cnt == 0) return;
     if (!(cnt <= gs || nv.type[k] == 1)) return;
-    if (k != 0) {
-        const int p = nparent[k];
-        if (nv.count[p] <= gs) return;  // the parent is (inside) a group already
-    }
+    const int p = nparent[k];
+    if (p >= 0 && nv.count[p] <= gs) return;  // the parent is (inside) a group already
     gmark[nv.first[k]] = (int)k;
 }
 
@@ -933,8 +937,8 @@ __global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, i
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= nn) return;
     const int cnt = nv.count[k];
-    if (k == 0 && cnt <= gs) {
-        bmark[nv.first[0]] = nv.first[0];
+    if (nv.depth[k] == 0 && cnt <= gs) {  // a root (one per system) small enough to bunch whole
+        bmark[nv.first[k]] = nv.first[k];
         return;
     }
     if (nv.type[k] != 2 || cnt <= gs) return;
@@ -1022,7 +1026,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
             auto launch = [&](auto kern) {
-                kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>(), t->sx.as<double>(),
+                kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                             t->sy.as<double>(), t->sz.as<double>(),
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
                                             theta2, G, (float)eps2, t->mom32.as<float>(),
@@ -1071,18 +1075,18 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
         const NodeView nv = node_view(t);
         if (!bunched)
-            k_lists<MODE_GROUP><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(), t->sx.as<double>(),
+            k_lists<MODE_GROUP><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
                                                        t->sm.as<double>(), t->n,
                                                        t->groups.as<int>(), ngroups, theta2, lo,
                                                        t->status.as<int64_t>());
         else if (!mixed)
-            k_lists<MODE_BUNCH><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(), t->sx.as<double>(),
+            k_lists<MODE_BUNCH><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
                                                        t->sm.as<double>(), t->n, t->bstart.as<int>(),
                                                        ngroups, theta2, lo, t->status.as<int64_t>());
         else
-            k_lists<MODE_BUNCH32><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>(),
+            k_lists<MODE_BUNCH32><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS,
                                                          t->sx.as<double>(), t->sy.as<double>(),
                                                          t->sz.as<double>(), t->sm.as<double>(),
                                                          t->n, t->bstart.as<int>(), ngroups, theta2,

@@ -47,7 +47,10 @@ class OctreeType:
         self.is_allocated = False
 
     # -- pre_init / init ------------------------------------------------------
-    def pre_init(self, m, method=10, group_size=64, precision="fp64"):
+    def pre_init(self, m, method=10, group_size=64, precision="fp64", counts=None):
+        """counts: sizes of the independent systems of a batch (consecutive
+        runs of m); each system gets its own tree, as if pre_init had been
+        called on it alone.  None is one system."""
         if method not in METHODS:
             raise ValueError(f"method not identified! ({method})")
         prec = {"fp64": L.HQ_FP64, "mixed": L.HQ_MIXED}[precision]
@@ -57,7 +60,17 @@ class OctreeType:
         self.method = method
         self.group_size = group_size
         lib = L.lib()
-        L.check(lib.hq_tree_create(C.byref(self._h), self.N, method, group_size, prec))
+        if counts is None:
+            self.counts = None
+            L.check(lib.hq_tree_create(C.byref(self._h), self.N, method, group_size, prec))
+        else:
+            cnt = np.ascontiguousarray(counts, dtype=np.int64)
+            if cnt.ndim != 1 or int(cnt.sum()) != self.N:
+                raise ValueError("counts must sum to len(m)")
+            self.counts = cnt
+            L.check(lib.hq_tree_create_batch(C.byref(self._h), len(cnt),
+                                              cnt.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              method, group_size, prec))
         p, where = _addr(self._m)
         L.check(lib.hq_tree_set_masses(self._h, p, where))
         self.is_allocated = True

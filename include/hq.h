@@ -77,6 +77,14 @@ typedef struct hq_tree hq_tree;
 /* pre_init(m, method, group_size) (:178-282).  group_size <= 0 gives 64.
  * precision: HQ_FP64 or HQ_MIXED. */
 int hq_tree_create(hq_tree **out, int64_t n, int method, int group_size, int precision);
+/* A batch of nsys independent systems in one handle, for the ensemble and
+ * Parareal drivers (parareal/parareal.f90:28-140 hold one OctreeType per
+ * system and time slice).  System k owns counts[k] >= 1 consecutive particles
+ * of every input array; each gets its own root cube and its own tree, and its
+ * forces are bit-identical to those of a handle built from that system alone.
+ * Every other call works on a batch handle as on a single one. */
+int hq_tree_create_batch(hq_tree **out, int nsys, const int64_t *counts, int method,
+                         int group_size, int precision);
 int hq_tree_set_masses(hq_tree *t, const double *m, int where);
 /* init(x, y, z) (:320-387): keys, radix sort, octree from sorted keys,
  * upward P2M/M2M pass, groups. */
