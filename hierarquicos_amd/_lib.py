@@ -21,6 +21,9 @@ HQ_HOST, HQ_DEVICE = 0, 1
 HQ_FP64, HQ_MIXED = 0, 1
 HQ_LAYOUT_3N, HQ_LAYOUT_N3 = 0, 1
 HQ_TUNE_BUNCH, HQ_TUNE_SUPER, HQ_TUNE_FUSED = 1, 2, 3
+HQ_SCHEME_VERLET, HQ_SCHEME_EULER = 0, 1
+HQ_FORCE_TREE, HQ_FORCE_BUNCHED, HQ_FORCE_DIRECT = 0, 1, 2
+HQ_CORR_REFERENCE, HQ_CORR_EXACT = 0, 1
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int)
@@ -29,9 +32,23 @@ _u64 = C.POINTER(C.c_uint64)
 _u32 = C.POINTER(C.c_uint32)
 _vp = C.c_void_p
 
+
+class PararealParams(C.Structure):
+    """hq_parareal_params of include/hq.h."""
+    _fields_ = [("n", C.c_int64), ("num_windows", C.c_int), ("itermax", C.c_int),
+                ("tf", C.c_double), ("maxerror", C.c_double), ("G", C.c_double),
+                ("eps2", C.c_double),
+                ("coarse_force", C.c_int), ("coarse_method", C.c_int),
+                ("coarse_precision", C.c_int), ("coarse_steps", C.c_int),
+                ("coarse_theta2", C.c_double),
+                ("fine_force", C.c_int), ("fine_method", C.c_int), ("fine_precision", C.c_int),
+                ("fine_steps", C.c_int), ("fine_theta2", C.c_double),
+                ("group_size", C.c_int), ("correction", C.c_int)]
+
 EXPORTS = (
     "hq_compute_forces_direct", "hq_compute_forces", "hq_generate_tree", "hq_tree_create",
-    "hq_tree_create_batch",
+    "hq_tree_create_batch", "hq_tree_set_state", "hq_tree_get_state", "hq_tree_integrate",
+    "hq_tree_energy", "hq_parareal",
     "hq_tree_set_masses", "hq_tree_build", "hq_tree_forces", "hq_tree_forces_bunched",
     "hq_tree_destroy", "hq_tree_tune", "hq_tree_info", "hq_tree_export", "hq_tree_order",
     "hq_direct_forces_device", "hq_morton_keys", "hq_morton_create", "hq_morton_build",
@@ -78,6 +95,13 @@ def lib():
     L.hq_generate_tree.argtypes = [C.c_int64, _d, _d, _d, _d, C.c_int]
     L.hq_tree_create.argtypes = [C.POINTER(_vp), C.c_int64, C.c_int, C.c_int, C.c_int]
     L.hq_tree_create_batch.argtypes = [C.POINTER(_vp), C.c_int, _i64, C.c_int, C.c_int, C.c_int]
+    L.hq_tree_set_state.argtypes = [_vp] + [_vp] * 6 + [C.c_int]
+    L.hq_tree_get_state.argtypes = [_vp] + [_vp] * 6 + [C.c_int]
+    L.hq_tree_integrate.argtypes = [_vp, C.c_int, C.c_double, C.c_int64, C.c_double, C.c_double,
+                                    C.c_double, C.c_int]
+    L.hq_tree_energy.argtypes = [_vp, C.c_double, C.c_double, _d]
+    L.hq_parareal.argtypes = [C.POINTER(PararealParams)] + [_vp] * 7 + [C.c_int, _vp, C.c_int,
+                                                                        _d, _i]
     L.hq_tree_set_masses.argtypes = [_vp, _vp, C.c_int]
     L.hq_tree_build.argtypes = [_vp, _vp, _vp, _vp, C.c_int]
     L.hq_tree_forces.argtypes = [_vp, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int]

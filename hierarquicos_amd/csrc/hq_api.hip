@@ -104,6 +104,10 @@ int hq_tree_create(hq_tree** out, int64_t n, int method, int group_size, int pre
     int rc;
     if ((rc = t->status.ensure(ST_SIZE * sizeof(int64_t)))) return rc;
     HQ_CHECK_HIP(hipHostMalloc((void**)&t->hstatus, ST_SIZE * sizeof(int64_t), hipHostMallocDefault));
+    // one system [0, n): a batch handle overwrites these
+    t->hsysoff = {0, n};
+    if ((rc = t->sysoff.ensure(2 * sizeof(int64_t)))) return rc;
+    HQ_CHECK_HIP(hipMemcpy(t->sysoff.p, t->hsysoff.data(), 2 * sizeof(int64_t), hipMemcpyHostToDevice));
     *out = t.release();
     return HQ_OK;
 }

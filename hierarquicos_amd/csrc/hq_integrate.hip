@@ -1,0 +1,547 @@
+// hq_integrate.hip -- time integration around the force path.
+//
+//   * velocity Verlet (kick-drift-kick, momenta form) with a rebuild before
+//     every force evaluation: benchmarks/integration_verlet/integration.f90
+//     :110-172 and the fine propagator F_til of parareal/parareal.f90:46-69;
+//   * symplectic Euler (kick with the force at the start of the step, then
+//     drift): the coarse propagator G_til of parareal/parareal.f90:8-44;
+//   * Parareal (parareal/parareal.f90:97-186): coarse sweep, fine propagators
+//     of all time windows as ONE batch handle (one tree per window), serial
+//     correction, stop at ||U_k+1 - U_k|| / (6N) <= maxerror.
+//
+// State lives in HBM in input order: positions in the handle's x/y/z, momenta
+// in px/py/pz.  Updates are FP64 with contraction off, written as the
+// reference writes them: p = p + (0.5 dt) F and q = q + (dt p) / m.
+#include <math.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "hq_tree.h"
+
+namespace hq {
+
+constexpr int DT = 256;  // targets per workgroup = sources per LDS tile
+
+// ------------------------------------------------------------------ forces
+// Direct sum inside each system (fortran/direct_forces.f90:8-64 pair force,
+// gather form).  A workgroup takes DT consecutive targets of ONE system; the
+// block -> system map is the prefix bofs[] of per-system block counts.
+__global__ __launch_bounds__(DT) void k_direct_systems(const int64_t* __restrict__ off,
+                                                       const int* __restrict__ bofs, int nsys,
+                                                       const double* __restrict__ m,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z, double G,
+                                                       double eps2, double* __restrict__ fx,
+                                                       double* __restrict__ fy,
+                                                       double* __restrict__ fz) {
+    __shared__ double4 tile[DT];
+    const int blk = blockIdx.x;
+    int lo = 0, hi = nsys - 1;  // last system whose first block <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bofs[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const int sys = lo;
+    const int64_t s0 = off[sys], s1 = off[sys + 1];
+    const int64_t a = s0 + (int64_t)(blk - bofs[sys]) * DT + threadIdx.x;
+    const bool ok = a < s1;
+    const int64_t aa = ok ? a : s0;
+    const double xa = x[aa], ya = y[aa], za = z[aa], ma = m[aa];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int64_t b0 = s0; b0 < s1; b0 += DT) {
+        const int64_t b = b0 + threadIdx.x;
+        __syncthreads();
+        tile[threadIdx.x] = b < s1 ? make_double4(x[b], y[b], z[b], m[b]) : make_double4(0, 0, 0, 0);
+        __syncthreads();
+        const int cnt = (int)((s1 - b0) < DT ? (s1 - b0) : DT);
+        for (int j = 0; j < cnt; ++j) {
+            const double4 s = tile[j];
+            if (b0 + j == aa) continue;
+            const double dx = s.x - xa, dy = s.y - ya, dz = s.z - za;
+            const double dist2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double dist = sqrt(dist2);
+            const double f = G * ma * s.w / (dist * dist2);
+            f0 += f * dx;
+            f1 += f * dy;
+            f2 += f * dz;
+        }
+    }
+    if (ok) {
+        fx[a] = f0;
+        fy[a] = f1;
+        fz[a] = f2;
+    }
+}
+
+int direct_forces_systems(hq_tree* t, double G, double eps2) {
+    int rc;
+    const int nsys = t->nsys;
+    std::vector<int> bofs(nsys + 1, 0);
+    for (int k = 0; k < nsys; ++k)
+        bofs[k + 1] = bofs[k] + (int)((t->hsysoff[k + 1] - t->hsysoff[k] + DT - 1) / DT);
+    const size_t b8 = t->n * sizeof(double);
+    if ((rc = t->fx.ensure(b8)) || (rc = t->fy.ensure(b8)) || (rc = t->fz.ensure(b8)) ||
+        (rc = t->dblk.ensure((nsys + 1) * sizeof(int))))
+        return rc;
+    hipStream_t s = t->stream;
+    HQ_CHECK_HIP(hipMemcpyAsync(t->dblk.p, bofs.data(), (nsys + 1) * sizeof(int),
+                                hipMemcpyHostToDevice, s));
+    k_direct_systems<<<bofs[nsys], DT, 0, s>>>(t->sysoff.as<int64_t>(), t->dblk.as<int>(), nsys,
+                                               t->m.as<double>(), t->x.as<double>(),
+                                               t->y.as<double>(), t->z.as<double>(), G, eps2,
+                                               t->fx.as<double>(), t->fy.as<double>(),
+                                               t->fz.as<double>());
+    HQ_CHECK_HIP(hipGetLastError());
+    // the host copy of bofs must outlive the async copy
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    return HQ_OK;
+}
+
+int forces_now(hq_tree* t, int force, double theta2, double G, double eps2, bool* sorted) {
+    int rc;
+    if (force == HQ_FORCE_DIRECT) {
+        *sorted = false;
+        return direct_forces_systems(t, G, eps2);
+    }
+    if ((rc = tree_build(t))) return rc;
+    *sorted = true;
+    if (force == HQ_FORCE_BUNCHED) return tree_forces_groups(t, theta2, G, eps2, t->precision, true);
+    if (t->method < 20) return tree_forces_bh(t, theta2, G, eps2);
+    if (t->method < 30) return tree_forces_dehnen(t, theta2, G, eps2);
+    return tree_forces_groups(t, theta2, G, eps2, HQ_FP64, false);
+}
+
+// ------------------------------------------------------------------ steps
+// nkick kicks p = p + h F (each rounded, as two successive half kicks are),
+// then, if dt != 0, the drift q = q + (dt p) / m.  F is in tree order when
+// idx is given (slot i belongs to particle idx[i]), else in input order.
+__global__ __launch_bounds__(256) void k_kick_drift(int64_t n, const int* __restrict__ idx,
+                                                    const double* __restrict__ fx,
+                                                    const double* __restrict__ fy,
+                                                    const double* __restrict__ fz,
+                                                    const double* __restrict__ m, double h,
+                                                    int nkick, double dt,
+                                                    double* __restrict__ x, double* __restrict__ y,
+                                                    double* __restrict__ z, double* __restrict__ px,
+                                                    double* __restrict__ py,
+                                                    double* __restrict__ pz) {
+#pragma clang fp contract(off)
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = idx ? idx[i] : i;
+    double p0 = px[o], p1 = py[o], p2 = pz[o];
+    const double f0 = fx[i], f1 = fy[i], f2 = fz[i];
+    for (int k = 0; k < nkick; ++k) {
+        p0 = p0 + h * f0;
+        p1 = p1 + h * f1;
+        p2 = p2 + h * f2;
+    }
+    px[o] = p0;
+    py[o] = p1;
+    pz[o] = p2;
+    if (dt != 0.0) {
+        const double mo = m[o];
+        x[o] = x[o] + dt * p0 / mo;
+        y[o] = y[o] + dt * p1 / mo;
+        z[o] = z[o] + dt * p2 / mo;
+    }
+}
+
+static void kick_drift(hq_tree* t, bool sorted, double h, int nkick, double dt) {
+    k_kick_drift<<<grid_for(t->n, 256), 256, 0, t->stream>>>(
+        t->n, sorted ? t->idx.as<int>() : nullptr, t->fx.as<double>(), t->fy.as<double>(),
+        t->fz.as<double>(), t->m.as<double>(), h, nkick, dt, t->x.as<double>(), t->y.as<double>(),
+        t->z.as<double>(), t->px.as<double>(), t->py.as<double>(), t->pz.as<double>());
+}
+
+int integrate(hq_tree* t, int scheme, double dt, int64_t nsteps, double theta2, double G,
+              double eps2, int force) {
+    int rc;
+    bool sorted = false;
+    if (nsteps <= 0) return HQ_OK;
+    if (scheme == HQ_SCHEME_VERLET) {
+        // integration.f90:137-171: F(q0); per step p += dt/2 F, q += dt p/m,
+        // rebuild, F(q), p += dt/2 F.  The closing kick of a step and the
+        // opening kick of the next share F and run in one pass as two adds.
+        const double h = 0.5 * dt;
+        if ((rc = forces_now(t, force, theta2, G, eps2, &sorted))) return rc;
+        for (int64_t st = 0; st < nsteps; ++st) {
+            kick_drift(t, sorted, h, st == 0 ? 1 : 2, dt);
+            if ((rc = forces_now(t, force, theta2, G, eps2, &sorted))) return rc;
+        }
+        kick_drift(t, sorted, h, 1, 0.0);
+    } else {
+        // parareal.f90:30-36: forces of q; p = p + dt F; q = q + dt p / m
+        for (int64_t st = 0; st < nsteps; ++st) {
+            if ((rc = forces_now(t, force, theta2, G, eps2, &sorted))) return rc;
+            kick_drift(t, sorted, dt, 1, dt);
+        }
+    }
+    HQ_CHECK_HIP(hipGetLastError());
+    HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    return HQ_OK;
+}
+
+// ------------------------------------------------------------------ diagnostics
+// Per particle: potential share -1/2 sum_b G m_a m_b / sqrt(|q_b - q_a|^2 + eps^2)
+// (parareal/api3d.f90:108-122, integration.f90:247-262), kinetic p.p / (2 m),
+// momentum p and angular momentum q x p; then one workgroup per system sums
+// them in a fixed order.
+__global__ __launch_bounds__(DT) void k_energy_terms(const int64_t* __restrict__ off,
+                                                     const int* __restrict__ bofs, int nsys,
+                                                     const double* __restrict__ m,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z,
+                                                     const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz, double G,
+                                                     double eps2, double* __restrict__ w,
+                                                     int64_t n) {
+    __shared__ double4 tile[DT];
+    const int blk = blockIdx.x;
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bofs[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t s0 = off[lo], s1 = off[lo + 1];
+    const int64_t a = s0 + (int64_t)(blk - bofs[lo]) * DT + threadIdx.x;
+    const bool ok = a < s1;
+    const int64_t aa = ok ? a : s0;
+    const double xa = x[aa], ya = y[aa], za = z[aa], ma = m[aa];
+    double v = 0.0;
+    for (int64_t b0 = s0; b0 < s1; b0 += DT) {
+        const int64_t b = b0 + threadIdx.x;
+        __syncthreads();
+        tile[threadIdx.x] = b < s1 ? make_double4(x[b], y[b], z[b], m[b]) : make_double4(0, 0, 0, 0);
+        __syncthreads();
+        const int cnt = (int)((s1 - b0) < DT ? (s1 - b0) : DT);
+        for (int j = 0; j < cnt; ++j) {
+            const double4 s = tile[j];
+            if (b0 + j == aa) continue;
+            const double dx = s.x - xa, dy = s.y - ya, dz = s.z - za;
+            v += s.w / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+        }
+    }
+    if (ok) {
+        const double p0 = px[a], p1 = py[a], p2 = pz[a];
+        w[a] = -0.5 * G * ma * v;
+        w[n + a] = (p0 * p0 + p1 * p1 + p2 * p2) / (2.0 * ma);
+        w[2 * n + a] = p0;
+        w[3 * n + a] = p1;
+        w[4 * n + a] = p2;
+        w[5 * n + a] = ya * p2 - za * p1;
+        w[6 * n + a] = za * p0 - xa * p2;
+        w[7 * n + a] = xa * p1 - ya * p0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_sum(const int64_t* __restrict__ off,
+                                                    const double* __restrict__ w, int64_t n,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    const int sys = blockIdx.x;
+    const int64_t s0 = off[sys], s1 = off[sys + 1];
+    for (int c = 0; c < 8; ++c) {
+        double acc = 0.0;
+        for (int64_t i = s0 + threadIdx.x; i < s1; i += 256) acc += w[c * n + i];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[8 * sys + c] = red[0];
+        __syncthreads();
+    }
+}
+
+int tree_energy(hq_tree* t, double G, double eps2, double* out_dev) {
+    int rc;
+    const int nsys = t->nsys;
+    std::vector<int> bofs(nsys + 1, 0);
+    for (int k = 0; k < nsys; ++k)
+        bofs[k + 1] = bofs[k] + (int)((t->hsysoff[k + 1] - t->hsysoff[k] + DT - 1) / DT);
+    if ((rc = t->dblk.ensure((nsys + 1) * sizeof(int))) ||
+        (rc = t->ework.ensure((size_t)8 * t->n * sizeof(double))))
+        return rc;
+    hipStream_t s = t->stream;
+    HQ_CHECK_HIP(hipMemcpyAsync(t->dblk.p, bofs.data(), (nsys + 1) * sizeof(int),
+                                hipMemcpyHostToDevice, s));
+    k_energy_terms<<<bofs[nsys], DT, 0, s>>>(
+        t->sysoff.as<int64_t>(), t->dblk.as<int>(), nsys, t->m.as<double>(), t->x.as<double>(),
+        t->y.as<double>(), t->z.as<double>(), t->px.as<double>(), t->py.as<double>(),
+        t->pz.as<double>(), G, eps2, t->ework.as<double>(), t->n);
+    k_energy_sum<<<nsys, 256, 0, s>>>(t->sysoff.as<int64_t>(), t->ework.as<double>(), t->n,
+                                      out_dev);
+    HQ_CHECK_HIP(hipGetLastError());
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    return HQ_OK;
+}
+
+// ------------------------------------------------------------------ Parareal
+// Correction of window i+1 from the fine result F, the new coarse result Gn
+// and the previous coarse result Go, component by component:
+//   HQ_CORR_REFERENCE  (Gn + F) - Go   (parareal.f90:154-155, in that order)
+//   HQ_CORR_EXACT      F + (Gn - Go)   (exactly F once the window's start
+//                                       state has converged, since Gn == Go)
+__global__ __launch_bounds__(256) void k_correct(int64_t len, const double* __restrict__ F,
+                                                 const double* __restrict__ Gn,
+                                                 const double* __restrict__ Go, int mode,
+                                                 double* __restrict__ U) {
+#pragma clang fp contract(off)
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    U[i] = mode == HQ_CORR_EXACT ? F[i] + (Gn[i] - Go[i]) : (Gn[i] + F[i]) - Go[i];
+}
+
+// sum of squares of a - b, one partial per workgroup (fixed order)
+__global__ __launch_bounds__(256) void k_diff2(int64_t len, const double* __restrict__ a,
+                                               const double* __restrict__ b,
+                                               double* __restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < len; i += (int64_t)gridDim.x * 256) {
+        const double d = a[i] - b[i];
+        acc += d * d;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace hq
+
+using namespace hq;
+
+namespace {
+
+using TreePtr = std::unique_ptr<hq_tree, int (*)(hq_tree*)>;
+
+// U is component-major: U[(c * nw + w) * n + a] for c in x, y, z, px, py, pz
+inline double* comp(double* U, int c, int nw, int64_t n, int w) {
+    return U + ((int64_t)c * nw + w) * n;
+}
+
+int load_window(hq_tree* t, double* U, int nw, int64_t n, int w, int nwin) {
+    hipStream_t s = t->stream;
+    DevBuf* dst[6] = {&t->x, &t->y, &t->z, &t->px, &t->py, &t->pz};
+    for (int c = 0; c < 6; ++c)
+        HQ_CHECK_HIP(hipMemcpyAsync(dst[c]->p, comp(U, c, nw, n, w), nwin * n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, s));
+    return HQ_OK;
+}
+
+int store_state(hq_tree* t, double* dst, int64_t stride) {
+    hipStream_t s = t->stream;
+    const DevBuf* src[6] = {&t->x, &t->y, &t->z, &t->px, &t->py, &t->pz};
+    for (int c = 0; c < 6; ++c)
+        HQ_CHECK_HIP(hipMemcpyAsync(dst + c * stride, src[c]->p, t->n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, s));
+    return HQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hq_tree_set_state(hq_tree* t, const double* x, const double* y, const double* z,
+                      const double* px, const double* py, const double* pz, int where) {
+    if (!t || !x || !y || !z || !px || !py || !pz) return fail(HQ_ERR_ARGS, "null argument");
+    if (where != HQ_HOST && where != HQ_DEVICE) return fail(HQ_ERR_ARGS, "bad location");
+    const hipMemcpyKind k = where == HQ_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    const size_t b = t->n * sizeof(double);
+    DevBuf* dst[6] = {&t->x, &t->y, &t->z, &t->px, &t->py, &t->pz};
+    const double* src[6] = {x, y, z, px, py, pz};
+    int rc;
+    for (int c = 0; c < 6; ++c) {
+        if ((rc = dst[c]->ensure(b))) return rc;
+        HQ_CHECK_HIP(hipMemcpyAsync(dst[c]->p, src[c], b, k, t->stream));
+    }
+    HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    t->have_state = true;
+    t->built = false;
+    return HQ_OK;
+}
+
+int hq_tree_get_state(hq_tree* t, double* x, double* y, double* z, double* px, double* py,
+                      double* pz, int where) {
+    if (!t) return fail(HQ_ERR_ARGS, "null handle");
+    if (!t->have_state) return fail(HQ_ERR_ARGS, "no state (call hq_tree_set_state)");
+    if (where != HQ_HOST && where != HQ_DEVICE) return fail(HQ_ERR_ARGS, "bad location");
+    const hipMemcpyKind k = where == HQ_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    const DevBuf* src[6] = {&t->x, &t->y, &t->z, &t->px, &t->py, &t->pz};
+    double* dst[6] = {x, y, z, px, py, pz};
+    for (int c = 0; c < 6; ++c)
+        if (dst[c]) HQ_CHECK_HIP(hipMemcpyAsync(dst[c], src[c]->p, t->n * sizeof(double), k, t->stream));
+    HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    return HQ_OK;
+}
+
+int hq_tree_integrate(hq_tree* t, int scheme, double dt, int64_t nsteps, double theta2, double G,
+                      double eps2, int force) {
+    if (!t) return fail(HQ_ERR_ARGS, "null handle");
+    if (!t->have_masses || !t->have_state)
+        return fail(HQ_ERR_ARGS, "masses and state must be set before integrating");
+    if (scheme != HQ_SCHEME_VERLET && scheme != HQ_SCHEME_EULER)
+        return fail(HQ_ERR_ARGS, "bad scheme %d", scheme);
+    if (force != HQ_FORCE_TREE && force != HQ_FORCE_BUNCHED && force != HQ_FORCE_DIRECT)
+        return fail(HQ_ERR_ARGS, "bad force mode %d", force);
+    if (nsteps < 0) return fail(HQ_ERR_ARGS, "negative step count");
+    return integrate(t, scheme, dt, nsteps, theta2, G, eps2, force);
+}
+
+int hq_tree_energy(hq_tree* t, double G, double eps2, double* out) {
+    if (!t || !out) return fail(HQ_ERR_ARGS, "null argument");
+    if (!t->have_masses || !t->have_state) return fail(HQ_ERR_ARGS, "masses and state must be set");
+    int rc;
+    DevBuf d;
+    if ((rc = d.ensure((size_t)8 * t->nsys * sizeof(double)))) return rc;
+    if ((rc = tree_energy(t, G, eps2, d.as<double>()))) return rc;
+    std::vector<double> h((size_t)8 * t->nsys);
+    HQ_CHECK_HIP(hipMemcpy(h.data(), d.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < t->nsys; ++k) {
+        const double* w = &h[(size_t)8 * k];
+        double* o = out + 7 * k;
+        o[0] = w[0] + w[1];  // E = V + T
+        for (int c = 0; c < 6; ++c) o[1 + c] = w[2 + c];
+    }
+    return HQ_OK;
+}
+
+int hq_parareal(const hq_parareal_params* prm, const double* m, const double* x0,
+                const double* y0, const double* z0, const double* px0, const double* py0,
+                const double* pz0, int where, double* U_out, int out_where, double* errors,
+                int* iterations) {
+    if (!prm || !m || !x0 || !y0 || !z0 || !px0 || !py0 || !pz0 || !U_out)
+        return fail(HQ_ERR_ARGS, "null argument");
+    const int64_t n = prm->n;
+    const int nw = prm->num_windows, itmax = prm->itermax;
+    if (n < 1 || nw < 2 || itmax < 1) return fail(HQ_ERR_ARGS, "need n >= 1, num_windows >= 2, itermax >= 1");
+    if ((int64_t)(nw - 1) * n > (int64_t)1 << 30) return fail(HQ_ERR_ARGS, "windows x particles exceeds 2^30");
+    if (prm->coarse_steps < 1 || prm->fine_steps < 1) return fail(HQ_ERR_ARGS, "step counts must be >= 1");
+    if (prm->correction != HQ_CORR_REFERENCE && prm->correction != HQ_CORR_EXACT)
+        return fail(HQ_ERR_ARGS, "bad correction mode");
+    if (where != HQ_HOST && where != HQ_DEVICE) return fail(HQ_ERR_ARGS, "bad location");
+    const double window = prm->tf / nw;
+    const double dtc = window / prm->coarse_steps, dtf = window / prm->fine_steps;
+    int rc;
+    // coarse handle: one system; fine handle: windows 1..nw-1 as a batch
+    hq_tree *tc_raw = nullptr, *tf_raw = nullptr;
+    const int cm = prm->coarse_force == HQ_FORCE_DIRECT ? 10 : prm->coarse_method;
+    const int fm = prm->fine_force == HQ_FORCE_DIRECT ? 10 : prm->fine_method;
+    if ((rc = hq_tree_create(&tc_raw, n, cm, prm->group_size, prm->coarse_precision))) return rc;
+    TreePtr tc(tc_raw, hq_tree_destroy);
+    std::vector<int64_t> counts(nw - 1, n);
+    if ((rc = hq_tree_create_batch(&tf_raw, nw - 1, counts.data(), fm, prm->group_size,
+                                   prm->fine_precision)))
+        return rc;
+    TreePtr tfh(tf_raw, hq_tree_destroy);
+    hipStream_t s = tc->stream;
+    if ((rc = hq_tree_set_masses(tc.get(), m, where))) return rc;
+    {
+        DevBuf mm;
+        if ((rc = mm.ensure((size_t)(nw - 1) * n * sizeof(double)))) return rc;
+        for (int w = 0; w < nw - 1; ++w)
+            HQ_CHECK_HIP(hipMemcpyAsync(mm.as<double>() + (int64_t)w * n, m, n * sizeof(double),
+                                        where == HQ_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+        if ((rc = hq_tree_set_masses(tfh.get(), mm.as<double>(), HQ_DEVICE))) return rc;
+    }
+    // iterates U_k (Up), U_k+1 (Un); coarse results of the last sweep (Go)
+    const int64_t len = (int64_t)6 * nw * n;
+    DevBuf Up, Un, Go, Gn, part;
+    if ((rc = Up.ensure(len * 8)) || (rc = Un.ensure(len * 8)) || (rc = Go.ensure(len * 8)) ||
+        (rc = Gn.ensure((size_t)6 * n * 8)) || (rc = part.ensure(64 * 8)))
+        return rc;
+    const double* init[6] = {x0, y0, z0, px0, py0, pz0};
+    const hipMemcpyKind kin = where == HQ_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    for (int c = 0; c < 6; ++c)
+        HQ_CHECK_HIP(hipMemcpyAsync(comp(Un.as<double>(), c, nw, n, 0), init[c], n * 8, kin, s));
+    if ((rc = hq_tree_set_state(tc.get(), x0, y0, z0, px0, py0, pz0, where))) return rc;
+    if ((rc = hq_tree_set_state(tfh.get(), Un.as<double>(), Un.as<double>(), Un.as<double>(),
+                                Un.as<double>(), Un.as<double>(), Un.as<double>(), HQ_DEVICE)))
+        return rc;  // sizes the fine handle's state buffers; contents are loaded per iteration
+    auto coarse = [&](double* U, int w) -> int {
+        // G_til of window w: the state U[.., w] through coarse_steps Euler steps into Gn
+        int r;
+        if ((r = load_window(tc.get(), U, nw, n, w, 1))) return r;
+        if ((r = integrate(tc.get(), HQ_SCHEME_EULER, dtc, prm->coarse_steps, prm->coarse_theta2,
+                           prm->G, prm->eps2, prm->coarse_force)))
+            return r;
+        return store_state(tc.get(), Gn.as<double>(), n);
+    };
+    auto d2d = [&](double* dst, const double* src, int64_t cnt) -> int {
+        HQ_CHECK_HIP(hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToDevice, s));
+        return HQ_OK;
+    };
+    // initial coarse sweep (:131-134)
+    for (int w = 0; w + 1 < nw; ++w) {
+        if ((rc = coarse(Un.as<double>(), w))) return rc;
+        for (int c = 0; c < 6; ++c) {
+            if ((rc = d2d(comp(Un.as<double>(), c, nw, n, w + 1), Gn.as<double>() + c * n, n)) ||
+                (rc = d2d(comp(Go.as<double>(), c, nw, n, w), Gn.as<double>() + c * n, n)))
+                return rc;
+        }
+    }
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    int iters = 1;
+    for (int k = 1; k < itmax; ++k) {
+        ++iters;
+        std::swap(Up.p, Un.p);
+        std::swap(Up.bytes, Un.bytes);
+        // fine propagators of windows 0..nw-2 from U_k, all in one batch (:143-151)
+        hq_tree* F = tfh.get();
+        if ((rc = load_window(F, Up.as<double>(), nw, n, 0, nw - 1))) return rc;
+        F->built = false;
+        if ((rc = integrate(F, HQ_SCHEME_VERLET, dtf, prm->fine_steps, prm->fine_theta2, prm->G,
+                            prm->eps2, prm->fine_force)))
+            return rc;
+        // serial correction (:153-156)
+        for (int c = 0; c < 6; ++c)
+            if ((rc = d2d(comp(Un.as<double>(), c, nw, n, 0), comp(Up.as<double>(), c, nw, n, 0), n)))
+                return rc;
+        const DevBuf* fst[6] = {&F->x, &F->y, &F->z, &F->px, &F->py, &F->pz};
+        for (int w = 0; w + 1 < nw; ++w) {
+            if ((rc = coarse(Un.as<double>(), w))) return rc;
+            for (int c = 0; c < 6; ++c) {
+                double* go = comp(Go.as<double>(), c, nw, n, w);
+                k_correct<<<grid_for(n, 256), 256, 0, s>>>(
+                    n, fst[c]->as<double>() + (int64_t)w * n, Gn.as<double>() + c * n, go,
+                    prm->correction, comp(Un.as<double>(), c, nw, n, w + 1));
+                if ((rc = d2d(go, Gn.as<double>() + c * n, n))) return rc;
+            }
+        }
+        // error = NORM2(U_k+1(nw) - U_k(nw)) / (6N) (:161)
+        double e2 = 0.0;
+        for (int c = 0; c < 6; ++c) {
+            k_diff2<<<64, 256, 0, s>>>(n, comp(Un.as<double>(), c, nw, n, nw - 1),
+                                       comp(Up.as<double>(), c, nw, n, nw - 1), part.as<double>());
+            double h[64];
+            HQ_CHECK_HIP(hipMemcpyAsync(h, part.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            HQ_CHECK_HIP(hipStreamSynchronize(s));
+            for (int b = 0; b < 64; ++b) e2 += h[b];
+        }
+        const double err = sqrt(e2) / (6.0 * (double)n);
+        if (errors) errors[k - 1] = err;
+        if (err <= prm->maxerror) break;
+    }
+    HQ_CHECK_HIP(hipGetLastError());
+    HQ_CHECK_HIP(hipMemcpyAsync(U_out, Un.p, len * 8,
+                                out_where == HQ_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    if (iterations) *iterations = iters;
+    return HQ_OK;
+}
+
+}  // extern "C"

@@ -46,6 +46,10 @@ struct hq_tree {
     // walk scratch
     hq::DevBuf fx, fy, fz, counts, clist, plist, lstat;
     int64_t list_ccap = 0, list_pcap = 0;
+    // integrator state: momenta in input order (positions are x, y, z above),
+    // block offsets of the per-system direct sum, energy scratch
+    hq::DevBuf px, py, pz, dblk, ework;
+    bool have_state = false;
     // status block: device + host mirror
     hq::DevBuf status;
     int64_t* hstatus = nullptr;
@@ -96,4 +100,9 @@ int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s);
 int direct_forces(int64_t n, const double* m, const double* x, const double* y, const double* z,
                   int64_t nt, const int64_t* targets, double G, double eps2, int precision,
                   double* F, hipStream_t s);
+// FP64 direct sum inside each system of the handle, input order, into t->fx/fy/fz
+int direct_forces_systems(hq_tree* t, double G, double eps2);
+// forces of the current positions t->x/y/z into t->fx/fy/fz; *sorted tells
+// whether they are in tree order (map t->idx) or input order
+int forces_now(hq_tree* t, int force, double theta2, double G, double eps2, bool* sorted);
 }  // namespace hq

@@ -145,6 +145,53 @@ class OctreeType:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_SUPER, int(super_size)))
         return self
 
+    # -- time integration (state held in HBM) ----------------------------------
+    def set_state(self, q, p):
+        """Positions and momenta, each (N, 3) numpy or a tuple of three float64
+        arrays (numpy or DeviceArray)."""
+        cols = []
+        for a in (q, p):
+            if isinstance(a, (tuple, list)):
+                cols += [c if _is_device(c) else L.f64(c) for c in a]
+            else:
+                a = np.asarray(a, dtype=np.float64)
+                cols += [np.ascontiguousarray(a[:, c]) for c in range(3)]
+        if any(int(c.shape[0]) != self.N for c in cols):
+            raise ValueError("state arrays must have the length of m")
+        addrs = [_addr(c) for c in cols]
+        if len({w for _, w in addrs}) != 1:
+            raise ValueError("state arrays must all be host or all device")
+        self._state_keep = cols
+        L.check(L.lib().hq_tree_set_state(self._h, *[a for a, _ in addrs], addrs[0][1]))
+        self._cache = None
+        return self
+
+    def get_state(self):
+        """(q, p) as (N, 3) numpy arrays."""
+        cols = [np.zeros(self.N) for _ in range(6)]
+        L.check(L.lib().hq_tree_get_state(self._h, *[C.c_void_p(c.ctypes.data) for c in cols],
+                                          L.HQ_HOST))
+        return np.stack(cols[:3], axis=1), np.stack(cols[3:], axis=1)
+
+    def integrate(self, dt, nsteps, theta2=0.0, G=1.0, eps2=0.0, scheme="verlet", force="tree"):
+        """nsteps steps of dt on the state (rebuilding before every force
+        evaluation).  scheme: "verlet" | "euler"; force: "tree" (the handle's
+        method), "bunched" (bunched walk at the handle's precision) or
+        "direct" (FP64 direct sum within each system)."""
+        sch = {"verlet": L.HQ_SCHEME_VERLET, "euler": L.HQ_SCHEME_EULER}[scheme]
+        frc = {"tree": L.HQ_FORCE_TREE, "bunched": L.HQ_FORCE_BUNCHED,
+               "direct": L.HQ_FORCE_DIRECT}[force]
+        L.check(L.lib().hq_tree_integrate(self._h, sch, dt, int(nsteps), theta2, G, eps2, frc))
+        self._cache = None
+        return self
+
+    def energy(self, G=1.0, eps2=0.0):
+        """Per system [E, P(3), J(3)] of the current state; (nsys, 7)."""
+        ns = 1 if getattr(self, "counts", None) is None else len(self.counts)
+        out = np.zeros((ns, 7))
+        L.check(L.lib().hq_tree_energy(self._h, G, eps2, L.ptr(out)))
+        return out
+
     # -- introspection ----------------------------------------------------------
     def info(self):
         inf = L.TreeInfo()

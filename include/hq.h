@@ -110,6 +110,67 @@ int hq_tree_destroy(hq_tree *t);
 #define HQ_TUNE_FUSED 3   /* 1 (default): walk and evaluate in one kernel; 0: two kernels */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
+/* ---- Time integration around the force path ----------------------------
+ * The handle holds a phase-space state in HBM, input order: positions
+ * (x, y, z) and momenta (px, py, pz), FP64.  Integration updates it in place
+ * and rebuilds the tree before every force evaluation, as the reference's
+ * drivers call init before each force loop. */
+#define HQ_SCHEME_VERLET 0 /* kick-drift-kick in momenta form:
+                              benchmarks/integration_verlet/integration.f90:110-172,
+                              parareal/parareal.f90:46-69 (F_til) */
+#define HQ_SCHEME_EULER 1  /* symplectic Euler, kick then drift:
+                              parareal/parareal.f90:8-44 (G_til) */
+#define HQ_FORCE_TREE 0    /* the handle's method: bh_forces_over_p (10-12),
+                              dehnen_eval (20-22), forces_groups (30-32), FP64 */
+#define HQ_FORCE_BUNCHED 1 /* hq_tree_forces_bunched at the handle's precision */
+#define HQ_FORCE_DIRECT 2  /* FP64 direct sum within each system
+                              (parareal/api3d.f90:46-75, fortran/direct_forces.f90) */
+int hq_tree_set_state(hq_tree *t, const double *x, const double *y, const double *z,
+                      const double *px, const double *py, const double *pz, int where);
+/* Copies the state out; a null pointer skips that component. */
+int hq_tree_get_state(hq_tree *t, double *x, double *y, double *z, double *px, double *py,
+                      double *pz, int where);
+/* nsteps steps of dt: Verlet p += (dt/2) F, q += (dt p)/m, F(q), p += (dt/2) F
+ * (integration.f90:153-171); Euler p += dt F(q), q += (dt p)/m (parareal.f90:30-36).
+ * theta2 and eps2 are squared as in OctreeType%forces; G as given. */
+int hq_tree_integrate(hq_tree *t, int scheme, double dt, int64_t nsteps, double theta2, double G,
+                      double eps2, int force);
+/* Per system k of the handle, out[7k..7k+6] = total energy (kinetic + softened
+ * pair potential, integration.f90:236-262), linear momentum (3) and angular
+ * momentum sum q x p (3) (parareal/api3d.f90:108-155) of the current state. */
+int hq_tree_energy(hq_tree *t, double G, double eps2, double *out);
+
+/* ---- Parareal (parareal/parareal.f90:97-186) ---------------------------
+ * num_windows states U(w) at t = w * tf / num_windows, w = 0..num_windows-1;
+ * an initial coarse sweep, then per iteration the fine propagators of windows
+ * 0..num_windows-2 (one batch handle, all windows at once) and the serial
+ * correction U_k+1(w+1) = G(U_k+1(w)) + F(U_k(w)) - G(U_k(w)); stop when
+ * ||U_k+1(last) - U_k(last)||_2 / (6N) <= maxerror or after itermax iterates.
+ * The reference's propagators are coarse = one Euler step per window with
+ * quadrupole BH forces, fine = Verlet with direct forces (qntt steps). */
+#define HQ_CORR_REFERENCE 0 /* (G_new + F) - G_old, the reference's order */
+#define HQ_CORR_EXACT 1     /* F + (G_new - G_old): a converged window equals the
+                               sequential fine solution bit for bit */
+typedef struct {
+    int64_t n;             /* particles per system */
+    int num_windows;       /* states per trajectory (>= 2) */
+    int itermax;           /* N_itermax: iterates including the coarse sweep */
+    double tf, maxerror, G, eps2;
+    int coarse_force, coarse_method, coarse_precision, coarse_steps;
+    double coarse_theta2;
+    int fine_force, fine_method, fine_precision, fine_steps;
+    double fine_theta2;
+    int group_size;        /* bunch size of bunched walks (<= 0: 64) */
+    int correction;        /* HQ_CORR_REFERENCE or HQ_CORR_EXACT */
+} hq_parareal_params;
+/* U_out: 6 * num_windows * n doubles, component-major:
+ * U_out[(c * num_windows + w) * n + a], c = x, y, z, px, py, pz.
+ * errors: itermax - 1 doubles (the error of iterates 2, 3, ...). */
+int hq_parareal(const hq_parareal_params *prm, const double *m, const double *x0,
+                const double *y0, const double *z0, const double *px0, const double *py0,
+                const double *pz0, int where, double *U_out, int out_where, double *errors,
+                int *iterations);
+
 typedef struct {
     int64_t n;
     int64_t nnodes;        /* number_of_nodes */

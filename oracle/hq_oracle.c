@@ -32,6 +32,10 @@
  *                           build_next :304-326, evaluate_quadrupoles :562-625,
  *                           barnes_hut :627-766, forces_par :150-180
  *   orc_errors_ma           benchmarks/test_forces.f90:66-127
+ *   orc_integrate           benchmarks/integration_verlet/integration.f90:85-172
+ *                           (Verlet), parareal/parareal.f90:8-69 (G_til, F_til)
+ *   orc_energy              integration.f90:236-262, parareal/api3d.f90:108-155
+ *   orc_parareal            parareal/parareal.f90:97-186
  */
 #include <math.h>
 #include <stdint.h>
@@ -1475,4 +1479,162 @@ void orc_errors_ma(int64_t N, const double *ms, const double *qs, const double *
     }
     double r = sqrt(s2) / a_scale;
     errors[0] = sqrt(r * r / (double)N);
+}
+
+/* ------------------------------------------------------------------ */
+/* Time integration (momenta form).                                    */
+/* force: 0 = tree of `method` (10-12, bh_forces_over_p), 2 = direct   */
+/* sum in the reference's pair order (parareal/api3d.f90:46-75, which   */
+/* is fortran/direct_forces.f90's serial loop).                         */
+/* scheme 0 = Verlet (integration.f90:153-171, parareal.f90:57-66),     */
+/* scheme 1 = symplectic Euler (parareal.f90:30-36).                    */
+/* The reference's benchmark drift divides by ms(1) (equal masses);     */
+/* Parareal's by m(a): per-particle masses here, identical for equal.   */
+/* ------------------------------------------------------------------ */
+static int orc_forces_of(int64_t n, const double *m, const double *x, const double *y,
+                         const double *z, int force, int method, double theta2, double G,
+                         double eps2, double *F)
+{
+    if (force == 2) {
+        orc_direct_forces(n, m, x, y, z, G, eps2, F);
+        return ORC_OK;
+    }
+    orc_tree *t = orc_tree_new(n, m, method, 64);
+    if (!t) return ORC_ERR_ALLOC;
+    int st = orc_tree_init(t, x, y, z);
+    if (st == ORC_OK) orc_tree_bh_forces(t, n, NULL, theta2, G, eps2, F, NULL);
+    orc_tree_free(t);
+    return st;
+}
+
+int orc_integrate(int64_t n, const double *m, double *x, double *y, double *z, double *px,
+                  double *py, double *pz, int scheme, double dt, int64_t nsteps, int force,
+                  int method, double theta2, double G, double eps2)
+{
+    double *F = (double *)malloc(sizeof(double) * 3 * (size_t)(n ? n : 1));
+    double *q[3] = {x, y, z}, *p[3] = {px, py, pz};
+    int st = ORC_OK;
+    if (!F) return ORC_ERR_ALLOC;
+    if (nsteps <= 0) goto done;
+    if (scheme == 0) {
+        double h = 0.5 * dt;
+        if ((st = orc_forces_of(n, m, x, y, z, force, method, theta2, G, eps2, F))) goto done;
+        for (int64_t s = 0; s < nsteps; ++s) {
+            for (int64_t a = 0; a < n; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    p[c][a] = p[c][a] + h * F[3 * a + c];
+                    q[c][a] = q[c][a] + dt * p[c][a] / m[a];
+                }
+            if ((st = orc_forces_of(n, m, x, y, z, force, method, theta2, G, eps2, F))) goto done;
+            for (int64_t a = 0; a < n; ++a)
+                for (int c = 0; c < 3; ++c) p[c][a] = p[c][a] + h * F[3 * a + c];
+        }
+    } else {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            if ((st = orc_forces_of(n, m, x, y, z, force, method, theta2, G, eps2, F))) goto done;
+            for (int64_t a = 0; a < n; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    p[c][a] = p[c][a] + dt * F[3 * a + c];
+                    q[c][a] = q[c][a] + dt * p[c][a] / m[a];
+                }
+        }
+    }
+done:
+    free(F);
+    return st;
+}
+
+/* out[0] = E = T + V with the softened pair potential, out[1..3] = sum p,
+ * out[4..6] = sum q x p (integration.f90:236-262; parareal/api3d.f90:108-155) */
+void orc_energy(int64_t n, const double *m, const double *x, const double *y, const double *z,
+                const double *px, const double *py, const double *pz, double G, double eps2,
+                double *out)
+{
+    double T = 0, V = 0, P[3] = {0, 0, 0}, J[3] = {0, 0, 0};
+    for (int64_t a = 0; a < n; ++a) {
+        T = T + (px[a] * px[a] + py[a] * py[a] + pz[a] * pz[a]) / (2.0 * m[a]);
+        for (int64_t b = 0; b < a; ++b) {
+            double dx = x[b] - x[a], dy = y[b] - y[a], dz = z[b] - z[a];
+            double dist = sqrt(dx * dx + dy * dy + dz * dz + eps2);
+            V = V - G * m[a] * m[b] / dist;
+        }
+        P[0] += px[a]; P[1] += py[a]; P[2] += pz[a];
+        J[0] += y[a] * pz[a] - z[a] * py[a];
+        J[1] += z[a] * px[a] - x[a] * pz[a];
+        J[2] += x[a] * py[a] - y[a] * px[a];
+    }
+    out[0] = T + V;
+    for (int c = 0; c < 3; ++c) { out[1 + c] = P[c]; out[4 + c] = J[c]; }
+}
+
+/* integrate_parareal (parareal.f90:97-186) with the reference's correction
+ * order (G_new + F) - G_old.  State of window w (0-based) is U[w*6n ...]:
+ * x, y, z, px, py, pz blocks of n.  U_out gets the last iterate; errors gets
+ * the error of iterates 2.. (itermax - 1 values).  cf/ff: force modes of the
+ * coarse and fine propagators (0 tree of cmethod/fmethod, 2 direct). */
+int orc_parareal(int64_t n, const double *m, const double *u0, double tf, int nw, int itermax,
+                 double maxerror, double G, double eps2, int cf, int cmethod, double ctheta2,
+                 int csteps, int ff, int fmethod, double ftheta2, int fsteps, double *U_out,
+                 double *errors, int *iterations)
+{
+    const size_t W = 6 * (size_t)n;
+    double *Up = (double *)calloc(W * nw, sizeof(double));
+    double *Un = (double *)calloc(W * nw, sizeof(double));
+    double *Ft = (double *)calloc(W * nw, sizeof(double));
+    double *Go = (double *)calloc(W * nw, sizeof(double));
+    double *g = (double *)calloc(W, sizeof(double));
+    int st = ORC_OK, iters = 1;
+    double window = tf / nw, dtc = window / csteps, dtf = window / fsteps;
+    if (!Up || !Un || !Ft || !Go || !g) { st = ORC_ERR_ALLOC; goto out; }
+#define ORC_PROP(dst, src, scheme, dt, steps, force, meth, th2)                                 \
+    do {                                                                                       \
+        memcpy((dst), (src), W * sizeof(double));                                              \
+        st = orc_integrate(n, m, (dst), (dst) + n, (dst) + 2 * n, (dst) + 3 * n, (dst) + 4 * n, \
+                           (dst) + 5 * n, (scheme), (dt), (steps), (force), (meth), (th2), G,   \
+                           eps2);                                                              \
+    } while (0)
+    memcpy(Un, u0, W * sizeof(double));
+    for (int w = 0; w + 1 < nw; ++w) {
+        ORC_PROP(Un + (w + 1) * W, Un + w * W, 1, dtc, csteps, cf, cmethod, ctheta2);
+        if (st) goto out;
+        memcpy(Go + w * W, Un + (w + 1) * W, W * sizeof(double));
+    }
+    for (int k = 1; k < itermax; ++k) {
+        double *tmp = Up; Up = Un; Un = tmp;
+        ++iters;
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+        for (int w = 0; w < nw - 1; ++w) {
+            double *d = Ft + w * W;
+            memcpy(d, Up + w * W, W * sizeof(double));
+            bad |= orc_integrate(n, m, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, 0,
+                                 dtf, fsteps, ff, fmethod, ftheta2, G, eps2);
+        }
+        if (bad) { st = bad; goto out; }
+        memcpy(Un, u0, W * sizeof(double));
+        for (int w = 0; w + 1 < nw; ++w) {
+            ORC_PROP(g, Un + w * W, 1, dtc, csteps, cf, cmethod, ctheta2);
+            if (st) goto out;
+            double *u = Un + (w + 1) * W, *f = Ft + w * W, *go = Go + w * W;
+            for (size_t i = 0; i < W; ++i) {
+                u[i] = g[i] + f[i];
+                u[i] = u[i] - go[i];
+            }
+            memcpy(go, g, W * sizeof(double));
+        }
+        double e2 = 0.0;
+        for (size_t i = 0; i < W; ++i) {
+            double d = Un[(nw - 1) * W + i] - Up[(nw - 1) * W + i];
+            e2 += d * d;
+        }
+        double err = sqrt(e2) / (6.0 * (double)n);
+        if (errors) errors[k - 1] = err;
+        if (err <= maxerror) break;
+    }
+#undef ORC_PROP
+    memcpy(U_out, Un, W * nw * sizeof(double));
+    if (iterations) *iterations = iters;
+out:
+    free(Up); free(Un); free(Ft); free(Go); free(g);
+    return st;
 }

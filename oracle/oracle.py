@@ -79,6 +79,14 @@ def lib():
         L.orc_morton_forces.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
                                         _d]
         L.orc_errors_ma.argtypes = [C.c_int64, _d, _d, _d, _d, _d]
+        L.orc_integrate.argtypes = [C.c_int64, _d, _d, _d, _d, _d, _d, _d, C.c_int, C.c_double,
+                                    C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double]
+        L.orc_energy.argtypes = [C.c_int64, _d, _d, _d, _d, _d, _d, _d, C.c_double, C.c_double,
+                                 _d]
+        L.orc_parareal.argtypes = [C.c_int64, _d, _d, C.c_double, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
+                                   C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _d, _d, _i]
         _LIB = L
     return _LIB
 
@@ -278,3 +286,51 @@ def rel_errors(F, Fd, m=None):
     den = np.linalg.norm(Fd, axis=1)
     r = num / np.where(den > 0, den, 1.0)
     return float(np.sqrt(np.mean(r * r))), float(np.max(r))
+
+
+FORCE_TREE, FORCE_DIRECT = 0, 2
+VERLET, EULER = 0, 1
+
+
+def integrate(m, q, p, scheme, dt, nsteps, force=FORCE_DIRECT, method=11, theta2=0.25, G=1.0,
+              eps2=0.0):
+    """Verlet (scheme 0, integration.f90:153-171) or symplectic Euler (scheme 1,
+    parareal.f90:30-36) from positions q and momenta p, both (N,3); returns the
+    new (q, p)."""
+    m = _f64(m)
+    q = [_f64(np.asarray(q)[:, c]).copy() for c in range(3)]
+    p = [_f64(np.asarray(p)[:, c]).copy() for c in range(3)]
+    st = lib().orc_integrate(len(m), _p(m), *[_p(a) for a in q], *[_p(a) for a in p], scheme,
+                             dt, int(nsteps), force, method, theta2, G, eps2)
+    if st != 0:
+        raise RuntimeError(f"oracle integration failed (status {st})")
+    return np.stack(q, axis=1), np.stack(p, axis=1)
+
+
+def energy(m, q, p, G=1.0, eps2=0.0):
+    """[E, P(3), J(3)] of one system (integration.f90:236-262)."""
+    m = _f64(m)
+    q, p = _f64(q), _f64(p)
+    out = np.zeros(7)
+    cols = [np.ascontiguousarray(a[:, c]) for a in (q, p) for c in range(3)]
+    lib().orc_energy(len(m), _p(m), *[_p(a) for a in cols], G, eps2, _p(out))
+    return out
+
+
+def parareal(m, q0, p0, tf, nw, itermax, maxerror, G, eps2, coarse=(FORCE_TREE, 11, 0.25, 1),
+             fine=(FORCE_DIRECT, 11, 0.25, 16)):
+    """integrate_parareal (parareal.f90:97-186); coarse/fine = (force, method,
+    theta2, steps per window).  Returns (U (nw, 6, N), errors, iterations)."""
+    m = _f64(m)
+    n = len(m)
+    u0 = np.concatenate([_f64(q0).T, _f64(p0).T]).reshape(-1).copy()
+    U = np.zeros(nw * 6 * n)
+    err = np.zeros(max(itermax - 1, 1))
+    it = np.zeros(1, dtype=np.int32)
+    st = lib().orc_parareal(n, _p(m), _p(u0), tf, nw, itermax, maxerror, G, eps2,
+                            coarse[0], coarse[1], coarse[2], coarse[3],
+                            fine[0], fine[1], fine[2], fine[3], _p(U), _p(err), _p(it, _i))
+    if st != 0:
+        raise RuntimeError(f"oracle parareal failed (status {st})")
+    k = int(it[0])
+    return U.reshape(nw, 6, n), err[:k - 1], k
