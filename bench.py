@@ -53,6 +53,147 @@ def cpu_baseline(args):
     return out
 
 
+def cpu_verlet(n, method, theta, eps, steps, seed=0):
+    """Seconds per Verlet step of the reference on the host cores (bounded sample)."""
+    env = dict(os.environ)
+    env.setdefault("OMP_STACKSIZE", "64M")
+    env.setdefault("OMP_PROC_BIND", "close")
+    threads = int(env.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env["OMP_NUM_THREADS"] = str(threads)
+    cmd = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), "--n", str(n),
+           "--seed", str(seed), "--theta", str(theta), "--eps", str(eps), "--method",
+           str(method), "--verlet-steps", str(steps), "--threads", str(threads)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def run_ensemble(args):
+    """Ensemble of Plummer systems with N uniform in [500, 20000], integrated
+    together (one batch handle) with velocity Verlet; particle-steps/s."""
+    from hierarquicos_amd import OctreeType, ics
+    from hierarquicos_amd.device import sync
+    rng = np.random.default_rng(args.seed)
+    sizes = [int(v) for v in rng.integers(500, 20001, args.systems)]
+    sys_ = [ics.plummer(n, seed=1000 + k, velocities=True) for k, n in enumerate(sizes)]
+    m = np.concatenate([a[0] for a in sys_])
+    q = np.concatenate([a[1] for a in sys_])
+    p = np.concatenate([a[2] * a[0][:, None] for a in sys_])
+    ntot = int(sum(sizes))
+    th2, e2, dt = args.theta ** 2, args.eps ** 2, args.dt
+    base = None
+    if not args.no_cpu_baseline:
+        samp = sorted(sizes)[:: max(1, len(sizes) // 4)][:4]
+        recs = [cpu_verlet(n, 11, args.theta, args.eps, 2) for n in samp]
+        if all(recs):
+            rate = sum(samp) / sum(r["s_per_step"] for r in recs)
+            base = {"value": rate, "unit": "particle-steps/s", "cores": recs[0]["cores"],
+                    "kind": recs[0]["kind"],
+                    "sample": (f"2 reference Verlet steps (quadrupole BH, theta {args.theta}) "
+                               f"of each of the systems N={samp}; rate = sum N / sum s_per_step")}
+
+    def make(precision):
+        t = OctreeType().pre_init(m, 11, 64, precision=precision, counts=sizes)
+        t.set_state(q, p)
+        return t
+    t = make("mixed")
+    e0 = t.energy(1.0, e2)
+    t.integrate(dt, args.warmup, th2, 1.0, e2, force="bunched")
+    sync()
+    a = time.perf_counter()
+    t.integrate(dt, args.steps, th2, 1.0, e2, force="bunched")
+    sync()
+    el = time.perf_counter() - a
+    e1 = t.energy(1.0, e2)
+    drift = float(np.max(np.abs((e1[:, 0] - e0[:, 0]) / e0[:, 0])))
+    t64 = make("fp64")
+    t64.integrate(dt, 1, th2, 1.0, e2, force="tree")
+    sync()
+    a = time.perf_counter()
+    t64.integrate(dt, 2, th2, 1.0, e2, force="tree")
+    sync()
+    ms64 = (time.perf_counter() - a) * 1e3 / 2
+    value = ntot * args.steps / el
+    print(json.dumps({
+        "metric": "ensemble Verlet particle-steps/s, 256 Plummer systems N in [500, 20000]",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Plummer spheres with velocities)",
+        "config": {"workload": f"ensemble_{args.systems}_plummer_500_20000_quadrupole",
+                   "systems": args.systems, "N_total": ntot, "theta": args.theta,
+                   "eps": args.eps, "dt": dt,
+                   "forces": "bunched walk, FP32 interactions, quadrupole, one tree per system"},
+        "energy_drift_max_rel": drift,
+        "fp64_tree_ms_per_step": ms64,
+        "cpu_baseline": base}))
+
+
+def run_parareal(args):
+    """Parareal on one N=10,000 Plummer sphere: 64 windows over t in [0, 4];
+    coarse = symplectic Euler, monopole BH theta 0.9, dt ~ 1e-2; fine = Verlet,
+    quadrupole BH theta 0.5 (bunched, FP32 interactions), dt ~ 1e-3; all fine
+    windows in one batch; iterate until ||U_k+1 - U_k|| / (6N) <= 1e-20."""
+    from hierarquicos_amd import OctreeType, ics
+    from hierarquicos_amd import integrate as I
+    from hierarquicos_amd.device import sync
+    n, nw, tf, eps = args.n_parareal, 64, 4.0, args.eps
+    m, q, v = ics.plummer(n, seed=args.seed, velocities=True)
+    p = v * m[:, None]
+    csteps, fsteps = 6, 64
+    coarse = {"force": "tree", "method": 10, "theta": 0.9, "steps": csteps}
+    fine = {"force": "bunched", "method": 11, "theta": args.theta, "steps": fsteps,
+            "precision": "mixed"}
+    dtf = tf / nw / fsteps
+    base = None
+    if not args.no_cpu_baseline:
+        r = cpu_verlet(n, 11, args.theta, eps, 3)
+        if r:
+            base = {"value": r["s_per_step"] * fsteps * (nw - 1), "unit": "s",
+                    "cores": r["cores"], "kind": r["kind"],
+                    "sample": (f"3 reference Verlet steps (quadrupole BH theta {args.theta}, "
+                               f"N={n}) scaled to the sequential fine solution "
+                               f"({nw - 1} windows x {fsteps} steps)")}
+    # warm the library (allocations, code objects) on a short run
+    I.integrate_parareal(m, q, p, tf / 16, 4, 2, 1.0, 1.0, eps, 1e-20, 0, coarse=coarse, fine=fine)
+    sync()
+    a = time.perf_counter()
+    q1, p1, info = I.integrate_parareal(m, q, p, tf, nw, nw + 1, 1.0, 1.0, eps, 1e-20, 0,
+                                        coarse=coarse, fine=fine)
+    t_par = time.perf_counter() - a
+    # the sequential fine solution it converges to
+    t = OctreeType().pre_init(m, 11, 64, precision="mixed")
+    t.set_state(q, p)
+    sync()
+    a = time.perf_counter()
+    dq = dp = 0.0
+    for w in range(nw - 1):
+        t.integrate(dtf, fsteps, args.theta ** 2, 1.0, eps * eps, force="bunched")
+    qs, ps = t.get_state()
+    t_seq = time.perf_counter() - a
+    dq = float(np.linalg.norm(q1[-1] - qs))
+    dp = float(np.linalg.norm(p1[-1] - ps))
+    err_ps = float(np.sqrt(dq * dq + dp * dp) / (6 * n))
+    print(json.dumps({
+        "metric": "Parareal wall time to ||U_k+1 - U_k||/(6N) <= 1e-20, N=10k, 64 windows",
+        "value": t_par, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_par * 1e3, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Plummer sphere with velocities)",
+        "config": {"workload": f"parareal_plummer_n{n}_w{nw}_t4", "N": n, "windows": nw,
+                   "tf": tf, "eps": eps,
+                   "coarse": f"symplectic Euler, monopole BH theta 0.9 (FP64), {csteps} steps "
+                             f"per window (dt {tf / nw / csteps:.4g})",
+                   "fine": f"Verlet, quadrupole BH theta {args.theta} bunched FP32, {fsteps} "
+                           f"steps per window (dt {dtf:.4g})",
+                   "correction": "F + (G_new - G_old)"},
+        "iterations": info["iterations"], "errors": [float(e) for e in info["errors"]],
+        "sequential_fine_s": t_seq, "speedup_vs_sequential": t_seq / t_par,
+        "par_minus_seq": err_ps,
+        "cpu_baseline": base}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,11 +211,24 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64 paths and counts")
+    ap.add_argument("--workload", choices=("forces", "ensemble", "parareal"), default="forces",
+                    help="forces: the headline force evaluation (default)")
+    ap.add_argument("--systems", type=int, default=256, help="ensemble: number of systems")
+    ap.add_argument("--dt", type=float, default=1e-3, help="ensemble: Verlet step")
+    ap.add_argument("--n-parareal", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the force path here runs on a single MI355X; "
               "see DESIGN.md for how it would shard across GPUs.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        if args.eps == 1e-3:
+            args.eps = 1e-2
+        return run_ensemble(args)
+    if args.workload == "parareal":
+        if args.eps == 1e-3:
+            args.eps = 1e-2
+        return run_parareal(args)
 
     from hierarquicos_amd import DeviceArray, OctreeType, direct_forces, ics
     from hierarquicos_amd import _lib as L

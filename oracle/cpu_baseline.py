@@ -42,7 +42,12 @@ def main():
     ap.add_argument("--method", type=int, default=12)
     ap.add_argument("--sample", type=int, default=65536)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--verlet-steps", type=int, default=0,
+                    help="time this many Verlet steps (build + forces of all N each) instead")
+    ap.add_argument("--dt", type=float, default=1e-3)
     args = ap.parse_args()
+    if args.verlet_steps:
+        return verlet(args)
 
     from hierarquicos_amd import ics
     m, q, _ = ics.plummer(args.n, a=1.0, rmax=10.0, seed=args.seed)
@@ -86,6 +91,56 @@ def main():
                    f"targets ({t_walk:.2f} s) on {threads} OpenMP threads, walk scaled to N"),
         "t_build_s": t_build, "t_walk_sample_s": t_walk,
         "checksum": float(np.abs(F).sum())}))
+
+
+def verlet(args):
+    """Seconds per Verlet step of the reference (integration.f90:149-171: half
+    kick, drift, tree%init, the force loop over all N with OpenMP, half kick)
+    on a seeded Plummer system with velocities; the kicks and drift in numpy."""
+    from hierarquicos_amd import ics
+    m, q, v = ics.plummer(args.n, a=1.0, rmax=10.0, seed=args.seed, velocities=True)
+    p = v * m[:, None]
+    th2, e2 = args.theta ** 2, args.eps ** 2
+    threads = args.threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref_path = os.path.join(HERE, "_ref", "libhqref.so")
+    d = C.POINTER(C.c_double)
+    allt = np.arange(args.n, dtype=np.int64)
+
+    def P(a, t=d):
+        return a.ctypes.data_as(t)
+
+    if os.path.exists(ref_path):
+        kind = "reference"
+        R = C.CDLL(ref_path)
+        R.ref_tree_build.argtypes = [C.c_int, d, d, d, d, C.c_int, C.c_int]
+        R.ref_tree_bh_targets.argtypes = [C.c_int, C.POINTER(C.c_int64), C.c_double, C.c_double,
+                                          C.c_double, d]
+
+        def forces(q):
+            x, y, z = (np.ascontiguousarray(q[:, k]) for k in range(3))
+            R.ref_tree_build(args.n, P(m), P(x), P(y), P(z), args.method, 64)
+            F = np.zeros((args.n, 3))
+            R.ref_tree_bh_targets(args.n, P(allt, C.POINTER(C.c_int64)), th2, 1.0, e2, P(F))
+            return F
+    else:
+        kind = "port"
+        import oracle as O
+
+        def forces(q):
+            t = O.Tree(m, args.method).init(q[:, 0], q[:, 1], q[:, 2])
+            return t.bh_forces(th2, 1.0, e2)
+    h = 0.5 * args.dt
+    F = forces(q)
+    t0 = time.perf_counter()
+    for _ in range(args.verlet_steps):
+        p = p + h * F
+        q = q + args.dt * p / m[:, None]
+        F = forces(q)
+        p = p + h * F
+    t1 = time.perf_counter()
+    print(json.dumps({"s_per_step": (t1 - t0) / args.verlet_steps, "n": args.n, "kind": kind,
+                      "cores": threads, "steps": args.verlet_steps, "method": args.method,
+                      "checksum": float(np.abs(q).sum())}))
 
 
 if __name__ == "__main__":
