@@ -207,6 +207,10 @@ def main():
     ap.add_argument("--bunch", type=int, default=64, help="bunch size (<= 64)")
     ap.add_argument("--super", type=int, default=256, help="super-group size of the bunching")
     ap.add_argument("--unfused", action="store_true", help="lists and evaluation as two kernels")
+    ap.add_argument("--leaf", type=int, default=4,
+                    help="leaf bucket: cells of <= leaf bodies enter the lists as their bodies")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick bunch / super-group / leaf bucket with the autotuner first")
     ap.add_argument("--sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -243,7 +247,12 @@ def main():
     dx, dy, dz = (DeviceArray.from_numpy(np.ascontiguousarray(q[:, k])) for k in range(3))
     dF = DeviceArray((3 * N,))
     tree = OctreeType().pre_init(dm, args.method, args.bunch, precision="mixed")
-    tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused)
+    tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused, leaf=args.leaf)
+    tuned = None
+    if args.autotune:
+        tree.init(dx, dy, dz)
+        tuned = tree.autotune(th2, 1.0, e2, "mixed", reps=2)
+        args.bunch, args.super, args.leaf = tuned["bunch"], tuned["super_size"], tuned["leaf"]
 
     def step():
         tree.init(dx, dy, dz)
@@ -353,7 +362,9 @@ def main():
                    "order": {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method],
                    "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
                    "walk": (f"bunched Barnes-1990 lists, <= {args.bunch} particles per bunch, "
-                            f"cut from sibling subtrees packed to <= {args.super}")},
+                            f"cut from sibling subtrees packed to <= {args.super}, leaf bucket "
+                            f"{args.leaf}"),
+                   "autotuned": tuned},
         "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
         "phases_ms": {"build": float(np.mean(builds)), "lists": float(np.mean(lists)),
                       "eval": eval_ms, "keys": info["ms_keys"], "sort": info["ms_sort"],

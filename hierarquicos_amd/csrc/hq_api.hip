@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <memory>
 
 #include "hq_tree.h"
@@ -161,6 +162,9 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
         t->super_size = (int)value;
     } else if (key == HQ_TUNE_FUSED) {
         t->fused = value != 0;
+    } else if (key == HQ_TUNE_LEAF) {
+        if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
+        t->leaf = (int)value;
     } else {
         return fail(HQ_ERR_ARGS, "unknown tuning key %d", key);
     }
@@ -209,6 +213,47 @@ int hq_tree_forces_bunched(hq_tree* t, double theta2, double G, double eps2, dou
     int rc = tree_forces_groups(t, theta2, G, eps2, precision, true);
     if (rc) return rc;
     return tree_scatter_forces(t, F, layout, where);
+}
+
+int hq_tree_autotune(hq_tree* t, double theta2, double G, double eps2, int precision, int reps,
+                     int* bunch, int* super_size, int* leaf, double* ms) {
+    if (!t) return fail(HQ_ERR_ARGS, "null handle");
+    if (!t->built) return fail(HQ_ERR_ARGS, "tree not built (call hq_tree_build)");
+    if (precision != HQ_FP64 && precision != HQ_MIXED) return fail(HQ_ERR_ARGS, "bad precision");
+    if (reps < 1) reps = 1;
+    static const int kB[] = {32, 48, 64}, kS[] = {128, 256, 512}, kL[] = {0, 2, 4, 8};
+    const int b0 = t->group_size, s0 = t->super_size, l0 = t->leaf;
+    int bb = b0, bs = s0, bl = l0;
+    double best = 1e300;
+    int rc;
+    for (int B : kB)
+        for (int S : kS)
+            for (int Lf : kL) {
+                if (S < B) continue;
+                t->group_size = B;
+                t->super_size = S;
+                t->leaf = Lf;
+                double tmin = 1e300;
+                for (int r = 0; r <= reps; ++r) {  // r == 0 warms up
+                    const auto a = std::chrono::steady_clock::now();
+                    if ((rc = tree_forces_groups(t, theta2, G, eps2, precision, true))) {
+                        t->group_size = b0, t->super_size = s0, t->leaf = l0;
+                        return rc;
+                    }
+                    const double el =
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+                    if (r > 0) tmin = std::min(tmin, el);
+                }
+                if (tmin < best) best = tmin, bb = B, bs = S, bl = Lf;
+            }
+    t->group_size = bb;
+    t->super_size = bs;
+    t->leaf = bl;
+    if (bunch) *bunch = bb;
+    if (super_size) *super_size = bs;
+    if (leaf) *leaf = bl;
+    if (ms) *ms = best;
+    return HQ_OK;
 }
 
 int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {

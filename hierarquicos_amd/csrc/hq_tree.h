@@ -41,6 +41,7 @@ struct hq_tree {
     // groups (methods 30-32) and bunches
     hq::DevBuf groups, gflag, bstart;
     int super_size = 256;
+    int leaf = 0;          // bunched walks: cells of <= leaf bodies enter as their bodies (0: off)
     bool fused = true;     // mixed bunched walk as one fused kernel  // bunching: sibling subtrees packed up to this many bodies
     int64_t ngroups = 0;
     // walk scratch

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
                                               const double* __restrict__ sz,
                                               const double* __restrict__ sm, int64_t n,
                                               const int* __restrict__ groups, int ngroups,
-                                              double theta2, ListOut out,
+                                              double theta2, int lbucket, ListOut out,
                                               int64_t* __restrict__ status) {
 #pragma clang fp contract(off)
     __shared__ int stk[SCAP];
@@ -249,7 +249,8 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
         const int base = top - take;
         const int k = lane < take ? stk[base + lane] : -1;
         top = base;
-        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell
+        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell, 4 bucket
+        int bfirst = 0, bcount = 0;
         if (k >= 0) {
             const double mass = nv.mass[k];
             if (k != gnode && mass != 0.0) {
@@ -263,6 +264,10 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
                     action = leaf ? 0 : 1;  // a bunch's own leaves are its in-bunch sum
                 } else if (leaf) {
                     action = 2;
+                } else if (MODE != MODE_GROUP && l - f + 1 <= lbucket) {
+                    action = 4;  // small cell: its bodies, exactly
+                    bfirst = f;
+                    bcount = l - f + 1;
                 } else {
                     const double cc[3] = {nv.cx[k], nv.cy[k], nv.cz[k]};
                     double dist2 = 0.0;
@@ -290,6 +295,28 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
             }
         }
         np += __popcll(mp);
+        if (MODE != MODE_GROUP) {
+            // buckets: the wavefront copies one small cell's bodies at a time
+            uint64_t mb = __ballot(action == 4);
+            while (mb) {
+                const int j = __ffsll((unsigned long long)mb) - 1;
+                mb &= mb - 1;
+                const int f = __shfl(bfirst, j, 64), c = __shfl(bcount, j, 64);
+                if (lane < c) {
+                    const int64_t pos = np + lane;
+                    if (pos < out.pcap) {
+                        const int p = f + lane;
+                        const int64_t o = (int64_t)gi * out.pcap + pos;
+                        if (MODE == MODE_BUNCH32)
+                            out.p32[o] = make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]),
+                                                     (float)(sz[p] - bc[2]), (float)sm[p]);
+                        else
+                            out.pidx[o] = p;
+                    }
+                }
+                np += c;
+            }
+        }
         const uint64_t mc = __ballot(action == 3);
         if (action == 3) {
             const int64_t pos = nc + __popcll(mc & lt);
@@ -723,7 +750,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    const double* __restrict__ sz,
                                                    const double* __restrict__ sm, int64_t n,
                                                    const int* __restrict__ bstart, int nbunch,
-                                                   double theta2, double G, float eps2,
+                                                   double theta2, double G, float eps2, int lbucket,
                                                    const float* __restrict__ mom,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz,
@@ -769,7 +796,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         const int base = top - take;
         const int k = lane < take ? stk[base + lane] : -1;
         top = base;
-        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell
+        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell, 4 bucket
+        int bfirst = 0, bcount = 0;
         if (k >= 0) {
             const double mass = nv.mass[k];
             if (mass != 0.0) {
@@ -780,6 +808,10 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                     action = leaf ? 0 : 1;
                 } else if (leaf) {
                     action = 2;
+                } else if (l - f + 1 <= lbucket) {
+                    action = 4;  // small cell: its bodies, exactly
+                    bfirst = f;
+                    bcount = l - f + 1;
                 } else {
                     const double cc[3] = {nv.cx[k], nv.cy[k], nv.cz[k]};
                     double dist2 = 0.0;
@@ -801,6 +833,31 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                             (float)sm[p]);
         }
         pcnt += __popcll(mp);
+        {
+            // buckets: one small cell's bodies at a time into the particle ring,
+            // draining it so that it never holds more than RING entries
+            __builtin_amdgcn_wave_barrier();
+            uint64_t mb = __ballot(action == 4);
+            while (mb) {
+                const int j = __ffsll((unsigned long long)mb) - 1;
+                mb &= mb - 1;
+                const int f = __shfl(bfirst, j, 64), c = __shfl(bcount, j, 64);
+                while (pcnt >= 64) {
+                    eval_parts(pring, phead, 64, tx, ty, tz, eps2, sum);
+                    phead += 64;
+                    pcnt -= 64;
+                    ntot_p += 64;
+                }
+                if (lane < c) {
+                    const int p = f + lane;
+                    pring[(phead + pcnt + lane) & (RING - 1)] =
+                        make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]),
+                                    (float)(sz[p] - bc[2]), (float)sm[p]);
+                }
+                pcnt += c;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
         const uint64_t mc = __ballot(action == 3);
         if (action == 3)
             cring[(chead + ccnt + __popcll(mc & lt)) & (RING - 1)] =
@@ -1029,7 +1086,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                             t->sy.as<double>(), t->sz.as<double>(),
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
-                                            theta2, G, (float)eps2, t->mom32.as<float>(),
+                                            theta2, G, (float)eps2, t->leaf, t->mom32.as<float>(),
                                             t->fx.as<double>(), t->fy.as<double>(),
                                             t->fz.as<double>(), t->status.as<int64_t>());
             };
@@ -1078,19 +1135,19 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             k_lists<MODE_GROUP><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
                                                        t->sm.as<double>(), t->n,
-                                                       t->groups.as<int>(), ngroups, theta2, lo,
+                                                       t->groups.as<int>(), ngroups, theta2, 0, lo,
                                                        t->status.as<int64_t>());
         else if (!mixed)
             k_lists<MODE_BUNCH><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
                                                        t->sm.as<double>(), t->n, t->bstart.as<int>(),
-                                                       ngroups, theta2, lo, t->status.as<int64_t>());
+                                                       ngroups, theta2, t->leaf, lo, t->status.as<int64_t>());
         else
             k_lists<MODE_BUNCH32><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS,
                                                          t->sx.as<double>(), t->sy.as<double>(),
                                                          t->sz.as<double>(), t->sm.as<double>(),
                                                          t->n, t->bstart.as<int>(), ngroups, theta2,
-                                                         lo, t->status.as<int64_t>());
+                                                         t->leaf, lo, t->status.as<int64_t>());
         HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
         if ((rc = read_status(t))) return rc;
         const int64_t flags = t->hstatus[ST_FLAGS];

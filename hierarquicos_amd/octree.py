@@ -133,10 +133,12 @@ class OctreeType:
             return lib.hq_tree_forces_bunched(h, th2, g, e2, p, layout, where, prec)
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
-    def tune(self, bunch=None, super_size=None, fused=None):
-        """Bunch size (<= 64), super-group size and fused/two-kernel mode of the
-        bunched walk."""
+    def tune(self, bunch=None, super_size=None, fused=None, leaf=None):
+        """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
+        bucket (cells of <= leaf bodies enter as their bodies) of the bunched walk."""
         lib = L.lib()
+        if leaf is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_LEAF, int(leaf)))
         if fused is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FUSED, int(bool(fused))))
         if bunch is not None:
@@ -191,6 +193,16 @@ class OctreeType:
         out = np.zeros((ns, 7))
         L.check(L.lib().hq_tree_energy(self._h, G, eps2, L.ptr(out)))
         return out
+
+    def autotune(self, theta2, G=1.0, eps2=0.0, precision="mixed", reps=2):
+        """Times the bunched walk of the built tree over bunch / super-group /
+        leaf-bucket settings and keeps the fastest; returns it with its time."""
+        prec = {"fp64": L.HQ_FP64, "mixed": L.HQ_MIXED}[precision]
+        b, s_, lf = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        L.check(L.lib().hq_tree_autotune(self._h, theta2, G, eps2, prec, reps, C.byref(b),
+                                         C.byref(s_), C.byref(lf), C.byref(ms)))
+        return {"bunch": b.value, "super_size": s_.value, "leaf": lf.value, "ms": ms.value}
 
     # -- introspection ----------------------------------------------------------
     def info(self):

@@ -108,7 +108,19 @@ int hq_tree_destroy(hq_tree *t);
 #define HQ_TUNE_BUNCH 1
 #define HQ_TUNE_SUPER 2
 #define HQ_TUNE_FUSED 3   /* 1 (default): walk and evaluate in one kernel; 0: two kernels */
+#define HQ_TUNE_LEAF 4    /* 0 (default) .. 64: in bunched walks a cell of at most this
+                             many bodies enters the lists as its bodies (exact P2P),
+                             the leaf-bucket size of a bucketed tree */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
+/* Autotuner (no reference counterpart): on the built tree, times the bunched
+ * walk at `precision` for bunch sizes {32, 48, 64}, super-group sizes
+ * {128, 256, 512} and leaf buckets {0, 2, 4, 8} (best of `reps` runs each,
+ * after one warm-up), keeps the fastest setting on the handle and returns it
+ * with its time in milliseconds.  Results of later bunched walks depend on
+ * the setting (within the walk's error budget), so a caller that needs
+ * bit-reproducible forces fixes the knobs with hq_tree_tune instead. */
+int hq_tree_autotune(hq_tree *t, double theta2, double G, double eps2, int precision, int reps,
+                     int *bunch, int *super_size, int *leaf, double *ms);
 
 /* ---- Time integration around the force path ----------------------------
  * The handle holds a phase-space state in HBM, input order: positions
@@ -139,6 +151,14 @@ int hq_tree_integrate(hq_tree *t, int scheme, double dt, int64_t nsteps, double 
  * pair potential, integration.f90:236-262), linear momentum (3) and angular
  * momentum sum q x p (3) (parareal/api3d.f90:108-155) of the current state. */
 int hq_tree_energy(hq_tree *t, double G, double eps2, double *out);
+
+/* ---- Harness error metric ----------------------------------------------
+ * evaluate_error_on_accelerations (benchmarks/test_forces.f90:66-127), with
+ * a = F/m: errors[0] = sqrt((||a - a_dir||_F / a_scale)^2 / N), a_scale =
+ * sum m/|q|^2 (McMillan-Aarseth); errors[1..3] = sum F; errors[4..6] = sum q x F.
+ * F and Fd in the given layout; all arrays host or all device. */
+int hq_errors_ma(int64_t n, const double *m, const double *x, const double *y, const double *z,
+                 const double *F, const double *Fd, int layout, int where, double *errors);
 
 /* ---- Parareal (parareal/parareal.f90:97-186) ---------------------------
  * num_windows states U(w) at t = w * tf / num_windows, w = 0..num_windows-1;

@@ -1,0 +1,72 @@
+"""GPU: tuning knobs of the bunched walk (leaf buckets, bunch and super-group
+sizes, fused or two-kernel) and the autotuner.
+
+A leaf bucket replaces a small cell by its bodies, which is exact; so with
+theta = 0 (every cell opened) each setting must give the direct sum, and at
+the usual theta the error against the FP64 direct sum can only shrink.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from hierarquicos_amd import OctreeType, direct_forces, ics
+
+pytestmark = pytest.mark.gpu
+
+
+def _tree(n, seed, method=12):
+    m, q, _ = ics.plummer(n, seed=seed)
+    x, y, z = q.T.copy()
+    return m, x, y, z, OctreeType().pre_init(m, method).init(x, y, z)
+
+
+@pytest.mark.parametrize("leaf", [0, 3, 16, 64])
+@pytest.mark.parametrize("fused", [True, False])
+def test_theta_zero_is_direct_for_every_bucket(leaf, fused):
+    m, x, y, z, t = _tree(6000, 21)
+    t.tune(leaf=leaf, fused=fused)
+    Fd = O.direct_targets(m, x, y, z, np.arange(6000), 1.0, 1e-6)
+    F64 = t.forces_bunched(0.0, 1.0, 1e-6, "fp64").T
+    assert float(np.max(np.linalg.norm(F64 - Fd, axis=1) / np.linalg.norm(Fd, axis=1))) < 1e-12
+    Fm = t.forces_bunched(0.0, 1.0, 1e-6, "mixed").T
+    Fmd = direct_forces(m, x, y, z, 1.0, 1e-6, precision="mixed")
+    rms, mx = O.rel_errors(Fm, Fmd)
+    assert rms < 2e-6 and mx < 2e-5, (rms, mx)
+
+
+def test_buckets_do_not_increase_error():
+    m, x, y, z, t = _tree(40000, 22)
+    sample = np.random.default_rng(1).choice(40000, 2048, replace=False)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    errs, cells, parts = {}, {}, {}
+    for leaf in (0, 4, 16):
+        t.tune(leaf=leaf)
+        F = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+        errs[leaf] = O.rel_errors(F[sample], Fd)[0]
+        inf = t.info()
+        cells[leaf], parts[leaf] = inf["exec_cells"], inf["exec_parts"]
+    # buckets trade cell interactions for body interactions
+    assert cells[16] < cells[4] < cells[0] and parts[16] > parts[4] > parts[0], (cells, parts)
+    assert errs[4] <= errs[0] * 1.001 and errs[16] <= errs[4] * 1.001, errs
+    assert errs[0] < 1e-4
+
+
+def test_fused_and_two_kernel_agree_with_buckets():
+    m, x, y, z, t = _tree(30000, 23)
+    t.tune(leaf=6)
+    a = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    t.tune(fused=False)
+    b = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    rel = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    assert float(rel.max()) < 1e-5
+
+
+def test_autotune_picks_a_valid_setting():
+    m, x, y, z, t = _tree(100000, 24)
+    r = t.autotune(0.25, 1.0, 1e-6, "mixed", reps=1)
+    assert r["bunch"] in (32, 48, 64) and r["super_size"] in (128, 256, 512)
+    assert r["leaf"] in (0, 2, 4, 8) and r["ms"] > 0
+    sample = np.arange(0, 100000, 97)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    F = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    assert O.rel_errors(F[sample], Fd)[0] < 1e-4

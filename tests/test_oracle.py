@@ -242,3 +242,24 @@ def test_tree_invariants_and_capacity():
                 assert np.isclose(e["half"][c], e["half"][k] / 2)
                 for a in ("cx", "cy", "cz"):
                     assert abs(e[a][c] - e[a][k]) <= e["half"][k] * (1 + 1e-15)
+
+
+def test_errors_ma_closed_form():
+    """evaluate_error_on_accelerations (test_forces.f90:66-127) on a case the
+    formula fixes by hand: N unit masses on the unit sphere, every
+    acceleration off by (1, 0, 0), tangential forces of known torque."""
+    n = 12
+    rng = np.random.default_rng(0)
+    q = rng.normal(size=(n, 3))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    m = np.ones(n)
+    Fd = np.zeros((n, 3))
+    F = Fd + np.array([1.0, 0.0, 0.0])
+    e = O.errors_ma(m, q, F, Fd)
+    # ||da|| = sqrt(N), a_scale = N  =>  errors(1) = sqrt((sqrt(N)/N)^2 / N) = 1/N
+    assert abs(e[0] - 1.0 / n) < 1e-15
+    assert np.allclose(e[1:4], [n, 0, 0])
+    assert np.allclose(e[4:7], np.cross(q, F).sum(0), atol=1e-14)
+    # masses enter a = F/m and a_scale: doubling both m and F leaves errors(1) / 2
+    e2 = O.errors_ma(2 * m, q, 2 * F, 2 * Fd)
+    assert abs(e2[0] - 0.5 / n) < 1e-15
