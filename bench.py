@@ -42,7 +42,8 @@ def cpu_baseline(args):
     env["OMP_NUM_THREADS"] = str(threads)
     cmd = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), "--n", str(args.n),
            "--seed", str(args.seed), "--theta", str(args.theta), "--eps", str(args.eps),
-           "--method", "12", "--sample", str(args.cpu_sample), "--threads", str(threads)]
+           "--method", str(args.method), "--profile", args.profile, "--sample",
+           str(args.cpu_sample), "--threads", str(threads)]
     r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
     if r.returncode != 0:
         return {"value": None, "unit": "particles/s", "cores": threads, "kind": "reference",
@@ -197,13 +198,14 @@ def run_parareal(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--eps", type=float, default=1e-3)
     ap.add_argument("--method", type=int, default=12)
+    ap.add_argument("--profile", choices=("plummer", "hernquist", "cube"), default="plummer")
     ap.add_argument("--bunch", type=int, default=64, help="bunch size (<= 64)")
     ap.add_argument("--super", type=int, default=256, help="super-group size of the bunching")
     ap.add_argument("--unfused", action="store_true", help="lists and evaluation as two kernels")
@@ -239,7 +241,12 @@ def main():
     from hierarquicos_amd.device import sync
 
     N = args.n
-    m, q, _ = ics.plummer(N, a=1.0, rmax=10.0, seed=args.seed)
+    if args.profile == "plummer":
+        m, q, _ = ics.plummer(N, a=1.0, rmax=10.0, seed=args.seed)
+    elif args.profile == "hernquist":
+        m, q, _ = ics.hernquist(N, a=1.0, rmax=50.0, seed=args.seed)
+    else:
+        m, q, _ = ics.uniform_cube(N, half=1.0, seed=args.seed)
     base = None if args.no_cpu_baseline else cpu_baseline(args)   # before the GPU is touched
 
     th2, e2 = args.theta ** 2, args.eps ** 2
@@ -267,10 +274,12 @@ def main():
     for _ in range(args.warmup):
         step()
     sync()
-    evals, lists, builds = [], [], []
+    evals, lists, builds, walls = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        a = time.perf_counter()
+        step()                     # synchronous: returns when the forces are in HBM
+        walls.append((time.perf_counter() - a) * 1e3)
         inf = tree.info()          # device (HIP event) times of this step's kernels
         evals.append(inf["ms_eval"])
         lists.append(inf["ms_lists"])
@@ -280,6 +289,14 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = N / (ms_step * 1e-3)
     info = tree.info()
+    q25, q50, q75 = (float(v) for v in np.percentile(walls, [25, 50, 75]))
+
+    # the same step from host arrays to host forces (PCIe both ways included)
+    hx, hy, hz = (np.ascontiguousarray(q[:, k]) for k in range(3))
+    a = time.perf_counter()
+    tree.init(hx, hy, hz)
+    tree.forces_bunched(th2, 1.0, e2, "mixed")
+    ms_host = (time.perf_counter() - a) * 1e3
 
     # accuracy against the FP64 direct sum on a sample of targets
     rng = np.random.default_rng(1234)
@@ -327,13 +344,15 @@ def main():
         }
 
     eval_ms = float(np.mean(evals))
-    kernel = "k_eval_mixed<8>" if args.unfused else "k_walk_mixed<8,1536>"
-    workload = f"plummer_n{N}_octupole_theta{args.theta}_eps{args.eps}"
+    mp = {10: 1, 11: 4, 12: 8}[args.method]
+    kernel = f"k_eval_mixed<{mp}>" if args.unfused else f"k_walk_mixed<{mp},1536>"
+    order = {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method]
+    workload = f"{args.profile}_n{N}_{order}_theta{args.theta}_eps{args.eps}"
     traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
     tf = os.path.join(REPO, "profiles", f"traffic_k_walk_mixed_{workload.split('_theta')[0]}.json")
     if os.path.exists(tf) and not args.unfused:
         rec = json.load(open(tf))
-        if rec.get("workload") == workload and args.method == 12:
+        if rec.get("workload") == workload and rec.get("leaf", 0) == args.leaf:
             traffic = rec["hbm_bytes_per_launch"]
     roof = None
     if flops_alg is not None:
@@ -357,15 +376,18 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Plummer sphere)",
-        "config": {"workload": f"plummer_n{N}_octupole_theta{args.theta}_eps{args.eps}",
+        "config": {"workload": workload, "profile": args.profile,
                    "N": N, "seed": args.seed, "theta": args.theta, "eps": args.eps,
-                   "order": {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method],
+                   "order": order,
                    "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
                    "walk": (f"bunched Barnes-1990 lists, <= {args.bunch} particles per bunch, "
                             f"cut from sibling subtrees packed to <= {args.super}, leaf bucket "
                             f"{args.leaf}"),
                    "autotuned": tuned},
         "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
+        "ms_per_step_median": q50, "ms_per_step_iqr": [q25, q75],
+        "host_arrays_ms_per_step": ms_host,
+        "host_arrays_particles_per_s": N / (ms_host * 1e-3),
         "phases_ms": {"build": float(np.mean(builds)), "lists": float(np.mean(lists)),
                       "eval": eval_ms, "keys": info["ms_keys"], "sort": info["ms_sort"],
                       "nodes": info["ms_nodes"], "upward": info["ms_upward"]},

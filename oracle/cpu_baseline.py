@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--eps", type=float, default=1e-3)
     ap.add_argument("--method", type=int, default=12)
+    ap.add_argument("--profile", choices=("plummer", "hernquist", "cube"), default="plummer")
     ap.add_argument("--sample", type=int, default=65536)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--verlet-steps", type=int, default=0,
@@ -50,7 +51,12 @@ def main():
         return verlet(args)
 
     from hierarquicos_amd import ics
-    m, q, _ = ics.plummer(args.n, a=1.0, rmax=10.0, seed=args.seed)
+    if args.profile == "hernquist":
+        m, q, _ = ics.hernquist(args.n, a=1.0, rmax=50.0, seed=args.seed)
+    elif args.profile == "cube":
+        m, q, _ = ics.uniform_cube(args.n, half=1.0, seed=args.seed)
+    else:
+        m, q, _ = ics.plummer(args.n, a=1.0, rmax=10.0, seed=args.seed)
     x, y, z = (np.ascontiguousarray(q[:, k]) for k in range(3))
     th2, e2 = args.theta ** 2, args.eps ** 2
     sample = np.linspace(0, args.n - 1, min(args.sample, args.n)).astype(np.int64)
@@ -85,7 +91,7 @@ def main():
     total = t_build + t_walk * args.n / len(sample)
     print(json.dumps({
         "value": args.n / total, "unit": "particles/s", "cores": threads, "kind": kind,
-        "sample": (f"Plummer N={args.n} seed {args.seed}, method {args.method}, theta "
+        "sample": (f"{args.profile} N={args.n} seed {args.seed}, method {args.method}, theta "
                    f"{args.theta}, eps {args.eps}: full insertion build + multipoles "
                    f"({t_build:.2f} s) and bh_forces_over_p on {len(sample)} evenly spaced "
                    f"targets ({t_walk:.2f} s) on {threads} OpenMP threads, walk scaled to N"),
