@@ -20,7 +20,8 @@ HQ_ERR_NONFINITE, HQ_ERR_DEPTH, HQ_ERR_NOMEM, HQ_ERR_OVERFLOW = 5, 6, 7, 8
 HQ_HOST, HQ_DEVICE = 0, 1
 HQ_FP64, HQ_MIXED = 0, 1
 HQ_LAYOUT_3N, HQ_LAYOUT_N3 = 0, 1
-HQ_TUNE_BUNCH, HQ_TUNE_SUPER, HQ_TUNE_FUSED, HQ_TUNE_LEAF = 1, 2, 3, 4
+HQ_TUNE_BUNCH, HQ_TUNE_SUPER, HQ_TUNE_FUSED, HQ_TUNE_LEAF, HQ_TUNE_FMM_LEAF = 1, 2, 3, 4, 5
+HQ_FMM_MFMA, HQ_FMM_FP64 = 0, 1
 HQ_SCHEME_VERLET, HQ_SCHEME_EULER = 0, 1
 HQ_FORCE_TREE, HQ_FORCE_BUNCHED, HQ_FORCE_DIRECT = 0, 1, 2
 HQ_CORR_REFERENCE, HQ_CORR_EXACT = 0, 1
@@ -48,7 +49,7 @@ class PararealParams(C.Structure):
 EXPORTS = (
     "hq_compute_forces_direct", "hq_compute_forces", "hq_generate_tree", "hq_tree_create",
     "hq_tree_create_batch", "hq_tree_set_state", "hq_tree_get_state", "hq_tree_integrate",
-    "hq_tree_energy", "hq_parareal", "hq_errors_ma", "hq_tree_autotune",
+    "hq_tree_energy", "hq_parareal", "hq_errors_ma", "hq_tree_autotune", "hq_tree_fmm",
     "hq_tree_set_masses", "hq_tree_build", "hq_tree_forces", "hq_tree_forces_bunched",
     "hq_tree_destroy", "hq_tree_tune", "hq_tree_info", "hq_tree_export", "hq_tree_order",
     "hq_direct_forces_device", "hq_morton_keys", "hq_morton_create", "hq_morton_build",
@@ -102,6 +103,8 @@ def lib():
     L.hq_tree_energy.argtypes = [_vp, C.c_double, C.c_double, _d]
     L.hq_tree_autotune.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                    _i, _i, _i, _d]
+    L.hq_tree_fmm.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp,
+                              C.c_int, C.c_int]
     L.hq_errors_ma.argtypes = [C.c_int64] + [_vp] * 6 + [C.c_int, C.c_int, _d]
     L.hq_parareal.argtypes = [C.POINTER(PararealParams)] + [_vp] * 7 + [C.c_int, _vp, C.c_int,
                                                                         _d, _i]

@@ -1,6 +1,7 @@
 // hq_api.hip -- the C ABI (include/hq.h) over the tree, walk, direct and
 // Morton kernels.  Host-side orchestration only; every force path runs on
 // the GPU.  Errors are returned as status codes with a per-thread message.
+#include <math.h>
 #include <stdarg.h>
 #include <string.h>
 
@@ -162,6 +163,9 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
         t->super_size = (int)value;
     } else if (key == HQ_TUNE_FUSED) {
         t->fused = value != 0;
+    } else if (key == HQ_TUNE_FMM_LEAF) {
+        if (value < 1 || value > 64) return fail(HQ_ERR_ARGS, "FMM leaf size %lld outside 1..64", (long long)value);
+        t->fmm_leaf = (int)value;
     } else if (key == HQ_TUNE_LEAF) {
         if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
         t->leaf = (int)value;
@@ -254,6 +258,20 @@ int hq_tree_autotune(hq_tree* t, double theta2, double G, double eps2, int preci
     if (leaf) *leaf = bl;
     if (ms) *ms = best;
     return HQ_OK;
+}
+
+int hq_tree_fmm(hq_tree* t, int order, double theta2, double G, double eps2, int mode, double* F,
+                int layout, int where) {
+    if (!t || !F) return fail(HQ_ERR_ARGS, "null argument");
+    if (!t->built) return fail(HQ_ERR_ARGS, "tree not built (call hq_tree_build)");
+    if (order != 4 && order != 6) return fail(HQ_ERR_ARGS, "FMM order %d (4 or 6)", order);
+    if (mode != HQ_FMM_MFMA && mode != HQ_FMM_FP64) return fail(HQ_ERR_ARGS, "bad FMM mode %d", mode);
+    if (!(theta2 > 0.0) || theta2 >= 1.0) return fail(HQ_ERR_ARGS, "FMM needs 0 < theta^2 < 1");
+    if (layout != HQ_LAYOUT_3N && layout != HQ_LAYOUT_N3) return fail(HQ_ERR_ARGS, "bad layout");
+    if (t->lost > 0) return fail(HQ_ERR_LOST, "body lost (%lld particles at max depth)", (long long)t->lost);
+    int rc = tree_forces_fmm(t, order, sqrt(theta2), G, eps2, mode == HQ_FMM_MFMA, t->fmm_leaf);
+    if (rc) return rc;
+    return tree_scatter_forces(t, F, layout, where);
 }
 
 int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {

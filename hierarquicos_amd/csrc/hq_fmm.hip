@@ -1,0 +1,1051 @@
+// hq_fmm.hip -- Cartesian fast multipole method of order p = 4 or 6 on the
+// octree of hq_build.hip (beyond the reference: its Dehnen method,
+// fortran/octree_default.f90:1095-1712, stops at quadrupole sources).
+//
+// Expansions are about the geometric cube centres, scaled by the cube side s:
+//   raw moments   Q~_m = sum_j m_j ((x_j - c) / s)^m                 |m| <= p
+//   local coeffs  L~_n = s^(|n|+1) L_n,  phi(c + u) = sum_n L_n u^n  |n| <= p
+// so that M2M and L2L between a cube and its children use constant
+// coefficients, and an M2L between two cubes of one level depends only on the
+// integer offset o = (c_B - c_A) / s:
+//   L~_A = T_o Q~_B,   T_o[n][m] = -(-1)^|m| C(n+m, n) a_{n+m}(o),
+// a_k = D^k(1/r) / k! (Taylor coefficients, computed by their three-term
+// recurrence).  A dual-tree walk (Dehnen 2002, mutual form of
+// octree_default.f90:1132-1248, here directed) pairs cells with the MAC
+// rmax_A + rmax_B <= theta |c_A - c_B|.  Same-level accepted pairs are binned
+// by (target group, offset) and evaluated as dense GEMMs on the matrix cores
+// (v_mfma_f32_16x16x4f32, FP32 in and out); pairs of different levels, or all
+// pairs in the FP64 mode, use a scalar FP64 M2L.  Near pairs of leaf cells
+// (<= S bodies) are summed directly; local expansions are passed down (L2L)
+// and evaluated at the bodies (L2P) in FP64.  Pair lists are binned with
+// atomic cursors and the scalar M2L adds with atomics, so sums are not
+// bit-reproducible run to run (round-off level).
+#include <math.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "hq_tree.h"
+
+namespace hq {
+
+constexpr int PMAX = 6;
+constexpr int KMAX = 84;     // (p+1)(p+2)(p+3)/6 at p = 6
+constexpr int K2MAX = 455;   // same at 2p = 12
+constexpr int GT = 512;      // targets per MFMA workgroup (8 waves x 4 column tiles x 16)
+
+__host__ __device__ constexpr int ncoef(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+__host__ __device__ constexpr int kpad(int p) { return (ncoef(p) + 15) / 16 * 16; }
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ host tables
+namespace {
+
+struct MI { int x, y, z; };
+
+std::vector<MI> graded(int p) {  // all multi-indices with |n| <= p, graded order
+    std::vector<MI> v;
+    for (int q = 0; q <= p; ++q)
+        for (int a = q; a >= 0; --a)
+            for (int b = q - a; b >= 0; --b) v.push_back({a, b, q - a - b});
+    return v;
+}
+
+double binom(int n, int k) {
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// Taylor coefficients a_k(R) = D^k(1/|R|) / k! for the multi-indices `ks`
+// (graded), from |k| r^2 a_k = -(2|k|-1) sum_d R_d a_{k-e_d} - (|k|-1) sum_d a_{k-2e_d}
+void taylor(const double R[3], const std::vector<MI>& ks, const std::vector<int>& rec,
+            std::vector<double>& a) {
+    const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+    a.assign(ks.size(), 0.0);
+    a[0] = 1.0 / sqrt(r2);
+    for (size_t i = 1; i < ks.size(); ++i) {
+        const int n = ks[i].x + ks[i].y + ks[i].z;
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            if (rec[6 * i + d] >= 0) s += (2 * n - 1) * R[d] * a[rec[6 * i + d]];
+            if (rec[6 * i + 3 + d] >= 0) s += (n - 1) * a[rec[6 * i + 3 + d]];
+        }
+        a[i] = -s / (n * r2);
+    }
+}
+
+}  // namespace
+
+struct FmmTables {
+    int p = 0, K = 0, Kp = 0, K2 = 0, R = 0, nd = 0, noff = 0;
+    double theta = -1.0;
+    DevBuf mi, mi2, rec2, m2lidx, m2lcoef, m2m, l2l, T;
+};
+
+struct FmmState {
+    FmmTables tab;
+    DevBuf flag, scan, fid, fnode, Qd, Qf, Ld, Lf, rmax, lvl;
+    DevBuf fa, fb, cnt, lat, gen, p2p, m2p, bins, binpos, lsrc, lcol, phist, ppos, plist;
+    DevBuf mhist, mpos, mlist;
+    int64_t nf = 0;
+    std::vector<int> fls;  // FMM node range per depth
+};
+
+void fmm_state_free(FmmState* s) { delete s; }
+
+static int build_tables(FmmTables& tb, int p, double theta) {
+    if (tb.p == p && tb.theta == theta) return HQ_OK;
+    const std::vector<MI> k1 = graded(PMAX), k2 = graded(2 * PMAX);
+    const int K = ncoef(p), Kp = kpad(p), K2 = ncoef(2 * p);
+    std::vector<int> pos(13 * 13 * 13, -1);
+    for (size_t i = 0; i < k2.size(); ++i) pos[(k2[i].x * 13 + k2[i].y) * 13 + k2[i].z] = (int)i;
+    auto at = [&](int x, int y, int z) { return (x < 0 || y < 0 || z < 0) ? -1 : pos[(x * 13 + y) * 13 + z]; };
+    std::vector<int> rec(6 * k2.size());
+    for (size_t i = 0; i < k2.size(); ++i) {
+        const MI k = k2[i];
+        rec[6 * i + 0] = at(k.x - 1, k.y, k.z);
+        rec[6 * i + 1] = at(k.x, k.y - 1, k.z);
+        rec[6 * i + 2] = at(k.x, k.y, k.z - 1);
+        rec[6 * i + 3] = at(k.x - 2, k.y, k.z);
+        rec[6 * i + 4] = at(k.x, k.y - 2, k.z);
+        rec[6 * i + 5] = at(k.x, k.y, k.z - 2);
+    }
+    // M2L index and coefficient -(-1)^|m| C(n+m, n) on the K x K block
+    std::vector<int> midx(K * K);
+    std::vector<double> mcoef(K * K);
+    for (int n = 0; n < K; ++n)
+        for (int m = 0; m < K; ++m) {
+            const MI a = k1[n], b = k1[m];
+            midx[n * K + m] = at(a.x + b.x, a.y + b.y, a.z + b.z);
+            const double sgn = ((b.x + b.y + b.z) & 1) ? 1.0 : -1.0;
+            mcoef[n * K + m] = sgn * binom(a.x + b.x, a.x) * binom(a.y + b.y, a.y) * binom(a.z + b.z, a.z);
+        }
+    // M2M / L2L for the child with digit dg (bits 4x + 2y + z: 1 = upper half),
+    // child centre = parent centre + delta s_parent / 4
+    std::vector<double> m2m(8 * K * K, 0.0), l2l(8 * K * K, 0.0);
+    for (int dg = 0; dg < 8; ++dg) {
+        const double del[3] = {(dg & 4) ? 1.0 : -1.0, (dg & 2) ? 1.0 : -1.0, (dg & 1) ? 1.0 : -1.0};
+        for (int n = 0; n < K; ++n)
+            for (int m = 0; m < K; ++m) {
+                const MI a = k1[n], b = k1[m];
+                if (b.x <= a.x && b.y <= a.y && b.z <= a.z) {
+                    // Q~_p,n += C(n,m) (delta/4)^(n-m) 2^-|m| Q~_c,m
+                    double c = binom(a.x, b.x) * binom(a.y, b.y) * binom(a.z, b.z) *
+                               pow(del[0] / 4, a.x - b.x) * pow(del[1] / 4, a.y - b.y) *
+                               pow(del[2] / 4, a.z - b.z);
+                    m2m[(dg * K + n) * K + m] = c * ldexp(1.0, -(b.x + b.y + b.z));
+                    // L~_c,m += C(n,m) (delta/4)^(n-m) 2^-(|m|+1) L~_p,n
+                    l2l[(dg * K + m) * K + n] = c * ldexp(1.0, -(b.x + b.y + b.z) - 1);
+                }
+            }
+    }
+    // lattice operators T_o for |o_d| <= R, o != 0
+    const int R = std::max(2, (int)floor(2.0 * sqrt(3.0) / theta + sqrt(3.0)));
+    const int nd = 2 * R + 1, noff = nd * nd * nd;
+    std::vector<MI> ks2(k2.begin(), k2.begin() + K2);
+    std::vector<float> T((size_t)noff * Kp * Kp, 0.0f);
+    std::vector<double> a;
+    for (int o = 0; o < noff; ++o) {
+        const int ox = o / (nd * nd) - R, oy = (o / nd) % nd - R, oz = o % nd - R;
+        if (!ox && !oy && !oz) continue;
+        // o is the source's offset from the target; the expansion needs
+        // r = c_A - c_B = -o s
+        const double Ro[3] = {(double)-ox, (double)-oy, (double)-oz};
+        taylor(Ro, ks2, rec, a);
+        float* t = T.data() + (size_t)o * Kp * Kp;
+        for (int n = 0; n < K; ++n)
+            for (int m = 0; m < K; ++m) t[n * Kp + m] = (float)(mcoef[n * K + m] * a[midx[n * K + m]]);
+    }
+    std::vector<int> mi(4 * KMAX), mi2(4 * K2MAX);
+    for (int i = 0; i < KMAX; ++i) {
+        mi[4 * i] = k1[i].x, mi[4 * i + 1] = k1[i].y, mi[4 * i + 2] = k1[i].z;
+        mi[4 * i + 3] = k1[i].x + k1[i].y + k1[i].z;
+    }
+    for (int i = 0; i < K2MAX; ++i) {
+        mi2[4 * i] = k2[i].x, mi2[4 * i + 1] = k2[i].y, mi2[4 * i + 2] = k2[i].z;
+        mi2[4 * i + 3] = k2[i].x + k2[i].y + k2[i].z;
+    }
+    int rc;
+    auto up = [&](DevBuf& d, const void* h, size_t b) -> int {
+        int r;
+        if ((r = d.ensure(b))) return r;
+        HQ_CHECK_HIP(hipMemcpy(d.p, h, b, hipMemcpyHostToDevice));
+        return HQ_OK;
+    };
+    if ((rc = up(tb.mi, mi.data(), mi.size() * 4)) || (rc = up(tb.mi2, mi2.data(), mi2.size() * 4)) ||
+        (rc = up(tb.rec2, rec.data(), rec.size() * 4)) ||
+        (rc = up(tb.m2lidx, midx.data(), midx.size() * 4)) ||
+        (rc = up(tb.m2lcoef, mcoef.data(), mcoef.size() * 8)) ||
+        (rc = up(tb.m2m, m2m.data(), m2m.size() * 8)) || (rc = up(tb.l2l, l2l.data(), l2l.size() * 8)) ||
+        (rc = up(tb.T, T.data(), T.size() * 4)))
+        return rc;
+    tb.p = p, tb.K = K, tb.Kp = Kp, tb.K2 = K2, tb.R = R, tb.nd = nd, tb.noff = noff, tb.theta = theta;
+    return HQ_OK;
+}
+
+// ------------------------------------------------------------------ node sets
+// flag: 0 outside the FMM tree, 1 leaf (<= S bodies, or a tree leaf), 2 internal
+__global__ __launch_bounds__(256) void k_fmm_flag(int64_t nn, int S, const int* __restrict__ count,
+                                                  const int8_t* __restrict__ type,
+                                                  const uint8_t* __restrict__ depth,
+                                                  const int* __restrict__ parent,
+                                                  int8_t* __restrict__ flag,
+                                                  uint32_t* __restrict__ one) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int c = count[k], p = parent[k];
+    int8_t f = 0;
+    if (c > 0 && (p < 0 || count[p] > S)) f = (c <= S || type[k] == 1 || depth[k] >= MAX_DEPTH) ? 1 : 2;
+    flag[k] = f;
+    one[k] = f ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_fmm_compact(int64_t nn, const int8_t* __restrict__ flag,
+                                                     const uint32_t* __restrict__ scan,
+                                                     int* __restrict__ fid, int* __restrict__ fnode) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int f = flag[k] ? (int)scan[k] : -1;
+    fid[k] = f;
+    if (f >= 0) fnode[f] = (int)k;
+}
+
+__global__ void k_fmm_levels(const uint32_t* __restrict__ scan, const int* __restrict__ ls,
+                             int* __restrict__ out) {
+    const int d = threadIdx.x;
+    if (d <= MAX_DEPTH + 1) out[d] = (int)scan[ls[d]];
+}
+
+struct FView {
+    const int *first, *count, *cstart, *fid, *fnode, *ssys;
+    const int8_t* flag;
+    const uint8_t *depth, *digit;
+    const double *cx, *cy, *cz, *sg;  // cube centres; per-system geometry table
+};
+
+__device__ __forceinline__ double side_of(const FView& v, int k) {
+    const int sys = v.ssys[v.first[k]];
+    return 2.0 * v.sg[SG * sys + S_HALFD + v.depth[k]];
+}
+
+// ------------------------------------------------------------------ upward
+// P2M of a leaf: one wavefront, lanes over bodies for the powers, then over
+// coefficients for the sums.
+template <int P>
+__global__ __launch_bounds__(64) void k_fmm_p2m(int nf, FView v, const double* __restrict__ sx,
+                                                const double* __restrict__ sy,
+                                                const double* __restrict__ sz,
+                                                const double* __restrict__ sm,
+                                                const int* __restrict__ mi, double* __restrict__ Qd,
+                                                double* __restrict__ rmax) {
+    constexpr int K = ncoef(P);
+    __shared__ double pw[3][P + 1][64];
+    __shared__ double mass[64];
+    const int f = blockIdx.x;
+    if (f >= nf) return;
+    const int k = v.fnode[f];
+    if (v.flag[k] != 1) return;
+    const int lane = threadIdx.x;
+    const int first = v.first[k], cnt = v.count[k];
+    const double s = side_of(v, k), c[3] = {v.cx[k], v.cy[k], v.cz[k]};
+    double r = 0.0;
+    if (lane < cnt) {
+        const int j = first + lane;
+        const double y[3] = {sx[j] - c[0], sy[j] - c[1], sz[j] - c[2]};
+        r = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        for (int d = 0; d < 3; ++d) {
+            const double t = s > 0.0 ? y[d] / s : 0.0;
+            double w = 1.0;
+            for (int e = 0; e <= P; ++e) { pw[d][e][lane] = w; w *= t; }
+        }
+        mass[lane] = sm[j];
+    }
+    for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+    __syncthreads();
+    for (int n = lane; n < K; n += 64) {
+        const int a = mi[4 * n], b = mi[4 * n + 1], cz = mi[4 * n + 2];
+        double acc = 0.0;
+        for (int j = 0; j < cnt; ++j) acc += mass[j] * pw[0][a][j] * pw[1][b][j] * pw[2][cz][j];
+        Qd[(int64_t)f * K + n] = acc;
+    }
+    if (lane == 0) rmax[f] = r;
+}
+
+// M2M of the internal FMM nodes of one level, one wavefront per node
+template <int P>
+__global__ __launch_bounds__(64) void k_fmm_m2m(int f0, int f1, FView v, const double* __restrict__ m2m,
+                                                double* __restrict__ Qd, double* __restrict__ rmax) {
+    constexpr int K = ncoef(P);
+    const int f = f0 + blockIdx.x;
+    if (f >= f1) return;
+    const int k = v.fnode[f];
+    if (v.flag[k] != 2) return;
+    const int lane = threadIdx.x;
+    const double s = side_of(v, k);
+    double acc[2] = {0.0, 0.0}, r = 0.0;
+    for (int c = v.cstart[k]; c < v.cstart[k + 1]; ++c) {
+        if (v.count[c] == 0) continue;
+        const int fc = v.fid[c];
+        const double* q = Qd + (int64_t)fc * K;
+        const double* t = m2m + (int64_t)v.digit[c] * K * K;
+        for (int h = 0; h < 2; ++h) {
+            const int n = lane + 64 * h;
+            if (n < K) {
+                double a = acc[h];
+                for (int m = 0; m <= n; ++m) a += t[n * K + m] * q[m];
+                acc[h] = a;
+            }
+        }
+        r = fmax(r, rmax[fc] + 0.4330127018922193 * s);  // |c_child - c| = sqrt(3) s / 4
+    }
+    for (int h = 0; h < 2; ++h)
+        if (lane + 64 * h < K) Qd[(int64_t)f * K + lane + 64 * h] = acc[h];
+    if (lane == 0) rmax[f] = fmin(r, 0.8660254037844386 * s);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_qf(int64_t nf, const double* __restrict__ Qd,
+                                                float* __restrict__ Qf) {
+    constexpr int K = ncoef(P), Kp = kpad(P);
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nf * Kp) return;
+    const int64_t f = i / Kp;
+    const int n = (int)(i % Kp);
+    Qf[i] = n < K ? (float)Qd[f * K + n] : 0.0f;
+}
+
+// ------------------------------------------------------------------ dual walk
+enum { ACT_EXPAND = 0, ACT_LATTICE = 1, ACT_GENERIC = 2, ACT_P2P = 3, ACT_M2P = 4, ACT_NONE = 5 };
+constexpr int NACT = 5;
+
+struct WalkPar {
+    double theta;
+    int R, nd;
+};
+
+// Classifies the directed pair (a <- b): accept with the MAC, sum directly
+// when both are leaves, else split (both when of one level, else the larger
+// or the non-leaf).  *nch = child pairs produced by a split; *off = lattice
+// offset id for same-level accepts.
+__device__ __forceinline__ int fmm_classify(const FView& v, const double* __restrict__ rmax,
+                                            const WalkPar& w, int a, int b, int* nch, int* off,
+                                            int* split) {
+    *nch = 0;
+    *off = -1;
+    *split = 0;
+    const int fa = v.fid[a], fb = v.fid[b];
+    const bool la = v.flag[a] == 1, lb = v.flag[b] == 1;
+    const int da = v.depth[a], db = v.depth[b];
+    auto nkids = [&](int k) {
+        int n = 0;
+        for (int c = v.cstart[k]; c < v.cstart[k + 1]; ++c) n += v.count[c] > 0;
+        return n;
+    };
+    if (a != b) {
+        const double dx = v.cx[b] - v.cx[a], dy = v.cy[b] - v.cy[a], dz = v.cz[b] - v.cz[a];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (rmax[fa] + rmax[fb] <= w.theta * d) {
+            // a leaf target much larger than its source: the local expansion
+            // would carry the larger error, so the source acts on the bodies
+            // directly (its multipole, or its bodies when it is a leaf)
+            if (la && (rmax[fa] > 2.0 * rmax[fb] || (rmax[fa] > rmax[fb] && v.count[a] <= 8)))
+                return lb ? ACT_P2P : ACT_M2P;
+            if (da == db) {
+                const double s = side_of(v, a);
+                const int ox = (int)rint(dx / s), oy = (int)rint(dy / s), oz = (int)rint(dz / s);
+                if (abs(ox) <= w.R && abs(oy) <= w.R && abs(oz) <= w.R) {
+                    *off = ((ox + w.R) * w.nd + (oy + w.R)) * w.nd + (oz + w.R);
+                    return ACT_LATTICE;
+                }
+            }
+            return ACT_GENERIC;
+        }
+    }
+    if (la && lb) return ACT_P2P;
+    // split: 1 = a, 2 = b, 3 = both
+    int sp;
+    if (a == b) sp = 3;
+    else if (la) sp = 2;
+    else if (lb) sp = 1;
+    else if (da == db) sp = 3;
+    else sp = da < db ? 1 : 2;
+    *split = sp;
+    const int na = (sp & 1) ? nkids(a) : 1, nb = (sp & 2) ? nkids(b) : 1;
+    *nch = na * nb;
+    return ACT_EXPAND;
+}
+
+constexpr int WB = 256;
+
+// per block: counts of next-frontier pairs, lattice, generic, p2p
+__global__ __launch_bounds__(WB) void k_fmm_walk_count(FView v, const double* __restrict__ rmax,
+                                                       WalkPar w, const int2* __restrict__ fr,
+                                                       int64_t nfr, uint32_t* __restrict__ cnt,
+                                                       int nblk) {
+    __shared__ uint32_t ws[NACT][WB / 64];
+    const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
+    uint32_t c[NACT] = {0, 0, 0, 0, 0};
+    if (i < nfr) {
+        int nch, off, sp;
+        const int act = fmm_classify(v, rmax, w, fr[i].x, fr[i].y, &nch, &off, &sp);
+        if (act == ACT_EXPAND) c[0] = nch;
+        else c[act] = 1;
+    }
+    for (int q = 0; q < NACT; ++q) {
+        const uint32_t t = wave_sum(c[q]);
+        if (__lane_id() == 0) ws[q][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NACT) {
+        uint32_t t = 0;
+        for (int k = 0; k < WB / 64; ++k) t += ws[threadIdx.x][k];
+        cnt[(int64_t)threadIdx.x * (nblk + 1) + blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(WB) void k_fmm_walk_emit(FView v, const double* __restrict__ rmax,
+                                                      WalkPar w, const int2* __restrict__ fr,
+                                                      int64_t nfr, const uint32_t* __restrict__ cnt,
+                                                      int nblk, int2* __restrict__ next,
+                                                      int3* __restrict__ lat, int64_t lat0,
+                                                      int2* __restrict__ gen, int64_t gen0,
+                                                      int2* __restrict__ p2p, int64_t p2p0,
+                                                      int2* __restrict__ m2p, int64_t m2p0) {
+    __shared__ uint32_t ws[NACT][WB / 64];
+    const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
+    int act = ACT_NONE, nch = 0, off = -1, sp = 0, a = 0, b = 0;
+    if (i < nfr) {
+        a = fr[i].x;
+        b = fr[i].y;
+        act = fmm_classify(v, rmax, w, a, b, &nch, &off, &sp);
+    }
+    uint32_t c[NACT] = {act == ACT_EXPAND ? (uint32_t)nch : 0u, act == ACT_LATTICE ? 1u : 0u,
+                        act == ACT_GENERIC ? 1u : 0u, act == ACT_P2P ? 1u : 0u,
+                        act == ACT_M2P ? 1u : 0u};
+    uint32_t pre[NACT];
+    for (int q = 0; q < NACT; ++q) {
+        const uint32_t incl = wave_incl_scan(c[q]);
+        pre[q] = incl - c[q];
+        if (__lane_id() == 63) ws[q][threadIdx.x >> 6] = incl;
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6;
+    for (int q = 0; q < NACT; ++q) {
+        uint32_t base = cnt[(int64_t)q * (nblk + 1) + blockIdx.x];
+        for (int k = 0; k < wv; ++k) base += ws[q][k];
+        pre[q] += base;
+    }
+    if (act == ACT_EXPAND) {
+        const int a0 = (sp & 1) ? v.cstart[a] : a, a1 = (sp & 1) ? v.cstart[a + 1] : a + 1;
+        const int b0 = (sp & 2) ? v.cstart[b] : b, b1 = (sp & 2) ? v.cstart[b + 1] : b + 1;
+        int64_t o = pre[0];
+        for (int x = a0; x < a1; ++x) {
+            if ((sp & 1) && v.count[x] == 0) continue;
+            for (int y = b0; y < b1; ++y) {
+                if ((sp & 2) && v.count[y] == 0) continue;
+                next[o++] = make_int2(x, y);
+            }
+        }
+    } else if (act == ACT_LATTICE) {
+        lat[lat0 + pre[1]] = make_int3(v.fid[a], v.fid[b], off);
+    } else if (act == ACT_GENERIC) {
+        gen[gen0 + pre[2]] = make_int2(v.fid[a], v.fid[b]);
+    } else if (act == ACT_P2P) {
+        p2p[p2p0 + pre[3]] = make_int2(v.fid[a], v.fid[b]);
+    } else if (act == ACT_M2P) {
+        m2p[m2p0 + pre[4]] = make_int2(v.fid[a], v.fid[b]);
+    }
+}
+
+// ------------------------------------------------------------------ binning
+// lattice pairs -> bins (target group, offset); near pairs -> bins (target)
+__global__ __launch_bounds__(256) void k_fmm_lat_hist(const int3* __restrict__ lat, int64_t n,
+                                                      int noff, uint32_t* __restrict__ hist) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int3 e = lat[i];
+        atomicAdd(&hist[(int64_t)(e.x / GT) * noff + e.z], 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fmm_lat_scatter(const int3* __restrict__ lat, int64_t n,
+                                                         int noff, uint32_t* __restrict__ pos,
+                                                         int* __restrict__ src,
+                                                         int16_t* __restrict__ col) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int3 e = lat[i];
+        const uint32_t o = atomicAdd(&pos[(int64_t)(e.x / GT) * noff + e.z], 1u);
+        src[o] = e.y;
+        col[o] = (int16_t)(e.x % GT);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fmm_p2p_hist(const int2* __restrict__ pr, int64_t n,
+                                                      uint32_t* __restrict__ hist) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&hist[pr[i].x], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict__ pr, int64_t n,
+                                                         uint32_t* __restrict__ pos,
+                                                         int* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = atomicAdd(&pos[pr[i].x], 1u);
+        out[o] = pr[i].y;
+    }
+}
+
+// ------------------------------------------------------------------ M2L
+// Lattice M2L on the matrix cores.  One workgroup per group of GT target
+// nodes; for every offset with pairs in the group, T_o is staged in LDS and
+// L~[:, targets] += T_o Q~[:, sources] runs as 16x16x4 FP32 MFMA tiles, the
+// accumulators staying in registers across offsets (no atomics).
+// v_mfma_f32_16x16x4f32 operand layout (wave64): A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k, D[4q + r][j] in lane j + 16q, element r.
+template <int P>
+__global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int nsplit,
+                                                      const uint32_t* __restrict__ bstart,
+                                                      const int* __restrict__ src,
+                                                      const int16_t* __restrict__ col,
+                                                      const float* __restrict__ T,
+                                                      const float* __restrict__ Qf,
+                                                      float* __restrict__ Lf) {
+    constexpr int Kp = kpad(P), RT = Kp / 16, KT = Kp / 4, CT = GT / 16 / 8;
+    __shared__ float tl[Kp * Kp];
+    __shared__ int colsrc[GT];
+    __shared__ uint32_t bs[GT + 1];  // bin starts of the current chunk of offsets
+    // block (g, sp): target group g, offsets [o_lo, o_hi); with several
+    // splits per group the partial sums meet in Lf by atomic adds
+    const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
+    const int per = (noff + nsplit - 1) / nsplit;
+    const int o_lo = sp * per, o_hi = min(noff, o_lo + per);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    floatx4 acc[CT][RT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) acc[ct][rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int o = o_lo; o < o_hi; ++o) {
+        if ((o - o_lo) % GT == 0) {  // next chunk of GT offsets: their bin starts into LDS
+            __syncthreads();
+            const int64_t base = (int64_t)g * noff + o;
+            const int cnt = min(GT, o_hi - o);
+            for (int i = tid; i <= cnt; i += 512) bs[i] = bstart[base + i];
+            __syncthreads();
+        }
+        const uint32_t b0 = bs[(o - o_lo) % GT], b1 = bs[(o - o_lo) % GT + 1];
+        if (b0 == b1) continue;
+        __syncthreads();
+        colsrc[tid] = -1;
+        const float* t = T + (int64_t)o * Kp * Kp;
+        for (int i = tid; i < Kp * Kp; i += 512) tl[i] = t[i];
+        __syncthreads();
+        for (uint32_t e = b0 + tid; e < b1; e += 512) colsrc[col[e]] = src[e];
+        __syncthreads();
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int cbase = wv * (GT / 8) + ct * 16;
+            const int s = colsrc[cbase + li];
+            if (__ballot(s >= 0) == 0) continue;
+            float bv[KT];
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) bv[kt] = s >= 0 ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt)
+                    acc[ct][rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        tl[(16 * rt + li) * Kp + 4 * kt + lk], bv[kt], acc[ct][rt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int64_t f = (int64_t)g * GT + wv * (GT / 8) + ct * 16 + li;
+        if (f >= nf) continue;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float* dst = &Lf[f * Kp + 16 * rt + 4 * lk + r];
+                if (nsplit == 1) *dst = acc[ct][rt][r];
+                else if (acc[ct][rt][r] != 0.0f) atomicAdd(dst, acc[ct][rt][r]);
+            }
+    }
+}
+
+// FP64 M2L of one directed pair per wavefront, any two levels:
+// L~_A,n += sum_m coef[n][m] (s_B / s_A)^|m| Q~_B,m a_{n+m}((c_A - c_B) / s_A)
+template <int P>
+__global__ __launch_bounds__(64) void k_fmm_m2l_fp64(const int2* __restrict__ pr, int64_t npr,
+                                                     const int3* __restrict__ lat, int64_t nlat,
+                                                     FView v, const int* __restrict__ mi,
+                                                     const int* __restrict__ mi2,
+                                                     const int* __restrict__ rec2,
+                                                     const int* __restrict__ midx,
+                                                     const double* __restrict__ mcoef,
+                                                     const double* __restrict__ Qd,
+                                                     double* __restrict__ Ld) {
+    constexpr int K = ncoef(P), K2 = ncoef(2 * P);
+    __shared__ double a[K2];
+    __shared__ double q[K];
+    for (int64_t i = blockIdx.x; i < npr + nlat; i += gridDim.x) {
+        int fa, fb;
+        if (i < npr) fa = pr[i].x, fb = pr[i].y;
+        else fa = lat[i - npr].x, fb = lat[i - npr].y;
+        const int lane = threadIdx.x;
+        __syncthreads();  // the previous pair is done with a[] and q[]
+        const int ka = v.fnode[fa], kb = v.fnode[fb];
+        const double sa = side_of(v, ka), sb = side_of(v, kb);
+        const double R[3] = {(v.cx[ka] - v.cx[kb]) / sa, (v.cy[ka] - v.cy[kb]) / sa,
+                             (v.cz[ka] - v.cz[kb]) / sa};
+        const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+        const double ratio = sb / sa;
+        for (int m = lane; m < K; m += 64) q[m] = Qd[(int64_t)fb * K + m] * pow(ratio, mi[4 * m + 3]);
+        if (lane == 0) a[0] = 1.0 / sqrt(r2);
+        __syncthreads();
+        int lo = 1;
+        for (int ord = 1; ord <= 2 * P; ++ord) {
+            const int hi = lo + (ord + 1) * (ord + 2) / 2;
+            for (int j = lo + lane; j < hi; j += 64) {
+                double s = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    const int e1 = rec2[6 * j + d], e2 = rec2[6 * j + 3 + d];
+                    if (e1 >= 0) s += (2 * ord - 1) * R[d] * a[e1];
+                    if (e2 >= 0) s += (ord - 1) * a[e2];
+                }
+                a[j] = -s / (ord * r2);
+            }
+            __syncthreads();
+            lo = hi;
+        }
+        for (int n = lane; n < K; n += 64) {
+            double acc = 0.0;
+            for (int m = 0; m < K; ++m) acc += mcoef[n * K + m] * q[m] * a[midx[n * K + m]];
+            unsafeAtomicAdd(&Ld[(int64_t)fa * K + n], acc);
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_merge(int64_t nf, const float* __restrict__ Lf,
+                                                   double* __restrict__ Ld) {
+    constexpr int K = ncoef(P), Kp = kpad(P);
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nf * K) return;
+    const int64_t f = i / K;
+    const int n = (int)(i % K);
+    Ld[i] += (double)Lf[f * Kp + n];
+}
+
+// ------------------------------------------------------------------ downward
+template <int P>
+__global__ __launch_bounds__(64) void k_fmm_l2l(int f0, int f1, FView v, const int* __restrict__ par,
+                                                const double* __restrict__ l2l,
+                                                double* __restrict__ Ld) {
+    constexpr int K = ncoef(P);
+    const int f = f0 + blockIdx.x;
+    if (f >= f1) return;
+    const int k = v.fnode[f];
+    const int p = par[k];
+    if (p < 0) return;
+    const int fp = v.fid[p];
+    if (fp < 0) return;
+    const double* t = l2l + (int64_t)v.digit[k] * K * K;
+    const double* lp = Ld + (int64_t)fp * K;
+    for (int m = threadIdx.x; m < K; m += 64) {
+        double acc = 0.0;
+        for (int n = 0; n < K; ++n) acc += t[m * K + n] * lp[n];
+        Ld[(int64_t)f * K + m] += acc;
+    }
+}
+
+// Leaves: L2P (FP64) plus the near field from the leaf's near list, one
+// wavefront per leaf, one lane per body.  FP32 mode sums the near field in
+// FP32 on coordinates recentred on the leaf centre, Kahan-compensated per
+// source leaf; FP64 mode in FP64.
+template <int P, bool F32>
+__global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ sz,
+                                                 const double* __restrict__ sm,
+                                                 const int* __restrict__ mi,
+                                                 const int* __restrict__ mi2,
+                                                 const int* __restrict__ rec2,
+                                                 const double* __restrict__ Ld,
+                                                 const double* __restrict__ Qd,
+                                                 const uint32_t* __restrict__ pstart,
+                                                 const int* __restrict__ plist,
+                                                 const uint32_t* __restrict__ mstart,
+                                                 const int* __restrict__ mlist, double G,
+                                                 double eps2, double* __restrict__ fx,
+                                                 double* __restrict__ fy, double* __restrict__ fz) {
+    constexpr int K = ncoef(P), K1 = ncoef(P + 1);
+    __shared__ double4 src[64];
+    __shared__ double at[K1][64];  // per-lane Taylor coefficients of the M2P
+    const int f = blockIdx.x;
+    if (f >= nf) return;
+    const int k = v.fnode[f];
+    if (v.flag[k] != 1) return;
+    const int lane = threadIdx.x;
+    const int first = v.first[k], cnt = v.count[k];
+    const bool ok = lane < cnt;
+    const int i = first + (ok ? lane : 0);
+    const double s = side_of(v, k), c[3] = {v.cx[k], v.cy[k], v.cz[k]};
+    const double xi = sx[i], yi = sy[i], zi = sz[i];
+    // L2P: a = -(1/s^2) sum_n L~_n n_d u~^(n - e_d); a zero-size cube (one
+    // body, or bodies at one point) has no far field
+    const double si = s > 0.0 ? 1.0 / s : 0.0;
+    const double u[3] = {(xi - c[0]) * si, (yi - c[1]) * si, (zi - c[2]) * si};
+    double pw[3][P + 1];
+    for (int d = 0; d < 3; ++d) {
+        pw[d][0] = 1.0;
+        for (int e = 1; e <= P; ++e) pw[d][e] = pw[d][e - 1] * u[d];
+    }
+    double g[3] = {0.0, 0.0, 0.0};
+    const double* L = Ld + (int64_t)f * K;
+    for (int n = 1; n < K; ++n) {
+        const int nx = mi[4 * n], ny = mi[4 * n + 1], nz = mi[4 * n + 2];
+        const double l = L[n];
+        if (nx) g[0] += l * nx * pw[0][nx - 1] * pw[1][ny] * pw[2][nz];
+        if (ny) g[1] += l * ny * pw[0][nx] * pw[1][ny - 1] * pw[2][nz];
+        if (nz) g[2] += l * nz * pw[0][nx] * pw[1][ny] * pw[2][nz - 1];
+    }
+    const double sc = -si * si;
+    double ax = sc * g[0], ay = sc * g[1], az = sc * g[2];
+    // near field
+    float ks[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
+    const float tx = (float)(xi - c[0]), ty = (float)(yi - c[1]), tz = (float)(zi - c[2]);
+    for (uint32_t e = pstart[f]; e < pstart[f + 1]; ++e) {
+        const int kb = v.fnode[plist[e]];
+        const int bf = v.first[kb], bc = v.count[kb];
+        __syncthreads();
+        if (lane < bc) {
+            const int j = bf + lane;
+            src[lane] = F32 ? make_double4((double)(float)(sx[j] - c[0]), (double)(float)(sy[j] - c[1]),
+                                           (double)(float)(sz[j] - c[2]), sm[j])
+                            : make_double4(sx[j], sy[j], sz[j], sm[j]);
+        }
+        __syncthreads();
+        if (F32) {
+            float p[3] = {0.f, 0.f, 0.f};
+            const float e2 = (float)eps2;
+            for (int j = 0; j < bc; ++j) {
+                if (bf + j == i) continue;
+                const double4 q = src[j];
+                const float dx = (float)q.x - tx, dy = (float)q.y - ty, dz = (float)q.z - tz;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
+                const float ri = __frsqrt_rn(d2);
+                const float w = (float)q.w * ri * ri * ri;
+                p[0] = fmaf(w, dx, p[0]);
+                p[1] = fmaf(w, dy, p[1]);
+                p[2] = fmaf(w, dz, p[2]);
+            }
+            for (int d = 0; d < 3; ++d) {
+                const float yv = p[d] - kc[d];
+                const float tv = ks[d] + yv;
+                kc[d] = (tv - ks[d]) - yv;
+                ks[d] = tv;
+            }
+        } else {
+            for (int j = 0; j < bc; ++j) {
+                if (bf + j == i) continue;
+                const double4 q = src[j];
+                const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const double d2 = dx * dx + dy * dy + dz * dz + eps2;
+                const double w = q.w / (sqrt(d2) * d2);
+                ax += w * dx;
+                ay += w * dy;
+                az += w * dz;
+            }
+        }
+    }
+    if (F32) {
+        ax += (double)ks[0] - (double)kc[0];
+        ay += (double)ks[1] - (double)kc[1];
+        az += (double)ks[2] - (double)kc[2];
+    }
+    // M2P: a_d = s^-2 sum_m (-1)^|m| Q~_m (m_d + 1) a_{m+e_d}((x - c_B) / s),
+    // accumulated as each a_k (|k| <= p+1) is produced
+    for (uint32_t e = mstart[f]; e < mstart[f + 1]; ++e) {
+        const int fb = mlist[e];
+        const int kb = v.fnode[fb];
+        const double sb = side_of(v, kb);
+        const double R[3] = {(xi - v.cx[kb]) / sb, (yi - v.cy[kb]) / sb, (zi - v.cz[kb]) / sb};
+        const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+        const double* q = Qd + (int64_t)fb * K;
+        double g2[3] = {0.0, 0.0, 0.0};
+        at[0][lane] = 1.0 / sqrt(r2);
+        for (int j = 1; j < K1; ++j) {
+            const int ord = mi2[4 * j + 3];
+            double sacc = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const int e1 = rec2[6 * j + d], e2 = rec2[6 * j + 3 + d];
+                if (e1 >= 0) sacc += (2 * ord - 1) * R[d] * at[e1][lane];
+                if (e2 >= 0) sacc += (ord - 1) * at[e2][lane];
+            }
+            const double aj = -sacc / (ord * r2);
+            at[j][lane] = aj;
+            for (int d = 0; d < 3; ++d) {
+                const int mm = rec2[6 * j + d];  // index of k - e_d
+                if (mm >= 0 && ord - 1 <= P) {
+                    const double sg = ((ord - 1) & 1) ? -1.0 : 1.0;
+                    g2[d] += sg * q[mm] * (double)mi2[4 * j + d] * aj;
+                }
+            }
+        }
+        const double sc2 = 1.0 / (sb * sb);
+        ax += sc2 * g2[0];
+        ay += sc2 * g2[1];
+        az += sc2 * g2[2];
+    }
+    if (ok) {
+        const double gm = G * sm[i];
+        fx[i] = gm * ax;
+        fy[i] = gm * ay;
+        fz[i] = gm * az;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fmm_roots(int nsys, int2* __restrict__ fr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsys) fr[k] = make_int2(k, k);
+}
+
+// ------------------------------------------------------------------ driver
+// workgroups for a grid-stride loop over n items, 256 per group
+static inline int capped(int64_t n) { return (int)std::min<int64_t>(grid_for(std::max<int64_t>(n, 1), 256), 1 << 16); }
+
+template <int P>
+static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2, bool mfma,
+                   int S) {
+    constexpr int K = ncoef(P), Kp = kpad(P);
+    hipStream_t s = t->stream;
+    int rc;
+    FmmTables& tb = st.tab;
+    if ((rc = build_tables(tb, P, theta))) return rc;
+    const int64_t nn = t->nn, n = t->n;
+    // 1. FMM node set
+    if ((rc = st.flag.ensure(nn)) || (rc = st.scan.ensure((nn + 1) * 4)) ||
+        (rc = st.fid.ensure(nn * 4)) || (rc = st.lvl.ensure(64 * 4)))
+        return rc;
+    k_fmm_flag<<<grid_for(nn, 256), 256, 0, s>>>(nn, S, t->ncount.as<int>(), t->ntype.as<int8_t>(),
+                                                 t->ndepth.as<uint8_t>(), t->nparent.as<int>(),
+                                                 st.flag.as<int8_t>(), st.scan.as<uint32_t>());
+    if ((rc = scan_excl_buf(st.scan.as<uint32_t>(), nn, t->scanpart, s))) return rc;
+    uint32_t nf32 = 0;
+    HQ_CHECK_HIP(hipMemcpyAsync(&nf32, st.scan.as<uint32_t>() + nn, 4, hipMemcpyDeviceToHost, s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    const int64_t nf = nf32;
+    st.nf = nf;
+    if ((rc = st.fnode.ensure(std::max<int64_t>(nf, 1) * 4))) return rc;
+    k_fmm_compact<<<grid_for(nn, 256), 256, 0, s>>>(nn, st.flag.as<int8_t>(), st.scan.as<uint32_t>(),
+                                                    st.fid.as<int>(), st.fnode.as<int>());
+    {
+        std::vector<int> ls(MAX_DEPTH + 2);
+        for (int d = 0; d <= MAX_DEPTH + 1; ++d) ls[d] = std::min<int64_t>(t->level_start[d], nn);
+        DevBuf dls;
+        if ((rc = dls.ensure(ls.size() * 4))) return rc;
+        HQ_CHECK_HIP(hipMemcpyAsync(dls.p, ls.data(), ls.size() * 4, hipMemcpyHostToDevice, s));
+        k_fmm_levels<<<1, 64, 0, s>>>(st.scan.as<uint32_t>(), dls.as<int>(), st.lvl.as<int>());
+        st.fls.assign(MAX_DEPTH + 2, 0);
+        HQ_CHECK_HIP(hipMemcpyAsync(st.fls.data(), st.lvl.p, (MAX_DEPTH + 2) * 4, hipMemcpyDeviceToHost, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    FView v;
+    v.first = t->nfirst.as<int>();
+    v.count = t->ncount.as<int>();
+    v.cstart = t->ncstart.as<int>();
+    v.fid = st.fid.as<int>();
+    v.fnode = st.fnode.as<int>();
+    v.ssys = t->ssys.as<int>();
+    v.flag = st.flag.as<int8_t>();
+    v.depth = t->ndepth.as<uint8_t>();
+    v.digit = t->ndigit.as<uint8_t>();
+    v.cx = t->ncx.as<double>();
+    v.cy = t->ncy.as<double>();
+    v.cz = t->ncz.as<double>();
+    v.sg = t->geom.as<double>() + G_SYS;
+    // 2. upward
+    if ((rc = st.Qd.ensure(nf * K * 8)) || (rc = st.Qf.ensure(nf * Kp * 4)) ||
+        (rc = st.Ld.ensure(nf * K * 8)) || (rc = st.Lf.ensure(nf * Kp * 4)) ||
+        (rc = st.rmax.ensure(nf * 8)))
+        return rc;
+    k_fmm_p2m<P><<<(int)nf, 64, 0, s>>>((int)nf, v, t->sx.as<double>(), t->sy.as<double>(),
+                                        t->sz.as<double>(), t->sm.as<double>(), tb.mi.as<int>(),
+                                        st.Qd.as<double>(), st.rmax.as<double>());
+    for (int d = MAX_DEPTH; d >= 0; --d) {
+        const int f0 = st.fls[d], f1 = st.fls[d + 1];
+        if (f1 > f0)
+            k_fmm_m2m<P><<<f1 - f0, 64, 0, s>>>(f0, f1, v, tb.m2m.as<double>(), st.Qd.as<double>(),
+                                                st.rmax.as<double>());
+    }
+    k_fmm_qf<P><<<grid_for(nf * Kp, 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
+    HQ_CHECK_HIP(hipMemsetAsync(st.Ld.p, 0, nf * K * 8, s));
+    // 3. dual walk: frontier ping-pong in fa / fb; lists appended
+    WalkPar w{theta, tb.R, tb.nd};
+    int64_t nfr = t->nsys, nlat = 0, ngen = 0, np2p = 0, nm2p = 0;
+    size_t lat_cap = 0, gen_cap = 0, p2p_cap = 0, m2p_cap = 0;
+    if ((rc = st.fa.ensure(std::max<int64_t>(nfr, 1024) * sizeof(int2)))) return rc;
+    k_fmm_roots<<<grid_for(nfr, 256), 256, 0, s>>>((int)nfr, st.fa.as<int2>());
+    DevBuf* cur = &st.fa;
+    DevBuf* nxt = &st.fb;
+    auto grow = [&](DevBuf& b, size_t& cap, int64_t used, int64_t need, size_t esz) -> int {
+        if ((size_t)need <= cap) return HQ_OK;
+        size_t nc = std::max<size_t>(need, 2 * cap);
+        DevBuf tmp;
+        int r;
+        if ((r = tmp.ensure(nc * esz))) return r;
+        if (used) HQ_CHECK_HIP(hipMemcpyAsync(tmp.p, b.p, used * esz, hipMemcpyDeviceToDevice, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+        std::swap(tmp.p, b.p);
+        std::swap(tmp.bytes, b.bytes);
+        cap = nc;
+        return HQ_OK;
+    };
+    while (nfr > 0) {
+        if (nfr > ((int64_t)1 << 31))
+            return fail(HQ_ERR_OVERFLOW, "FMM pair frontier of %lld pairs", (long long)nfr);
+        const int nblk = grid_for(nfr, WB);
+        if ((rc = st.cnt.ensure((size_t)NACT * (nblk + 1) * 4))) return rc;
+        uint32_t* cnt = st.cnt.as<uint32_t>();
+        k_fmm_walk_count<<<nblk, WB, 0, s>>>(v, st.rmax.as<double>(), w, cur->as<int2>(), nfr, cnt, nblk);
+        for (int q = 0; q < NACT; ++q)
+            if ((rc = scan_excl_buf(cnt + (int64_t)q * (nblk + 1), nblk, t->scanpart, s))) return rc;
+        uint32_t tot[NACT];
+        for (int q = 0; q < NACT; ++q)
+            HQ_CHECK_HIP(hipMemcpyAsync(&tot[q], cnt + (int64_t)q * (nblk + 1) + nblk, 4,
+                                        hipMemcpyDeviceToHost, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+        if ((rc = nxt->ensure(std::max<int64_t>(tot[0], 1) * sizeof(int2))) ||
+            (rc = grow(st.lat, lat_cap, nlat, nlat + tot[1] + 1, sizeof(int3))) ||
+            (rc = grow(st.gen, gen_cap, ngen, ngen + tot[2] + 1, sizeof(int2))) ||
+            (rc = grow(st.p2p, p2p_cap, np2p, np2p + tot[3] + 1, sizeof(int2))) ||
+            (rc = grow(st.m2p, m2p_cap, nm2p, nm2p + tot[4] + 1, sizeof(int2))))
+            return rc;
+        k_fmm_walk_emit<<<nblk, WB, 0, s>>>(v, st.rmax.as<double>(), w, cur->as<int2>(), nfr, cnt,
+                                            nblk, nxt->as<int2>(), st.lat.as<int3>(), nlat,
+                                            st.gen.as<int2>(), ngen, st.p2p.as<int2>(), np2p,
+                                            st.m2p.as<int2>(), nm2p);
+        nlat += tot[1];
+        ngen += tot[2];
+        np2p += tot[3];
+        nm2p += tot[4];
+        std::swap(cur, nxt);
+        nfr = tot[0];
+    }
+    st.nf = nf;
+    t->list_cells = nlat + ngen;
+    t->list_parts = np2p;
+    // 4. M2L
+    HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
+    if (mfma) {
+        const int ngroups = grid_for(nf, GT);
+        const int64_t nbin = (int64_t)ngroups * tb.noff;
+        if ((rc = st.bins.ensure((nbin + 1) * 4)) || (rc = st.binpos.ensure((nbin + 1) * 4)) ||
+            (rc = st.lsrc.ensure(std::max<int64_t>(nlat, 1) * 4)) ||
+            (rc = st.lcol.ensure(std::max<int64_t>(nlat, 1) * 2)))
+            return rc;
+        HQ_CHECK_HIP(hipMemsetAsync(st.bins.p, 0, (nbin + 1) * 4, s));
+        if (nlat)
+            k_fmm_lat_hist<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, tb.noff,
+                                                              st.bins.as<uint32_t>());
+        if ((rc = scan_excl_buf(st.bins.as<uint32_t>(), nbin, t->scanpart, s))) return rc;
+        HQ_CHECK_HIP(hipMemcpyAsync(st.binpos.p, st.bins.p, (nbin + 1) * 4, hipMemcpyDeviceToDevice, s));
+        if (nlat)
+            k_fmm_lat_scatter<<<capped(nlat), 256, 0, s>>>(
+                st.lat.as<int3>(), nlat, tb.noff, st.binpos.as<uint32_t>(), st.lsrc.as<int>(),
+                st.lcol.as<int16_t>());
+        // enough workgroups to fill the device: split the offsets of a group
+        const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
+        if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
+        k_fmm_m2l_mfma<P><<<ngroups * nsplit, 512, 0, s>>>(
+            nf, tb.noff, nsplit, st.bins.as<uint32_t>(), st.lsrc.as<int>(), st.lcol.as<int16_t>(),
+            tb.T.as<float>(), st.Qf.as<float>(), st.Lf.as<float>());
+        if (ngen)
+            k_fmm_m2l_fp64<P><<<(int)std::min<int64_t>(ngen, 1 << 18), 64, 0, s>>>(
+                st.gen.as<int2>(), ngen, nullptr, 0, v, tb.mi.as<int>(), tb.mi2.as<int>(),
+                tb.rec2.as<int>(), tb.m2lidx.as<int>(), tb.m2lcoef.as<double>(), st.Qd.as<double>(),
+                st.Ld.as<double>());
+        k_fmm_merge<P><<<grid_for(nf * K, 256), 256, 0, s>>>(nf, st.Lf.as<float>(), st.Ld.as<double>());
+    } else if (ngen + nlat) {
+        k_fmm_m2l_fp64<P><<<(int)std::min<int64_t>(ngen + nlat, 1 << 18), 64, 0, s>>>(
+            st.gen.as<int2>(), ngen, st.lat.as<int3>(), nlat, v, tb.mi.as<int>(), tb.mi2.as<int>(),
+            tb.rec2.as<int>(), tb.m2lidx.as<int>(), tb.m2lcoef.as<double>(), st.Qd.as<double>(),
+            st.Ld.as<double>());
+    }
+    HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
+    // 5. L2L, top-down
+    for (int d = 1; d <= MAX_DEPTH; ++d) {
+        const int f0 = st.fls[d], f1 = st.fls[d + 1];
+        if (f1 > f0)
+            k_fmm_l2l<P><<<f1 - f0, 64, 0, s>>>(f0, f1, v, t->nparent.as<int>(), tb.l2l.as<double>(),
+                                                st.Ld.as<double>());
+    }
+    // 6. near lists by target, then leaves
+    if ((rc = st.phist.ensure((nf + 1) * 4)) || (rc = st.ppos.ensure((nf + 1) * 4)) ||
+        (rc = st.plist.ensure(std::max<int64_t>(np2p, 1) * 4)))
+        return rc;
+    HQ_CHECK_HIP(hipMemsetAsync(st.phist.p, 0, (nf + 1) * 4, s));
+    if (np2p)
+        k_fmm_p2p_hist<<<capped(np2p), 256, 0, s>>>(st.p2p.as<int2>(), np2p,
+                                                          st.phist.as<uint32_t>());
+    if ((rc = scan_excl_buf(st.phist.as<uint32_t>(), nf, t->scanpart, s))) return rc;
+    HQ_CHECK_HIP(hipMemcpyAsync(st.ppos.p, st.phist.p, (nf + 1) * 4, hipMemcpyDeviceToDevice, s));
+    if (np2p) {
+        k_fmm_p2p_scatter<<<capped(np2p), 256, 0, s>>>(st.p2p.as<int2>(), np2p,
+                                                             st.ppos.as<uint32_t>(), st.plist.as<int>());
+    }
+    // M2P lists by target, the same way
+    if ((rc = st.mhist.ensure((nf + 1) * 4)) || (rc = st.mpos.ensure((nf + 1) * 4)) ||
+        (rc = st.mlist.ensure(std::max<int64_t>(nm2p, 1) * 4)))
+        return rc;
+    HQ_CHECK_HIP(hipMemsetAsync(st.mhist.p, 0, (nf + 1) * 4, s));
+    if (nm2p)
+        k_fmm_p2p_hist<<<capped(nm2p), 256, 0, s>>>(st.m2p.as<int2>(), nm2p,
+                                                          st.mhist.as<uint32_t>());
+    if ((rc = scan_excl_buf(st.mhist.as<uint32_t>(), nf, t->scanpart, s))) return rc;
+    HQ_CHECK_HIP(hipMemcpyAsync(st.mpos.p, st.mhist.p, (nf + 1) * 4, hipMemcpyDeviceToDevice, s));
+    if (nm2p) {
+        k_fmm_p2p_scatter<<<capped(nm2p), 256, 0, s>>>(st.m2p.as<int2>(), nm2p,
+                                                             st.mpos.as<uint32_t>(), st.mlist.as<int>());
+    }
+    const size_t b8 = n * sizeof(double);
+    if ((rc = t->fx.ensure(b8)) || (rc = t->fy.ensure(b8)) || (rc = t->fz.ensure(b8))) return rc;
+    auto leaf = [&](auto kern) {
+        kern<<<(int)nf, 64, 0, s>>>((int)nf, v, t->sx.as<double>(), t->sy.as<double>(),
+                                    t->sz.as<double>(), t->sm.as<double>(), tb.mi.as<int>(),
+                                    tb.mi2.as<int>(), tb.rec2.as<int>(), st.Ld.as<double>(),
+                                    st.Qd.as<double>(),
+                                    st.phist.as<uint32_t>(), st.plist.as<int>(),
+                                    st.mhist.as<uint32_t>(), st.mlist.as<int>(), G, eps2,
+                                    t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>());
+    };
+    if (mfma) leaf(k_fmm_leaf<P, true>);
+    else leaf(k_fmm_leaf<P, false>);
+    HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    HQ_CHECK_HIP(hipGetLastError());
+    hipEventElapsedTime(&t->ms_lists, t->ev[5], t->ev[7]);  // M2L
+    hipEventElapsedTime(&t->ms_eval, t->ev[7], t->ev[8]);   // L2L, near field, L2P
+    t->n_m2p = nlat + ngen;
+    t->n_p2p = np2p;
+    t->exec_cells = nlat;  // same-level (lattice) M2L pairs
+    t->exec_parts = ngen;  // M2L pairs of two levels
+    t->max_cells = nm2p;   // M2P pairs
+    return HQ_OK;
+}
+
+int tree_forces_fmm(hq_tree* t, int order, double theta, double G, double eps2, bool mfma, int S) {
+    if (!t->fmm) t->fmm.reset(new FmmState());
+    if (order == 4) return fmm_run<4>(t, *t->fmm, theta, G, eps2, mfma, S);
+    return fmm_run<6>(t, *t->fmm, theta, G, eps2, mfma, S);
+}
+
+}  // namespace hq

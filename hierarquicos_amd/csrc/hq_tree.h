@@ -7,9 +7,18 @@
 // (sorted key) order, so the children of a node are one contiguous range
 // [child_start[k], child_start[k+1]).
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "hq_internal.h"
+
+namespace hq {
+struct FmmState;  // hq_fmm.hip
+void fmm_state_free(FmmState* s);
+struct FmmFree {
+    void operator()(FmmState* s) const { fmm_state_free(s); }
+};
+}  // namespace hq
 
 struct hq_tree {
     int64_t n = 0;
@@ -42,6 +51,8 @@ struct hq_tree {
     hq::DevBuf groups, gflag, bstart;
     int super_size = 256;
     int leaf = 0;          // bunched walks: cells of <= leaf bodies enter as their bodies (0: off)
+    int fmm_leaf = 32;     // FMM: cells of <= fmm_leaf bodies are leaves (1..64)
+    std::unique_ptr<hq::FmmState, hq::FmmFree> fmm;
     bool fused = true;     // mixed bunched walk as one fused kernel  // bunching: sibling subtrees packed up to this many bodies
     int64_t ngroups = 0;
     // walk scratch
@@ -93,6 +104,7 @@ int tree_build(hq_tree* t);
 int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2);
 int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int precision, bool bunched);
 int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2);
+int tree_forces_fmm(hq_tree* t, int order, double theta, double G, double eps2, bool mfma, int S);
 int tree_scatter_forces(hq_tree* t, double* F, int layout, int where);
 int read_status(hq_tree* t);
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,

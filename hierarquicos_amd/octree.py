@@ -133,10 +133,23 @@ class OctreeType:
             return lib.hq_tree_forces_bunched(h, th2, g, e2, p, layout, where, prec)
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
-    def tune(self, bunch=None, super_size=None, fused=None, leaf=None):
+    def fmm(self, theta2, G=1.0, eps2=0.0, order=4, mode="mfma", out=None):
+        """Cartesian FMM of order 4 or 6 on the built tree; (3, N).  mode
+        "mfma": same-level M2L as FP32 matrix-core products and an FP32 near
+        field; "fp64": everything in scalar FP64."""
+        md = {"mfma": L.HQ_FMM_MFMA, "fp64": L.HQ_FMM_FP64}[mode]
+        lib = L.lib()
+
+        def fn(h, th2, g, e2, p, layout, where):
+            return lib.hq_tree_fmm(h, int(order), th2, g, e2, md, p, layout, where)
+        return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
+
+    def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None):
         """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
         bucket (cells of <= leaf bodies enter as their bodies) of the bunched walk."""
         lib = L.lib()
+        if fmm_leaf is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FMM_LEAF, int(fmm_leaf)))
         if leaf is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_LEAF, int(leaf)))
         if fused is not None:

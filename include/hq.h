@@ -111,7 +111,22 @@ int hq_tree_destroy(hq_tree *t);
 #define HQ_TUNE_LEAF 4    /* 0 (default) .. 64: in bunched walks a cell of at most this
                              many bodies enters the lists as its bodies (exact P2P),
                              the leaf-bucket size of a bucketed tree */
+#define HQ_TUNE_FMM_LEAF 5 /* 1..64 (default 32): FMM cells of at most this many bodies are leaves */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
+
+/* Cartesian FMM of order 4 or 6 on the built tree (SURVEY "Dehnen p=4/6";
+ * the reference's Dehnen method, octree_default.f90:1095-1712, stops at
+ * quadrupole sources).  Expansions about the cube centres; a dual-tree walk
+ * with rmax_A + rmax_B <= theta |c_A - c_B|; M2L of same-level pairs as dense
+ * products with per-offset operators.  mode HQ_FMM_MFMA: that M2L in FP32 on
+ * the matrix cores and the near field in FP32 (Kahan); HQ_FMM_FP64: every M2L
+ * and the near field in scalar FP64.  Moments, L2L and L2P are FP64.  The far
+ * field is unsoftened; eps2 softens the near field.  Single or batch handles;
+ * HQ_ERR_LOST if bodies were lost at max depth. */
+#define HQ_FMM_MFMA 0
+#define HQ_FMM_FP64 1
+int hq_tree_fmm(hq_tree *t, int order, double theta2, double G, double eps2, int mode, double *F,
+                int layout, int where);
 /* Autotuner (no reference counterpart): on the built tree, times the bunched
  * walk at `precision` for bunch sizes {32, 48, 64}, super-group sizes
  * {128, 256, 512} and leaf buckets {0, 2, 4, 8} (best of `reps` runs each,
