@@ -195,6 +195,77 @@ def run_parareal(args):
         "cpu_baseline": base}))
 
 
+def run_fmm(args):
+    """Cartesian FMM at N=4,194,304: uniform cube [-1,1]^3 and Plummer, orders 4
+    and 6, theta 0.5; same-level M2L on the matrix cores (FP32) beside the
+    scalar FP64 M2L; error against the FP64 direct sum on a sample."""
+    from hierarquicos_amd import DeviceArray, OctreeType, direct_forces, ics
+    from hierarquicos_amd.device import sync
+    N = args.n_fmm
+    th2 = args.theta ** 2
+    rng = np.random.default_rng(1234)
+    sample = np.sort(rng.choice(N, args.sample, replace=False))
+    cases = []
+    for prof in args.fmm_profiles.split(","):
+        if prof == "cube":
+            m, q, _ = ics.uniform_cube(N, half=1.0, seed=args.seed)
+        else:
+            m, q, _ = ics.plummer(N, a=1.0, rmax=10.0, seed=args.seed)
+        dm = DeviceArray.from_numpy(m)
+        dx, dy, dz = (DeviceArray.from_numpy(np.ascontiguousarray(q[:, k])) for k in range(3))
+        Fd = direct_forces(dm, dx, dy, dz, 1.0, 0.0, targets=sample)
+        dF = DeviceArray((3 * N,))
+        t = OctreeType().pre_init(dm, 10)
+        for order in (int(o) for o in args.fmm_orders.split(",")):
+            for mode in ("mfma", "fp64"):
+                if mode == "fp64" and (order == 6 or args.no_extras):
+                    continue
+                t.init(dx, dy, dz)
+                t.fmm(th2, 1.0, 0.0, order, mode, out=dF)     # warm: tables, buffers
+                sync()
+                walls = []
+                for _ in range(args.steps):
+                    a = time.perf_counter()
+                    t.init(dx, dy, dz)
+                    t.fmm(th2, 1.0, 0.0, order, mode, out=dF)
+                    walls.append(time.perf_counter() - a)
+                inf = t.info()
+                F = dF.numpy().reshape(N, 3)[sample]
+                r = np.linalg.norm(F - Fd, axis=1) / np.linalg.norm(Fd, axis=1)
+                K = (order + 1) * (order + 2) * (order + 3) // 6
+                flops = 2.0 * K * K * inf["exec_cells"]
+                ms = float(np.median(walls)) * 1e3
+                cases.append({"profile": prof, "order": order, "m2l": mode,
+                              "ms_per_eval": ms, "particles_per_s": N / (ms * 1e-3),
+                              "rms_rel_err": float(np.sqrt(np.mean(r * r))),
+                              "max_rel_err": float(r.max()),
+                              "m2l_kernel_ms": inf["ms_lists"], "build_ms": inf["ms_build"],
+                              "fmm_ms": inf["ms_walk"], "near_l2p_ms": inf["ms_eval"],
+                              "same_level_m2l_pairs": inf["exec_cells"],
+                              "other_m2l_pairs": inf["exec_parts"], "near_pairs": inf["n_p2p"],
+                              "m2p_pairs": inf["max_cells"],
+                              "m2l_tflops": flops / (inf["ms_lists"] * 1e-3) / 1e12})
+                print(f"fmm {prof} p={order} {mode}: {ms:.1f} ms", file=sys.stderr, flush=True)
+        t.close()
+    head = cases[0]
+    peak = 157.3 if head["m2l"] == "mfma" else 78.6
+    print(json.dumps({
+        "metric": "FMM particles/s at N=4M uniform cube, p=4, theta=0.5 (matrix-core M2L)",
+        "value": head["particles_per_s"], "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": head["ms_per_eval"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded uniform cube and Plummer sphere)",
+        "config": {"workload": f"fmm_cube_n{N}_p4_theta{args.theta}", "N": N,
+                   "theta": args.theta, "leaf": 32},
+        "rms_rel_err": head["rms_rel_err"], "max_rel_err": head["max_rel_err"],
+        "roofline": {"bound": "mfma", "kernel": "k_fmm_m2l_mfma<4>",
+                     "achieved": head["m2l_tflops"], "peak": peak, "unit": "TFLOP/s",
+                     "frac": head["m2l_tflops"] / peak, "traffic": None,
+                     "flops_per_launch": 2.0 * 35 * 35 * head["same_level_m2l_pairs"],
+                     "flops_note": "2 K^2 per same-level pair, K = 35 coefficients (no padding)"},
+        "cases": cases, "cpu_baseline": None}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,11 +288,15 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64 paths and counts")
-    ap.add_argument("--workload", choices=("forces", "ensemble", "parareal"), default="forces",
+    ap.add_argument("--workload", choices=("forces", "ensemble", "parareal", "fmm"),
+                    default="forces",
                     help="forces: the headline force evaluation (default)")
     ap.add_argument("--systems", type=int, default=256, help="ensemble: number of systems")
     ap.add_argument("--dt", type=float, default=1e-3, help="ensemble: Verlet step")
     ap.add_argument("--n-parareal", type=int, default=10000)
+    ap.add_argument("--n-fmm", type=int, default=1 << 22)
+    ap.add_argument("--fmm-profiles", default="cube,plummer")
+    ap.add_argument("--fmm-orders", default="4,6")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the force path here runs on a single MI355X; "
@@ -231,6 +306,10 @@ def main():
         if args.eps == 1e-3:
             args.eps = 1e-2
         return run_ensemble(args)
+    if args.workload == "fmm":
+        if args.steps == 20:
+            args.steps = 3
+        return run_fmm(args)
     if args.workload == "parareal":
         if args.eps == 1e-3:
             args.eps = 1e-2

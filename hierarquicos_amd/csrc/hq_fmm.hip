@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "hq_tree.h"
@@ -80,7 +81,9 @@ void taylor(const double R[3], const std::vector<MI>& ks, const std::vector<int>
 }  // namespace
 
 struct FmmTables {
-    int p = 0, K = 0, Kp = 0, K2 = 0, R = 0, nd = 0, noff = 0;
+    // offsets: class 0 (same level) ids [0, noff0); class 1 (target one level
+    // coarser) [noff0, noff0 + noff1); class 2 (target one level finer) after
+    int p = 0, K = 0, Kp = 0, K2 = 0, R = 0, nd = 0, noff0 = 0, nd1 = 0, noff1 = 0, noff = 0;
     double theta = -1.0;
     DevBuf mi, mi2, rec2, m2lidx, m2lcoef, m2m, l2l, T;
 };
@@ -142,23 +145,43 @@ static int build_tables(FmmTables& tb, int p, double theta) {
                 }
             }
     }
-    // lattice operators T_o for |o_d| <= R, o != 0
+    // lattice operators.  Class 0: integer offsets |o_d| <= R in units of the
+    // common side, T[n][m] = coef a_{n+m}(-o).  Classes 1 and 2: half-integer
+    // offsets u (|u_d| <= R + 1/2) in units of the finer side, stored at index
+    // (2u + 2R + 1) / 2 per axis; class 1 (target coarser) scales rows by
+    // 2^(|n|+1), class 2 (target finer) columns by 2^|m|.
     const int R = std::max(2, (int)floor(2.0 * sqrt(3.0) / theta + sqrt(3.0)));
-    const int nd = 2 * R + 1, noff = nd * nd * nd;
+    const int nd = 2 * R + 1, noff0 = nd * nd * nd;
+    const int nd1 = 2 * R + 2, noff1 = nd1 * nd1 * nd1;
+    const int noff = noff0 + 2 * noff1;
     std::vector<MI> ks2(k2.begin(), k2.begin() + K2);
     std::vector<float> T((size_t)noff * Kp * Kp, 0.0f);
     std::vector<double> a;
-    for (int o = 0; o < noff; ++o) {
-        const int ox = o / (nd * nd) - R, oy = (o / nd) % nd - R, oz = o % nd - R;
-        if (!ox && !oy && !oz) continue;
-        // o is the source's offset from the target; the expansion needs
-        // r = c_A - c_B = -o s
-        const double Ro[3] = {(double)-ox, (double)-oy, (double)-oz};
+    auto fill = [&](int o, const double u[3], int cls) {
+        // u is the source's offset from the target; the expansion needs r = -u
+        const double Ro[3] = {-u[0], -u[1], -u[2]};
         taylor(Ro, ks2, rec, a);
         float* t = T.data() + (size_t)o * Kp * Kp;
         for (int n = 0; n < K; ++n)
-            for (int m = 0; m < K; ++m) t[n * Kp + m] = (float)(mcoef[n * K + m] * a[midx[n * K + m]]);
+            for (int m = 0; m < K; ++m) {
+                double c = mcoef[n * K + m] * a[midx[n * K + m]];
+                if (cls == 1) c = ldexp(c, k1[n].x + k1[n].y + k1[n].z + 1);
+                if (cls == 2) c = ldexp(c, k1[m].x + k1[m].y + k1[m].z);
+                t[n * Kp + m] = (float)c;
+            }
+    };
+    for (int o = 0; o < noff0; ++o) {
+        const int ox = o / (nd * nd) - R, oy = (o / nd) % nd - R, oz = o % nd - R;
+        if (!ox && !oy && !oz) continue;
+        const double u[3] = {(double)ox, (double)oy, (double)oz};
+        fill(o, u, 0);
     }
+    for (int cls = 1; cls <= 2; ++cls)
+        for (int o = 0; o < noff1; ++o) {
+            const int ix = o / (nd1 * nd1), iy = (o / nd1) % nd1, iz = o % nd1;
+            const double u[3] = {ix - R - 0.5, iy - R - 0.5, iz - R - 0.5};
+            fill(noff0 + (cls - 1) * noff1 + o, u, cls);
+        }
     std::vector<int> mi(4 * KMAX), mi2(4 * K2MAX);
     for (int i = 0; i < KMAX; ++i) {
         mi[4 * i] = k1[i].x, mi[4 * i + 1] = k1[i].y, mi[4 * i + 2] = k1[i].z;
@@ -182,7 +205,8 @@ static int build_tables(FmmTables& tb, int p, double theta) {
         (rc = up(tb.m2m, m2m.data(), m2m.size() * 8)) || (rc = up(tb.l2l, l2l.data(), l2l.size() * 8)) ||
         (rc = up(tb.T, T.data(), T.size() * 4)))
         return rc;
-    tb.p = p, tb.K = K, tb.Kp = Kp, tb.K2 = K2, tb.R = R, tb.nd = nd, tb.noff = noff, tb.theta = theta;
+    tb.p = p, tb.K = K, tb.Kp = Kp, tb.K2 = K2, tb.R = R, tb.nd = nd, tb.noff0 = noff0;
+    tb.nd1 = nd1, tb.noff1 = noff1, tb.noff = noff, tb.theta = theta;
     return HQ_OK;
 }
 
@@ -323,7 +347,7 @@ constexpr int NACT = 5;
 
 struct WalkPar {
     double theta;
-    int R, nd;
+    int R, nd, nd1, noff0, noff1;
 };
 
 // Classifies the directed pair (a <- b): accept with the MAC, sum directly
@@ -358,6 +382,18 @@ __device__ __forceinline__ int fmm_classify(const FView& v, const double* __rest
                 const int ox = (int)rint(dx / s), oy = (int)rint(dy / s), oz = (int)rint(dz / s);
                 if (abs(ox) <= w.R && abs(oy) <= w.R && abs(oz) <= w.R) {
                     *off = ((ox + w.R) * w.nd + (oy + w.R)) * w.nd + (oz + w.R);
+                    return ACT_LATTICE;
+                }
+            } else if (da == db - 1 || da == db + 1) {
+                // half-integer offset in units of the finer cube
+                const double s = side_of(v, da > db ? a : b);
+                const int wx = (int)rint(2.0 * dx / s), wy = (int)rint(2.0 * dy / s),
+                          wz = (int)rint(2.0 * dz / s);
+                const int lim = 2 * w.R + 1;
+                if ((wx & 1) && (wy & 1) && (wz & 1) && abs(wx) <= lim && abs(wy) <= lim &&
+                    abs(wz) <= lim) {
+                    const int ix = (wx + lim) / 2, iy = (wy + lim) / 2, iz = (wz + lim) / 2;
+                    *off = w.noff0 + (da < db ? 0 : w.noff1) + (ix * w.nd1 + iy) * w.nd1 + iz;
                     return ACT_LATTICE;
                 }
             }
@@ -517,9 +553,10 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
                                                       const float* __restrict__ Qf,
                                                       float* __restrict__ Lf) {
     constexpr int Kp = kpad(P), RT = Kp / 16, KT = Kp / 4, CT = GT / 16 / 8;
-    __shared__ float tl[Kp * Kp];
-    __shared__ int colsrc[GT];
-    __shared__ uint32_t bs[GT + 1];  // bin starts of the current chunk of offsets
+    __shared__ float4 tl4[Kp * Kp / 4];
+    __shared__ int colsrc[GT], coltag[GT];  // source of each target column, tagged by offset
+    __shared__ uint32_t bs[GT + 1];         // bin starts of the current chunk of offsets
+    const float* tl = reinterpret_cast<const float*>(tl4);
     // block (g, sp): target group g, offsets [o_lo, o_hi); with several
     // splits per group the partial sums meet in Lf by atomic adds
     const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
@@ -527,6 +564,7 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
     const int o_lo = sp * per, o_hi = min(noff, o_lo + per);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
+    coltag[tid] = -1;
     floatx4 acc[CT][RT];
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct)
@@ -542,17 +580,18 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
         }
         const uint32_t b0 = bs[(o - o_lo) % GT], b1 = bs[(o - o_lo) % GT + 1];
         if (b0 == b1) continue;
-        __syncthreads();
-        colsrc[tid] = -1;
-        const float* t = T + (int64_t)o * Kp * Kp;
-        for (int i = tid; i < Kp * Kp; i += 512) tl[i] = t[i];
-        __syncthreads();
-        for (uint32_t e = b0 + tid; e < b1; e += 512) colsrc[col[e]] = src[e];
+        __syncthreads();  // the previous offset's tile is no longer read
+        const float4* t4 = reinterpret_cast<const float4*>(T + (int64_t)o * Kp * Kp);
+        for (int i = tid; i < Kp * Kp / 4; i += 512) tl4[i] = t4[i];
+        for (uint32_t e = b0 + tid; e < b1; e += 512) {
+            colsrc[col[e]] = src[e];
+            coltag[col[e]] = o;
+        }
         __syncthreads();
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
             const int cbase = wv * (GT / 8) + ct * 16;
-            const int s = colsrc[cbase + li];
+            const int s = coltag[cbase + li] == o ? colsrc[cbase + li] : -1;
             if (__ballot(s >= 0) == 0) continue;
             float bv[KT];
 #pragma unroll
@@ -676,19 +715,16 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
                                                  const double* __restrict__ sz,
                                                  const double* __restrict__ sm,
                                                  const int* __restrict__ mi,
-                                                 const int* __restrict__ mi2,
-                                                 const int* __restrict__ rec2,
                                                  const double* __restrict__ Ld,
-                                                 const double* __restrict__ Qd,
                                                  const uint32_t* __restrict__ pstart,
-                                                 const int* __restrict__ plist,
-                                                 const uint32_t* __restrict__ mstart,
-                                                 const int* __restrict__ mlist, double G,
+                                                 const int* __restrict__ plist, double G,
                                                  double eps2, double* __restrict__ fx,
                                                  double* __restrict__ fy, double* __restrict__ fz) {
-    constexpr int K = ncoef(P), K1 = ncoef(P + 1);
-    __shared__ double4 src[64];
-    __shared__ double at[K1][64];  // per-lane Taylor coefficients of the M2P
+    constexpr int K = ncoef(P);
+    constexpr int NB = 256;  // staged source bodies (LDS)
+    using V4 = typename std::conditional<F32, float4, double4>::type;
+    __shared__ V4 sbuf[NB];
+    __shared__ int sidx[NB];
     const int f = blockIdx.x;
     if (f >= nf) return;
     const int k = v.fnode[f];
@@ -719,30 +755,43 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
     }
     const double sc = -si * si;
     double ax = sc * g[0], ay = sc * g[1], az = sc * g[2];
-    // near field
+    // near field: the bodies of consecutive source leaves are staged in LDS,
+    // up to NB at a time, then every lane sums them (FP32 recentred on the
+    // leaf centre with a Kahan-compensated total, or FP64)
     float ks[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
     const float tx = (float)(xi - c[0]), ty = (float)(yi - c[1]), tz = (float)(zi - c[2]);
-    for (uint32_t e = pstart[f]; e < pstart[f + 1]; ++e) {
-        const int kb = v.fnode[plist[e]];
-        const int bf = v.first[kb], bc = v.count[kb];
-        __syncthreads();
-        if (lane < bc) {
-            const int j = bf + lane;
-            src[lane] = F32 ? make_double4((double)(float)(sx[j] - c[0]), (double)(float)(sy[j] - c[1]),
-                                           (double)(float)(sz[j] - c[2]), sm[j])
-                            : make_double4(sx[j], sy[j], sz[j], sm[j]);
+    const uint32_t e0 = pstart[f], e1 = pstart[f + 1];
+    uint32_t e = e0;
+    while (e < e1) {
+        int fill = 0;
+        __syncthreads();  // the previous batch is no longer read
+        while (e < e1) {
+            const int kb = v.fnode[plist[e]];
+            const int bc = v.count[kb];
+            if (fill + bc > NB) break;
+            const int bf = v.first[kb];
+            if (lane < bc) {
+                const int j = bf + lane;
+                if constexpr (F32)
+                    sbuf[fill + lane] = make_float4((float)(sx[j] - c[0]), (float)(sy[j] - c[1]),
+                                                    (float)(sz[j] - c[2]), (float)sm[j]);
+                else
+                    sbuf[fill + lane] = make_double4(sx[j], sy[j], sz[j], sm[j]);
+                sidx[fill + lane] = j;
+            }
+            fill += bc;
+            ++e;
         }
         __syncthreads();
-        if (F32) {
+        if constexpr (F32) {
             float p[3] = {0.f, 0.f, 0.f};
             const float e2 = (float)eps2;
-            for (int j = 0; j < bc; ++j) {
-                if (bf + j == i) continue;
-                const double4 q = src[j];
-                const float dx = (float)q.x - tx, dy = (float)q.y - ty, dz = (float)q.z - tz;
+            for (int j = 0; j < fill; ++j) {
+                const float4 q = sbuf[j];
+                const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
                 const float ri = __frsqrt_rn(d2);
-                const float w = (float)q.w * ri * ri * ri;
+                const float w = sidx[j] == i ? 0.0f : q.w * ri * ri * ri;
                 p[0] = fmaf(w, dx, p[0]);
                 p[1] = fmaf(w, dy, p[1]);
                 p[2] = fmaf(w, dz, p[2]);
@@ -754,9 +803,9 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
                 ks[d] = tv;
             }
         } else {
-            for (int j = 0; j < bc; ++j) {
-                if (bf + j == i) continue;
-                const double4 q = src[j];
+            for (int j = 0; j < fill; ++j) {
+                if (sidx[j] == i) continue;
+                const V4 q = sbuf[j];
                 const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
                 const double d2 = dx * dx + dy * dy + dz * dz + eps2;
                 const double w = q.w / (sqrt(d2) * d2);
@@ -771,6 +820,39 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
         ay += (double)ks[1] - (double)kc[1];
         az += (double)ks[2] - (double)kc[2];
     }
+    if (ok) {
+        const double gm = G * sm[i];
+        fx[i] = gm * ax;
+        fy[i] = gm * ay;
+        fz[i] = gm * az;
+    }
+}
+
+// M2P of the lopsided pairs (a leaf target much larger than its source cell):
+// one wavefront per target leaf, one lane per body, added to its force.
+template <int P>
+__global__ __launch_bounds__(64) void k_fmm_m2p(int nf, FView v, const double* __restrict__ sx,
+                                                const double* __restrict__ sy,
+                                                const double* __restrict__ sz,
+                                                const double* __restrict__ sm,
+                                                const int* __restrict__ mi2,
+                                                const int* __restrict__ rec2,
+                                                const double* __restrict__ Qd,
+                                                const uint32_t* __restrict__ mstart,
+                                                const int* __restrict__ mlist, double G,
+                                                double* __restrict__ fx, double* __restrict__ fy,
+                                                double* __restrict__ fz) {
+    constexpr int K = ncoef(P), K1 = ncoef(P + 1);
+    __shared__ double at[K1][64];  // per-lane Taylor coefficients
+    const int f = blockIdx.x;
+    if (f >= nf || mstart[f] == mstart[f + 1]) return;
+    const int k = v.fnode[f];
+    const int lane = threadIdx.x;
+    const int first = v.first[k], cnt = v.count[k];
+    const bool ok = lane < cnt;
+    const int i = first + (ok ? lane : 0);
+    const double xi = sx[i], yi = sy[i], zi = sz[i];
+    double ax = 0.0, ay = 0.0, az = 0.0;
     // M2P: a_d = s^-2 sum_m (-1)^|m| Q~_m (m_d + 1) a_{m+e_d}((x - c_B) / s),
     // accumulated as each a_k (|k| <= p+1) is produced
     for (uint32_t e = mstart[f]; e < mstart[f + 1]; ++e) {
@@ -807,9 +889,9 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
     }
     if (ok) {
         const double gm = G * sm[i];
-        fx[i] = gm * ax;
-        fy[i] = gm * ay;
-        fz[i] = gm * az;
+        fx[i] += gm * ax;
+        fy[i] += gm * ay;
+        fz[i] += gm * az;
     }
 }
 
@@ -831,6 +913,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     FmmTables& tb = st.tab;
     if ((rc = build_tables(tb, P, theta))) return rc;
     const int64_t nn = t->nn, n = t->n;
+    HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
     // 1. FMM node set
     if ((rc = st.flag.ensure(nn)) || (rc = st.scan.ensure((nn + 1) * 4)) ||
         (rc = st.fid.ensure(nn * 4)) || (rc = st.lvl.ensure(64 * 4)))
@@ -889,7 +972,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     k_fmm_qf<P><<<grid_for(nf * Kp, 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
     HQ_CHECK_HIP(hipMemsetAsync(st.Ld.p, 0, nf * K * 8, s));
     // 3. dual walk: frontier ping-pong in fa / fb; lists appended
-    WalkPar w{theta, tb.R, tb.nd};
+    WalkPar w{theta, tb.R, tb.nd, tb.nd1, tb.noff0, tb.noff1};
     int64_t nfr = t->nsys, nlat = 0, ngen = 0, np2p = 0, nm2p = 0;
     size_t lat_cap = 0, gen_cap = 0, p2p_cap = 0, m2p_cap = 0;
     if ((rc = st.fa.ensure(std::max<int64_t>(nfr, 1024) * sizeof(int2)))) return rc;
@@ -944,7 +1027,6 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     t->list_cells = nlat + ngen;
     t->list_parts = np2p;
     // 4. M2L
-    HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
     if (mfma) {
         const int ngroups = grid_for(nf, GT);
         const int64_t nbin = (int64_t)ngroups * tb.noff;
@@ -965,22 +1047,26 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         // enough workgroups to fill the device: split the offsets of a group
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
         if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
+        HQ_CHECK_HIP(hipEventRecord(t->ev[6], s));
         k_fmm_m2l_mfma<P><<<ngroups * nsplit, 512, 0, s>>>(
             nf, tb.noff, nsplit, st.bins.as<uint32_t>(), st.lsrc.as<int>(), st.lcol.as<int16_t>(),
             tb.T.as<float>(), st.Qf.as<float>(), st.Lf.as<float>());
+        HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
         if (ngen)
             k_fmm_m2l_fp64<P><<<(int)std::min<int64_t>(ngen, 1 << 18), 64, 0, s>>>(
                 st.gen.as<int2>(), ngen, nullptr, 0, v, tb.mi.as<int>(), tb.mi2.as<int>(),
                 tb.rec2.as<int>(), tb.m2lidx.as<int>(), tb.m2lcoef.as<double>(), st.Qd.as<double>(),
                 st.Ld.as<double>());
         k_fmm_merge<P><<<grid_for(nf * K, 256), 256, 0, s>>>(nf, st.Lf.as<float>(), st.Ld.as<double>());
-    } else if (ngen + nlat) {
-        k_fmm_m2l_fp64<P><<<(int)std::min<int64_t>(ngen + nlat, 1 << 18), 64, 0, s>>>(
-            st.gen.as<int2>(), ngen, st.lat.as<int3>(), nlat, v, tb.mi.as<int>(), tb.mi2.as<int>(),
-            tb.rec2.as<int>(), tb.m2lidx.as<int>(), tb.m2lcoef.as<double>(), st.Qd.as<double>(),
-            st.Ld.as<double>());
+    } else {
+        HQ_CHECK_HIP(hipEventRecord(t->ev[6], s));
+        if (ngen + nlat)
+            k_fmm_m2l_fp64<P><<<(int)std::min<int64_t>(ngen + nlat, 1 << 18), 64, 0, s>>>(
+                st.gen.as<int2>(), ngen, st.lat.as<int3>(), nlat, v, tb.mi.as<int>(),
+                tb.mi2.as<int>(), tb.rec2.as<int>(), tb.m2lidx.as<int>(), tb.m2lcoef.as<double>(),
+                st.Qd.as<double>(), st.Ld.as<double>());
+        HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
     }
-    HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
     // 5. L2L, top-down
     for (int d = 1; d <= MAX_DEPTH; ++d) {
         const int f0 = st.fls[d], f1 = st.fls[d + 1];
@@ -1021,18 +1107,23 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     auto leaf = [&](auto kern) {
         kern<<<(int)nf, 64, 0, s>>>((int)nf, v, t->sx.as<double>(), t->sy.as<double>(),
                                     t->sz.as<double>(), t->sm.as<double>(), tb.mi.as<int>(),
-                                    tb.mi2.as<int>(), tb.rec2.as<int>(), st.Ld.as<double>(),
-                                    st.Qd.as<double>(),
-                                    st.phist.as<uint32_t>(), st.plist.as<int>(),
-                                    st.mhist.as<uint32_t>(), st.mlist.as<int>(), G, eps2,
-                                    t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>());
+                                    st.Ld.as<double>(), st.phist.as<uint32_t>(), st.plist.as<int>(),
+                                    G, eps2, t->fx.as<double>(), t->fy.as<double>(),
+                                    t->fz.as<double>());
     };
     if (mfma) leaf(k_fmm_leaf<P, true>);
     else leaf(k_fmm_leaf<P, false>);
+    if (nm2p)
+        k_fmm_m2p<P><<<(int)nf, 64, 0, s>>>((int)nf, v, t->sx.as<double>(), t->sy.as<double>(),
+                                            t->sz.as<double>(), t->sm.as<double>(), tb.mi2.as<int>(),
+                                            tb.rec2.as<int>(), st.Qd.as<double>(),
+                                            st.mhist.as<uint32_t>(), st.mlist.as<int>(), G,
+                                            t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>());
     HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
     HQ_CHECK_HIP(hipStreamSynchronize(s));
     HQ_CHECK_HIP(hipGetLastError());
-    hipEventElapsedTime(&t->ms_lists, t->ev[5], t->ev[7]);  // M2L
+    hipEventElapsedTime(&t->ms_walk, t->ev[5], t->ev[8]);   // the whole evaluation
+    hipEventElapsedTime(&t->ms_lists, t->ev[6], t->ev[7]);  // the same-level M2L kernel
     hipEventElapsedTime(&t->ms_eval, t->ev[7], t->ev[8]);   // L2L, near field, L2P
     t->n_m2p = nlat + ngen;
     t->n_p2p = np2p;
