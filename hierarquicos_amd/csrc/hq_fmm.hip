@@ -34,7 +34,8 @@ namespace hq {
 constexpr int PMAX = 6;
 constexpr int KMAX = 84;     // (p+1)(p+2)(p+3)/6 at p = 6
 constexpr int K2MAX = 455;   // same at 2p = 12
-constexpr int GT = 512;      // targets per MFMA workgroup (8 waves x 4 column tiles x 16)
+// target cubes per M2L workgroup (their FP32 locals live in LDS)
+__host__ __device__ constexpr int group_of(int) { return 256; }
 
 __host__ __device__ constexpr int ncoef(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int kpad(int p) { return (ncoef(p) + 15) / 16 * 16; }
@@ -499,24 +500,24 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(FView v, const double* __r
 // ------------------------------------------------------------------ binning
 // lattice pairs -> bins (target group, offset); near pairs -> bins (target)
 __global__ __launch_bounds__(256) void k_fmm_lat_hist(const int3* __restrict__ lat, int64_t n,
-                                                      int noff, uint32_t* __restrict__ hist) {
+                                                      int noff, int gt, uint32_t* __restrict__ hist) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int3 e = lat[i];
-        atomicAdd(&hist[(int64_t)(e.x / GT) * noff + e.z], 1u);
+        atomicAdd(&hist[(int64_t)(e.x / gt) * noff + e.z], 1u);
     }
 }
 
 __global__ __launch_bounds__(256) void k_fmm_lat_scatter(const int3* __restrict__ lat, int64_t n,
-                                                         int noff, uint32_t* __restrict__ pos,
+                                                         int noff, int gt, uint32_t* __restrict__ pos,
                                                          int* __restrict__ src,
                                                          int16_t* __restrict__ col) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int3 e = lat[i];
-        const uint32_t o = atomicAdd(&pos[(int64_t)(e.x / GT) * noff + e.z], 1u);
+        const uint32_t o = atomicAdd(&pos[(int64_t)(e.x / gt) * noff + e.z], 1u);
         src[o] = e.y;
-        col[o] = (int16_t)(e.x % GT);
+        col[o] = (int16_t)(e.x % gt);
     }
 }
 
@@ -539,11 +540,14 @@ __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict_
 
 // ------------------------------------------------------------------ M2L
 // Lattice M2L on the matrix cores.  One workgroup per group of GT target
-// nodes; for every offset with pairs in the group, T_o is staged in LDS and
-// L~[:, targets] += T_o Q~[:, sources] runs as 16x16x4 FP32 MFMA tiles, the
-// accumulators staying in registers across offsets (no atomics).
-// v_mfma_f32_16x16x4f32 operand layout (wave64): A[i][k] from lane i + 16k,
-// B[k][j] from lane j + 16k, D[4q + r][j] in lane j + 16q, element r.
+// cubes keeps their FP32 locals in LDS.  The group's pairs are binned by
+// operator; wavefront w takes the non-empty bins w, w+8, ..., loads the bin's
+// operator T (Kp x Kp) from L2 into registers once, multiplies it with the
+// bin's source moments 16 columns at a time by v_mfma_f32_16x16x4f32 and adds
+// the result into the targets' locals with LDS atomics: no barriers inside
+// the loop.
+// Operand layout (wave64): A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, D[4q + r][j] in lane j + 16q, element r.
 template <int P>
 __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int nsplit,
                                                       const uint32_t* __restrict__ bstart,
@@ -552,11 +556,9 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
                                                       const float* __restrict__ T,
                                                       const float* __restrict__ Qf,
                                                       float* __restrict__ Lf) {
-    constexpr int Kp = kpad(P), RT = Kp / 16, KT = Kp / 4, CT = GT / 16 / 8;
-    __shared__ float4 tl4[Kp * Kp / 4];
-    __shared__ int colsrc[GT], coltag[GT];  // source of each target column, tagged by offset
-    __shared__ uint32_t bs[GT + 1];         // bin starts of the current chunk of offsets
-    const float* tl = reinterpret_cast<const float*>(tl4);
+    constexpr int K = ncoef(P), Kp = kpad(P), RT = Kp / 16, KT = Kp / 4, NW = 8;
+    constexpr int GT = group_of(P);
+    __shared__ float acc[GT * K];
     // block (g, sp): target group g, offsets [o_lo, o_hi); with several
     // splits per group the partial sums meet in Lf by atomic adds
     const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
@@ -564,58 +566,52 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
     const int o_lo = sp * per, o_hi = min(noff, o_lo + per);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    coltag[tid] = -1;
-    floatx4 acc[CT][RT];
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) acc[ct][rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int i = tid; i < GT * K; i += 512) acc[i] = 0.0f;
+    __syncthreads();
+    const uint32_t* bsg = bstart + (int64_t)g * noff;
+    float av[RT][KT];
+    int nbin = 0;  // running count of non-empty bins: bin k belongs to wavefront k % NW
     for (int o = o_lo; o < o_hi; ++o) {
-        if ((o - o_lo) % GT == 0) {  // next chunk of GT offsets: their bin starts into LDS
-            __syncthreads();
-            const int64_t base = (int64_t)g * noff + o;
-            const int cnt = min(GT, o_hi - o);
-            for (int i = tid; i <= cnt; i += 512) bs[i] = bstart[base + i];
-            __syncthreads();
-        }
-        const uint32_t b0 = bs[(o - o_lo) % GT], b1 = bs[(o - o_lo) % GT + 1];
+        const uint32_t b0 = bsg[o], b1 = bsg[o + 1];
         if (b0 == b1) continue;
-        __syncthreads();  // the previous offset's tile is no longer read
-        const float4* t4 = reinterpret_cast<const float4*>(T + (int64_t)o * Kp * Kp);
-        for (int i = tid; i < Kp * Kp / 4; i += 512) tl4[i] = t4[i];
-        for (uint32_t e = b0 + tid; e < b1; e += 512) {
-            colsrc[col[e]] = src[e];
-            coltag[col[e]] = o;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-            const int cbase = wv * (GT / 8) + ct * 16;
-            const int s = coltag[cbase + li] == o ? colsrc[cbase + li] : -1;
-            if (__ballot(s >= 0) == 0) continue;
-            float bv[KT];
-#pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bv[kt] = s >= 0 ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-                for (int kt = 0; kt < KT; ++kt)
-                    acc[ct][rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                        tl[(16 * rt + li) * Kp + 4 * kt + lk], bv[kt], acc[ct][rt], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-        const int64_t f = (int64_t)g * GT + wv * (GT / 8) + ct * 16 + li;
-        if (f >= nf) continue;
+        if (nbin++ % NW != wv) continue;
+        const int nt = (int)((b1 - b0 + 15) / 16);
+        const float* to = T + (int64_t)o * Kp * Kp;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float* dst = &Lf[f * Kp + 16 * rt + 4 * lk + r];
-                if (nsplit == 1) *dst = acc[ct][rt][r];
-                else if (acc[ct][rt][r] != 0.0f) atomicAdd(dst, acc[ct][rt][r]);
+            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kp + 4 * kt + lk];
+        for (int t = 0; t < nt; ++t) {
+            const uint32_t e = b0 + (uint32_t)(t * 16) + li;
+            const bool ok = e < b1;
+            const int s = ok ? src[e] : -1;
+            const int c = ok ? col[e] : 0;
+            float bv[KT];
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) bv[kt] = ok ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                floatx4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt)
+                    d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d, 0, 0, 0);
+                if (ok)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * rt + 4 * lk + r;
+                        if (row < K) atomicAdd(&acc[c * K + row], d[r]);
+                    }
             }
+        }
+    }
+    __syncthreads();
+    const int64_t f0 = (int64_t)g * GT;
+    for (int i = tid; i < GT * K; i += 512) {
+        const int64_t f = f0 + i / K;
+        if (f >= nf) break;
+        float* dst = &Lf[f * Kp + i % K];
+        if (nsplit == 1) *dst = acc[i];
+        else if (acc[i] != 0.0f) atomicAdd(dst, acc[i]);
     }
 }
 
@@ -1028,6 +1024,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     t->list_parts = np2p;
     // 4. M2L
     if (mfma) {
+        const int GT = group_of(P);
         const int ngroups = grid_for(nf, GT);
         const int64_t nbin = (int64_t)ngroups * tb.noff;
         if ((rc = st.bins.ensure((nbin + 1) * 4)) || (rc = st.binpos.ensure((nbin + 1) * 4)) ||
@@ -1036,13 +1033,13 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
             return rc;
         HQ_CHECK_HIP(hipMemsetAsync(st.bins.p, 0, (nbin + 1) * 4, s));
         if (nlat)
-            k_fmm_lat_hist<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, tb.noff,
+            k_fmm_lat_hist<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, tb.noff, GT,
                                                               st.bins.as<uint32_t>());
         if ((rc = scan_excl_buf(st.bins.as<uint32_t>(), nbin, t->scanpart, s))) return rc;
         HQ_CHECK_HIP(hipMemcpyAsync(st.binpos.p, st.bins.p, (nbin + 1) * 4, hipMemcpyDeviceToDevice, s));
         if (nlat)
             k_fmm_lat_scatter<<<capped(nlat), 256, 0, s>>>(
-                st.lat.as<int3>(), nlat, tb.noff, st.binpos.as<uint32_t>(), st.lsrc.as<int>(),
+                st.lat.as<int3>(), nlat, tb.noff, GT, st.binpos.as<uint32_t>(), st.lsrc.as<int>(),
                 st.lcol.as<int16_t>());
         // enough workgroups to fill the device: split the offsets of a group
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
