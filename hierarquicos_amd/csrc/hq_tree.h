@@ -49,11 +49,11 @@ struct hq_tree {
     int64_t lost = 0;
     // groups (methods 30-32) and bunches
     hq::DevBuf groups, gflag, bstart;
-    int super_size = 256;
+    int super_size = 256;  // bunching: sibling subtrees packed up to this many bodies
     int leaf = 0;          // bunched walks: cells of <= leaf bodies enter as their bodies (0: off)
     int fmm_leaf = 32;     // FMM: cells of <= fmm_leaf bodies are leaves (1..64)
     std::unique_ptr<hq::FmmState, hq::FmmFree> fmm;
-    bool fused = true;     // mixed bunched walk as one fused kernel  // bunching: sibling subtrees packed up to this many bodies
+    bool fused = true;     // mixed bunched walk as one fused kernel
     int64_t ngroups = 0;
     // walk scratch
     hq::DevBuf fx, fy, fz, counts, clist, plist, lstat;
