@@ -143,7 +143,8 @@ def run_parareal(args):
     m, q, v = ics.plummer(n, seed=args.seed, velocities=True)
     p = v * m[:, None]
     csteps, fsteps = 6, 64
-    coarse = {"force": "tree", "method": 10, "theta": 0.9, "steps": csteps}
+    coarse = {"force": args.coarse_force, "method": 10, "theta": 0.9, "steps": csteps,
+              "scheme": args.coarse_scheme, "precision": "mixed"}
     fine = {"force": "bunched", "method": 11, "theta": args.theta, "steps": fsteps,
             "precision": "mixed"}
     dtf = tf / nw / fsteps
@@ -184,8 +185,8 @@ def run_parareal(args):
         "data": "synthetic (seeded Plummer sphere with velocities)",
         "config": {"workload": f"parareal_plummer_n{n}_w{nw}_t4", "N": n, "windows": nw,
                    "tf": tf, "eps": eps,
-                   "coarse": f"symplectic Euler, monopole BH theta 0.9 (FP64), {csteps} steps "
-                             f"per window (dt {tf / nw / csteps:.4g})",
+                   "coarse": f"{args.coarse_scheme}, monopole BH theta 0.9 ({args.coarse_force} "
+                             f"walk), {csteps} steps per window (dt {tf / nw / csteps:.4g})",
                    "fine": f"Verlet, quadrupole BH theta {args.theta} bunched FP32, {fsteps} "
                            f"steps per window (dt {dtf:.4g})",
                    "correction": "F + (G_new - G_old)"},
@@ -294,6 +295,10 @@ def main():
     ap.add_argument("--systems", type=int, default=256, help="ensemble: number of systems")
     ap.add_argument("--dt", type=float, default=1e-3, help="ensemble: Verlet step")
     ap.add_argument("--n-parareal", type=int, default=10000)
+    ap.add_argument("--coarse-scheme", choices=("euler", "verlet"), default="euler",
+                    help="parareal: coarse propagator (euler is the reference's G_til)")
+    ap.add_argument("--coarse-force", choices=("tree", "bunched"), default="bunched",
+                    help="parareal: coarse walk (tree = per-target FP64, bunched = mixed)")
     ap.add_argument("--n-fmm", type=int, default=1 << 22)
     ap.add_argument("--fmm-profiles", default="cube,plummer")
     ap.add_argument("--fmm-orders", default="4,6")

@@ -44,7 +44,7 @@ class PararealParams(C.Structure):
                 ("coarse_theta2", C.c_double),
                 ("fine_force", C.c_int), ("fine_method", C.c_int), ("fine_precision", C.c_int),
                 ("fine_steps", C.c_int), ("fine_theta2", C.c_double),
-                ("group_size", C.c_int), ("correction", C.c_int)]
+                ("group_size", C.c_int), ("correction", C.c_int), ("coarse_scheme", C.c_int)]
 
 EXPORTS = (
     "hq_compute_forces_direct", "hq_compute_forces", "hq_generate_tree", "hq_tree_create",

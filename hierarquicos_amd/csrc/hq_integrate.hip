@@ -432,6 +432,8 @@ int hq_parareal(const hq_parareal_params* prm, const double* m, const double* x0
     if (prm->coarse_steps < 1 || prm->fine_steps < 1) return fail(HQ_ERR_ARGS, "step counts must be >= 1");
     if (prm->correction != HQ_CORR_REFERENCE && prm->correction != HQ_CORR_EXACT)
         return fail(HQ_ERR_ARGS, "bad correction mode");
+    if (prm->coarse_scheme != HQ_SCHEME_EULER && prm->coarse_scheme != HQ_SCHEME_VERLET)
+        return fail(HQ_ERR_ARGS, "bad coarse scheme");
     if (where != HQ_HOST && where != HQ_DEVICE) return fail(HQ_ERR_ARGS, "bad location");
     const double window = prm->tf / nw;
     const double dtc = window / prm->coarse_steps, dtf = window / prm->fine_steps;
@@ -473,10 +475,10 @@ int hq_parareal(const hq_parareal_params* prm, const double* m, const double* x0
                                 Un.as<double>(), Un.as<double>(), Un.as<double>(), HQ_DEVICE)))
         return rc;  // sizes the fine handle's state buffers; contents are loaded per iteration
     auto coarse = [&](double* U, int w) -> int {
-        // G_til of window w: the state U[.., w] through coarse_steps Euler steps into Gn
+        // G_til of window w: the state U[.., w] through coarse_steps steps into Gn
         int r;
         if ((r = load_window(tc.get(), U, nw, n, w, 1))) return r;
-        if ((r = integrate(tc.get(), HQ_SCHEME_EULER, dtc, prm->coarse_steps, prm->coarse_theta2,
+        if ((r = integrate(tc.get(), prm->coarse_scheme, dtc, prm->coarse_steps, prm->coarse_theta2,
                            prm->G, prm->eps2, prm->coarse_force)))
             return r;
         return store_state(tc.get(), Gn.as<double>(), n);

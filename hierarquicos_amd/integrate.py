@@ -124,7 +124,8 @@ def integrate_parareal(m, q0, p0, tf, num_windows, N_itermax, theta, G, eps, max
     direct when theta == -1), fine = qntt Verlet steps with direct forces.
     coarse / fine may instead be dicts with keys force ("tree" | "bunched" |
     "direct"), method (10-12, 30-32), theta (opening angle), steps (per
-    window) and precision ("fp64" | "mixed", bunched only).
+    window) and precision ("fp64" | "mixed", bunched only); coarse may also set
+    scheme ("euler", the reference's G_til, or "verlet").
 
     correction: "exact" computes F + (G_new - G_old), so a converged window is
     the sequential fine solution bit for bit; "reference" keeps
@@ -154,6 +155,8 @@ def integrate_parareal(m, q0, p0, tf, num_windows, N_itermax, theta, G, eps, max
         setattr(prm, pre + "_theta2", th * th)
     prm.group_size = group_size
     prm.correction = {"exact": L.HQ_CORR_EXACT, "reference": L.HQ_CORR_REFERENCE}[correction]
+    prm.coarse_scheme = {"euler": L.HQ_SCHEME_EULER,
+                         "verlet": L.HQ_SCHEME_VERLET}[coarse.get("scheme", "euler")]
     cols = [np.ascontiguousarray(a[:, c]) for a in (q0, p0) for c in range(3)]
     U = np.zeros(6 * num_windows * n)
     errors = np.zeros(max(N_itermax - 1, 1))

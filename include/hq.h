@@ -197,6 +197,7 @@ typedef struct {
     double fine_theta2;
     int group_size;        /* bunch size of bunched walks (<= 0: 64) */
     int correction;        /* HQ_CORR_REFERENCE or HQ_CORR_EXACT */
+    int coarse_scheme;     /* HQ_SCHEME_EULER (the reference's G_til) or HQ_SCHEME_VERLET */
 } hq_parareal_params;
 /* U_out: 6 * num_windows * n doubles, component-major:
  * U_out[(c * num_windows + w) * n + a], c = x, y, z, px, py, pz.
