@@ -170,11 +170,37 @@ static __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(uint32_t* __
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) a[len] = part[gridDim.x];
 }
 
+// One-workgroup scan of an array of at most 1024 * SCAN_ONE_ITEMS entries:
+// thread t owns a contiguous run of ceil(len / 1024) entries.
+constexpr int SCAN_ONE_ITEMS = 32;
+static __global__ __launch_bounds__(1024) void k_scan_one(uint32_t* __restrict__ a, int64_t len) {
+    __shared__ uint32_t ws[16];
+    const int per = (int)((len + 1023) / 1024);
+    const int64_t b = (int64_t)threadIdx.x * per;
+    uint32_t v[SCAN_ONE_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ONE_ITEMS; ++it) {
+        const int64_t i = b + it;
+        v[it] = (it < per && i < len) ? a[i] : 0u;
+        sum += v[it];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(sum, ws, tot);
+#pragma unroll
+    for (int it = 0; it < SCAN_ONE_ITEMS; ++it) {
+        const int64_t i = b + it;
+        if (it < per && i < len) a[i] = run;
+        run += v[it];
+    }
+    if (threadIdx.x == 0) a[len] = tot;
+}
+
 // Exclusive scan in place; grand total to a[len]; part: scratch (>= tiles + 1)
 static inline void scan_excl(uint32_t* a, int64_t len, uint32_t* part, hipStream_t s) {
     const int tiles = (int)((len + SCAN_TILE - 1) / SCAN_TILE);
-    if (tiles <= 0) {
-        k_scan_small<<<1, 1024, 0, s>>>(a, 0);
+    if (len <= 1024 * SCAN_ONE_ITEMS) {
+        k_scan_one<<<1, 1024, 0, s>>>(a, len);
         return;
     }
     k_scan_tiles<<<tiles, SCAN_THREADS, 0, s>>>(a, len, part);
