@@ -201,11 +201,17 @@ def run_fmm(args):
     and 6, theta 0.5; same-level M2L on the matrix cores (FP32) beside the
     scalar FP64 M2L; error against the FP64 direct sum on a sample."""
     from hierarquicos_amd import DeviceArray, OctreeType, direct_forces, ics
+    from hierarquicos_amd import _lib as L
     from hierarquicos_amd.device import sync
     N = args.n_fmm
     th2 = args.theta ** 2
     rng = np.random.default_rng(1234)
     sample = np.sort(rng.choice(N, args.sample, replace=False))
+    # a plain 256 MiB device-to-device copy: the reference point that scales
+    # the FETCH_SIZE / WRITE_SIZE counters of a --pmc profile to bytes
+    probe_a, probe_b = DeviceArray((1 << 25,)), DeviceArray((1 << 25,))
+    L.check(L.lib().hq_dev_copy(probe_b.ptr, probe_a.ptr, probe_a.nbytes, 2))
+    del probe_a, probe_b
     cases = []
     for prof in args.fmm_profiles.split(","):
         if prof == "cube":
@@ -250,20 +256,27 @@ def run_fmm(args):
         t.close()
     head = cases[0]
     peak = 157.3 if head["m2l"] == "mfma" else 78.6
+    K0 = (head["order"] + 1) * (head["order"] + 2) * (head["order"] + 3) // 6
+    workload = f"fmm_{head['profile']}_n{N}_p{head['order']}_theta{args.theta}"
+    traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
+    tf = os.path.join(REPO, "profiles", f"traffic_k_fmm_m2l_mfma_{head['profile']}_n{N}_p{head['order']}.json")
+    if os.path.exists(tf) and head["m2l"] == "mfma":
+        rec = json.load(open(tf))
+        if rec.get("workload") == workload:
+            traffic = rec["hbm_bytes_per_launch"]
     print(json.dumps({
         "metric": "FMM particles/s at N=4M uniform cube, p=4, theta=0.5 (matrix-core M2L)",
         "value": head["particles_per_s"], "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 1, "ms_per_step": head["ms_per_eval"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded uniform cube and Plummer sphere)",
-        "config": {"workload": f"fmm_cube_n{N}_p4_theta{args.theta}", "N": N,
-                   "theta": args.theta, "leaf": 32},
+        "config": {"workload": workload, "N": N, "theta": args.theta, "leaf": 32},
         "rms_rel_err": head["rms_rel_err"], "max_rel_err": head["max_rel_err"],
-        "roofline": {"bound": "mfma", "kernel": "k_fmm_m2l_mfma<4>",
+        "roofline": {"bound": "mfma", "kernel": f"k_fmm_m2l_mfma<{head['order']}>",
                      "achieved": head["m2l_tflops"], "peak": peak, "unit": "TFLOP/s",
-                     "frac": head["m2l_tflops"] / peak, "traffic": None,
-                     "flops_per_launch": 2.0 * 35 * 35 * head["same_level_m2l_pairs"],
-                     "flops_note": "2 K^2 per same-level pair, K = 35 coefficients (no padding)"},
+                     "frac": head["m2l_tflops"] / peak, "traffic": traffic,
+                     "flops_per_launch": 2.0 * K0 * K0 * head["same_level_m2l_pairs"],
+                     "flops_note": f"2 K^2 per same-level pair, K = {K0} coefficients (no padding)"},
         "cases": cases, "cpu_baseline": None}))
 
 
