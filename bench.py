@@ -317,8 +317,8 @@ def main():
     ap.add_argument("--fmm-orders", default="4,6")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the force path here runs on a single MI355X; "
-              "see DESIGN.md for how it would shard across GPUs.", file=sys.stderr)
+        print("bench.py measures one GPU: the force path here runs on a single MI355X "
+              "and has no multi-GPU mode (--gpus must be 1).", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
         if args.eps == 1e-3:
