@@ -577,18 +577,37 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
         if (nbin++ % NW != wv) continue;
         const int nt = (int)((b1 - b0 + 15) / 16);
         const float* to = T + (int64_t)o * Kp * Kp;
+        // the operator and the first tile's sources are fetched together; each
+        // later tile's sources are fetched while the previous tile computes
+        float bn[KT];
+        int cn;
+        bool okn;
+        {
+            const uint32_t e = b0 + (uint32_t)li;
+            okn = e < b1;
+            const int s = okn ? src[e] : 0;
+            cn = okn ? col[e] : 0;
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+        }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
             for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kp + 4 * kt + lk];
         for (int t = 0; t < nt; ++t) {
-            const uint32_t e = b0 + (uint32_t)(t * 16) + li;
-            const bool ok = e < b1;
-            const int s = ok ? src[e] : -1;
-            const int c = ok ? col[e] : 0;
             float bv[KT];
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bv[kt] = ok ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+            for (int kt = 0; kt < KT; ++kt) bv[kt] = bn[kt];
+            const bool ok = okn;
+            const int c = cn;
+            if (t + 1 < nt) {
+                const uint32_t e = b0 + (uint32_t)((t + 1) * 16) + li;
+                okn = e < b1;
+                const int s = okn ? src[e] : 0;
+                cn = okn ? col[e] : 0;
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+            }
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
                 floatx4 d = {0.f, 0.f, 0.f, 0.f};
