@@ -166,6 +166,8 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
     } else if (key == HQ_TUNE_FMM_LEAF) {
         if (value < 1 || value > 64) return fail(HQ_ERR_ARGS, "FMM leaf size %lld outside 1..64", (long long)value);
         t->fmm_leaf = (int)value;
+    } else if (key == HQ_TUNE_GRAPH) {
+        t->graph = value != 0;
     } else if (key == HQ_TUNE_LEAF) {
         if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
         t->leaf = (int)value;

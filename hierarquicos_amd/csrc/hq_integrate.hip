@@ -77,27 +77,38 @@ __global__ __launch_bounds__(DT) void k_direct_systems(const int64_t* __restrict
     }
 }
 
-int direct_forces_systems(hq_tree* t, double G, double eps2) {
+// Block prefix of the per-system direct sum in t->dblk (uploaded only when the
+// systems change) and the force buffers; *nblk = workgroups of one launch.
+static int direct_prepare(hq_tree* t, int* nblk) {
     int rc;
     const int nsys = t->nsys;
     std::vector<int> bofs(nsys + 1, 0);
     for (int k = 0; k < nsys; ++k)
         bofs[k + 1] = bofs[k] + (int)((t->hsysoff[k + 1] - t->hsysoff[k] + DT - 1) / DT);
+    *nblk = bofs[nsys];
     const size_t b8 = t->n * sizeof(double);
-    if ((rc = t->fx.ensure(b8)) || (rc = t->fy.ensure(b8)) || (rc = t->fz.ensure(b8)) ||
-        (rc = t->dblk.ensure((nsys + 1) * sizeof(int))))
-        return rc;
-    hipStream_t s = t->stream;
-    HQ_CHECK_HIP(hipMemcpyAsync(t->dblk.p, bofs.data(), (nsys + 1) * sizeof(int),
-                                hipMemcpyHostToDevice, s));
-    k_direct_systems<<<bofs[nsys], DT, 0, s>>>(t->sysoff.as<int64_t>(), t->dblk.as<int>(), nsys,
-                                               t->m.as<double>(), t->x.as<double>(),
-                                               t->y.as<double>(), t->z.as<double>(), G, eps2,
-                                               t->fx.as<double>(), t->fy.as<double>(),
-                                               t->fz.as<double>());
+    if ((rc = t->fx.ensure(b8)) || (rc = t->fy.ensure(b8)) || (rc = t->fz.ensure(b8))) return rc;
+    if (bofs != t->hbofs) {
+        if ((rc = t->dblk.ensure((nsys + 1) * sizeof(int)))) return rc;
+        hipStream_t s = t->stream;
+        HQ_CHECK_HIP(hipStreamSynchronize(s));  // no copy from the old table is pending
+        t->hbofs = bofs;
+        HQ_CHECK_HIP(hipMemcpyAsync(t->dblk.p, t->hbofs.data(), (nsys + 1) * sizeof(int),
+                                    hipMemcpyHostToDevice, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    return HQ_OK;
+}
+
+int direct_forces_systems(hq_tree* t, double G, double eps2) {
+    int rc, nblk;
+    if ((rc = direct_prepare(t, &nblk))) return rc;
+    k_direct_systems<<<nblk, DT, 0, t->stream>>>(t->sysoff.as<int64_t>(), t->dblk.as<int>(),
+                                                  t->nsys, t->m.as<double>(), t->x.as<double>(),
+                                                  t->y.as<double>(), t->z.as<double>(), G, eps2,
+                                                  t->fx.as<double>(), t->fy.as<double>(),
+                                                  t->fz.as<double>());
     HQ_CHECK_HIP(hipGetLastError());
-    // the host copy of bofs must outlive the async copy
-    HQ_CHECK_HIP(hipStreamSynchronize(s));
     return HQ_OK;
 }
 
@@ -158,11 +169,70 @@ static void kick_drift(hq_tree* t, bool sorted, double h, int nkick, double dt) 
         t->z.as<double>(), t->px.as<double>(), t->py.as<double>(), t->pz.as<double>());
 }
 
+// Captures the launches body() makes on t->stream into a HIP graph and
+// replays it reps times.  body must only launch kernels on buffers that
+// already have their size (no allocation, copy from host or synchronisation).
+template <class F>
+static int replay(hq_tree* t, F body, int64_t reps) {
+    hipStream_t s = t->stream;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HQ_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess)
+        return fail(HQ_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    for (int64_t r = 0; e == hipSuccess && r < reps; ++r) e = hipGraphLaunch(ge, s);
+    if (ge) hipGraphExecDestroy(ge);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(HQ_ERR_HIP, "graph replay failed: %s", hipGetErrorString(e));
+    return HQ_OK;
+}
+
+// Direct-force integration with each step captured once and replayed: the
+// same kernels with the same arguments as the launched loop below, so the
+// same results bit for bit.
+static int integrate_direct_graph(hq_tree* t, int scheme, double dt, int64_t nsteps, double G,
+                                  double eps2) {
+    int rc, nblk;
+    if ((rc = direct_prepare(t, &nblk))) return rc;
+    auto forces = [&]() {
+        k_direct_systems<<<nblk, DT, 0, t->stream>>>(
+            t->sysoff.as<int64_t>(), t->dblk.as<int>(), t->nsys, t->m.as<double>(),
+            t->x.as<double>(), t->y.as<double>(), t->z.as<double>(), G, eps2, t->fx.as<double>(),
+            t->fy.as<double>(), t->fz.as<double>());
+        return HQ_OK;
+    };
+    if (scheme == HQ_SCHEME_VERLET) {
+        const double h = 0.5 * dt;
+        forces();
+        kick_drift(t, false, h, 1, dt);
+        forces();
+        if ((rc = replay(t, [&]() { kick_drift(t, false, h, 2, dt); return forces(); }, nsteps - 1)))
+            return rc;
+        kick_drift(t, false, h, 1, 0.0);
+    } else {
+        if ((rc = replay(t, [&]() { forces(); kick_drift(t, false, dt, 1, dt); return HQ_OK; },
+                         nsteps)))
+            return rc;
+    }
+    HQ_CHECK_HIP(hipGetLastError());
+    HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    return HQ_OK;
+}
+
 int integrate(hq_tree* t, int scheme, double dt, int64_t nsteps, double theta2, double G,
               double eps2, int force) {
     int rc;
     bool sorted = false;
     if (nsteps <= 0) return HQ_OK;
+    if (force == HQ_FORCE_DIRECT && t->graph && nsteps >= 3)
+        return integrate_direct_graph(t, scheme, dt, nsteps, G, eps2);
     if (scheme == HQ_SCHEME_VERLET) {
         // integration.f90:137-171: F(q0); per step p += dt/2 F, q += dt p/m,
         // rebuild, F(q), p += dt/2 F.  The closing kick of a step and the

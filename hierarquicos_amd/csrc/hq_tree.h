@@ -61,7 +61,9 @@ struct hq_tree {
     // integrator state: momenta in input order (positions are x, y, z above),
     // block offsets of the per-system direct sum, energy scratch
     hq::DevBuf px, py, pz, dblk, ework;
+    std::vector<int> hbofs;  // host copy of dblk (per-system block prefix of the direct sum)
     bool have_state = false;
+    bool graph = true;       // integrate: replay captured direct-force steps
     // status block: device + host mirror
     hq::DevBuf status;
     int64_t* hstatus = nullptr;

@@ -144,10 +144,15 @@ class OctreeType:
             return lib.hq_tree_fmm(h, int(order), th2, g, e2, md, p, layout, where)
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
-    def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None):
+    def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None,
+             graph=None):
         """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
-        bucket (cells of <= leaf bodies enter as their bodies) of the bunched walk."""
+        bucket (cells of <= leaf bodies enter as their bodies) of the bunched
+        walk; FMM leaf size; graph: whether integrate() with direct forces
+        replays one captured step (default True)."""
         lib = L.lib()
+        if graph is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_GRAPH, int(bool(graph))))
         if fmm_leaf is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FMM_LEAF, int(fmm_leaf)))
         if leaf is not None:

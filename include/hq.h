@@ -112,10 +112,12 @@ int hq_tree_destroy(hq_tree *t);
                              many bodies enters the lists as its bodies (exact P2P),
                              the leaf-bucket size of a bucketed tree */
 #define HQ_TUNE_FMM_LEAF 5 /* 1..64 (default 32): FMM cells of at most this many bodies are leaves */
+#define HQ_TUNE_GRAPH 6    /* 1 (default): hq_tree_integrate with HQ_FORCE_DIRECT captures one
+                              step in a HIP graph and replays it; 0: launches every step */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
-/* Cartesian FMM of order 4 or 6 on the built tree (SURVEY "Dehnen p=4/6";
- * the reference's Dehnen method, octree_default.f90:1095-1712, stops at
+/* Cartesian FMM of order 4 or 6 on the built tree: Dehnen's method carried to
+ * higher orders (the reference's, octree_default.f90:1095-1712, stops at
  * quadrupole sources).  Expansions about the cube centres; a dual-tree walk
  * with rmax_A + rmax_B <= theta |c_A - c_B|; M2L of same-level pairs as dense
  * products with per-offset operators.  mode HQ_FMM_MFMA: that M2L in FP32 on
