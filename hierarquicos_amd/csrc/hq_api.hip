@@ -30,6 +30,21 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+thread_local bool g_capturing = false;
+bool capturing() { return g_capturing; }
+
+__global__ void k_fill_i64(int64_t* __restrict__ p, int64_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 int morton_keys(int64_t n, const double* m, const double* q, int L, double ssa, int64_t* keys,
                 int* perm, int* L_out, hipStream_t s, double* par_out = nullptr);
 

@@ -679,11 +679,42 @@ __global__ __launch_bounds__(128) void k_upward(int64_t a, int64_t b, int depth,
     if (k < b) upward_node(u, k, depth);
 }
 
+// Graph mode: level `depth` with its bounds lv[depth], lv[depth + 1] read from
+// the device, grid-stride.
+__global__ __launch_bounds__(128) void k_upward_dev(const int64_t* __restrict__ lv, int depth,
+                                                    UpArgs u) {
+    const int64_t a = lv[depth], b = lv[depth + 1];
+    for (int64_t k = a + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < b;
+         k += (int64_t)gridDim.x * blockDim.x)
+        upward_node(u, k, depth);
+}
+
+// Graph mode: the checks the host makes after node emission, as flags that
+// stay set over replayed steps; node count and level bounds for the rest of
+// the build (0 when the tree is unusable, so that no later kernel reads it).
+__global__ void k_graph_build_check(const int64_t* __restrict__ st, int64_t cap, int nsys,
+                                    int64_t* __restrict__ gs) {
+    if (threadIdx.x != 0) return;
+    const int64_t fl = st[ST_FLAGS], nn = st[ST_NNODES];
+    int64_t f = 0;
+    if (fl & 1) f |= GF_INPUT;
+    if (fl & 4) f |= GF_TIES;
+    if ((fl & 2) || nn > cap) f |= GF_CAPACITY;
+    if (st[ST_LEVEL0 + 1] != nsys) f |= GF_ROOTS;
+    if (st[ST_LOST] > 0) f |= GF_LOST;
+    gs[GS_STICKY] |= f;
+    const bool ok = (f & (GF_INPUT | GF_TIES | GF_CAPACITY | GF_ROOTS)) == 0;
+    gs[GS_NN] = ok ? nn : 0;
+    for (int d = 0; d <= NDEP; ++d) gs[GS_LEVEL0 + d] = ok ? min(st[ST_LEVEL0 + d], cap) : 0;
+}
+
 // Parent pointers of every node; roots (the first nroots nodes) get -1.
 __global__ __launch_bounds__(256) void k_parents(int64_t nn, int64_t nroots,
                                                  const int* __restrict__ ncstart,
-                                                 int* __restrict__ nparent) {
+                                                 int* __restrict__ nparent,
+                                                 const int64_t* __restrict__ dnn) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (dnn) nn = *dnn;  // graph mode: the count is on the device
     if (k >= nn) return;
     if (k < nroots) nparent[k] = -1;
     for (int c = ncstart[k]; c < ncstart[k + 1]; ++c) nparent[c] = (int)k;
@@ -693,8 +724,10 @@ __global__ __launch_bounds__(256) void k_parents(int64_t nn, int64_t nroots,
 // ancestor that has one, else END (the threading of a depth-first walk).
 __global__ __launch_bounds__(256) void k_next(int64_t nn, const int* __restrict__ ncstart,
                                               const int* __restrict__ nparent,
-                                              int* __restrict__ nnext) {
+                                              int* __restrict__ nnext,
+                                              const int64_t* __restrict__ dnn) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (dnn) nn = *dnn;
     if (k >= nn) return;
     int x = (int)k;
     int nx = END;
@@ -711,6 +744,38 @@ __global__ __launch_bounds__(256) void k_next(int64_t nn, const int* __restrict_
 }
 
 // ------------------------------------------------------------------ driver
+static UpArgs up_args(const hq_tree* t) {
+    return UpArgs{t->multipole, t->ntype.as<int8_t>(), t->nfirst.as<int>(), t->ncstart.as<int>(),
+                  t->ndigit.as<uint8_t>(), t->sx.as<double>(), t->sy.as<double>(),
+                  t->sz.as<double>(), t->sm.as<double>(), t->nmass.as<double>(),
+                  t->ncomx.as<double>(), t->ncomy.as<double>(), t->ncomz.as<double>(),
+                  t->multipole >= 4 ? t->nquad.as<double>() : nullptr,
+                  t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), t->cap};
+}
+
+// Graph mode, after node emission: the checks on the device, then pointers and
+// moments over the device-held node count and level bounds (grids sized by the
+// capacity).  The same kernels' arithmetic as the launched build.
+static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
+    hipStream_t s = t->stream;
+    int64_t* gs = t->gstate.as<int64_t>();
+    k_graph_build_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), cap, nsys, gs);
+    k_parents<<<grid_for(cap, 256), 256, 0, s>>>(cap, nsys, t->ncstart.as<int>(),
+                                                 t->nparent.as<int>(), gs + GS_NN);
+    k_next<<<grid_for(cap, 256), 256, 0, s>>>(cap, t->ncstart.as<int>(), t->nparent.as<int>(),
+                                              t->nnext.as<int>(), gs + GS_NN);
+    // grids from the level sizes of the last launched build, with a margin
+    // (the kernel strides over whatever the level holds at replay)
+    const UpArgs ua = up_args(t);
+    for (int d = NDEP - 1; d >= 0; --d) {
+        const int64_t sz = t->level_start[d + 1] - t->level_start[d];
+        const int g = (int)std::min<int64_t>(grid_for(sz + sz / 4 + 128, 128), grid_for(cap, 128));
+        k_upward_dev<<<g, 128, 0, s>>>(gs + GS_LEVEL0, d, ua);
+    }
+    HQ_CHECK_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
 int read_status(hq_tree* t) {
     HQ_CHECK_HIP(hipMemcpyAsync(t->hstatus, t->status.p, ST_SIZE * sizeof(int64_t),
                                 hipMemcpyDeviceToHost, t->stream));
@@ -813,8 +878,11 @@ int tree_build(hq_tree* t) {
     hipStream_t s = t->stream;
     double* sg = t->geom.as<double>() + G_SYS;
     const int* sysid = batch ? t->sysid.as<int>() : nullptr;
-    HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
-    HQ_CHECK_HIP(hipEventRecord(t->ev[0], s));
+    const bool gm = t->graph_mode;
+    auto mark = [&](int e) { return gm ? hipSuccess : hipEventRecord(t->ev[e], s); };
+    if (gm) k_fill_i64<<<1, 64, 0, s>>>(t->status.as<int64_t>(), 0, ST_SIZE);
+    else HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+    HQ_CHECK_HIP(mark(0));
     // 1. root cube(s)
     if (!batch) {
         const int nbb = (int)std::min<int64_t>(1024, grid_for(n, 256));
@@ -835,7 +903,7 @@ int tree_build(hq_tree* t) {
                                             t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
                                             t->idx.as<int>(),
                                             batch ? t->khio.as<uint64_t>() : nullptr);
-    HQ_CHECK_HIP(hipEventRecord(t->ev[1], s));
+    HQ_CHECK_HIP(mark(1));
     // 3. sort: by octant key; a batch then stably by system
     if ((rc = radix_sort(t))) return rc;
     if (batch) {
@@ -859,7 +927,7 @@ int tree_build(hq_tree* t) {
                                               t->sx.as<double>(), t->sy.as<double>(),
                                               t->sz.as<double>(), t->sm.as<double>(),
                                               t->sklo.as<uint32_t>(), sysid, t->ssys.as<int>());
-    HQ_CHECK_HIP(hipEventRecord(t->ev[2], s));
+    HQ_CHECK_HIP(mark(2));
     // 4. nodes
     k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),
                                            t->ssys.as<int>(), n, t->lcp.as<int8_t>());
@@ -875,6 +943,7 @@ int tree_build(hq_tree* t) {
         t->ncy.as<double>(), t->ncz.as<double>(), t->nkhi.as<uint64_t>(), t->nklo.as<uint32_t>(),
         t->status.as<int64_t>());
     k_levels<<<1, 64, 0, s>>>(t->hist.as<uint32_t>(), nblk, t->status.as<int64_t>());
+    if (gm) return build_graph_tail(t, cap, nsys);
     HQ_CHECK_HIP(hipEventRecord(t->ev[3], s));
     if ((rc = read_status(t))) return rc;
     int64_t flags = t->hstatus[ST_FLAGS];
@@ -895,16 +964,11 @@ int tree_build(hq_tree* t) {
     // 5. parent and skip pointers (the walks' threading; the roots end every walk)
     const int64_t nn = t->nn;
     k_parents<<<grid_for(nn, 256), 256, 0, s>>>(nn, nsys, t->ncstart.as<int>(),
-                                                t->nparent.as<int>());
+                                                t->nparent.as<int>(), nullptr);
     k_next<<<grid_for(nn, 256), 256, 0, s>>>(nn, t->ncstart.as<int>(), t->nparent.as<int>(),
-                                             t->nnext.as<int>());
+                                             t->nnext.as<int>(), nullptr);
     // 6. moments, deepest level first
-    UpArgs ua{t->multipole, t->ntype.as<int8_t>(), t->nfirst.as<int>(), t->ncstart.as<int>(),
-              t->ndigit.as<uint8_t>(), t->sx.as<double>(), t->sy.as<double>(),
-              t->sz.as<double>(), t->sm.as<double>(), t->nmass.as<double>(),
-              t->ncomx.as<double>(), t->ncomy.as<double>(), t->ncomz.as<double>(),
-              t->multipole >= 4 ? t->nquad.as<double>() : nullptr,
-              t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), cap};
+    const UpArgs ua = up_args(t);
     for (int d = t->most_depth; d >= 0; --d) {
         const int64_t a = t->level_start[d], b = t->level_start[d + 1];
         if (b > a) k_upward<<<grid_for(b - a, 128), 128, 0, s>>>(a, b, d, ua);

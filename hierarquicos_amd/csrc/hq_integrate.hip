@@ -178,18 +178,20 @@ static int replay(hq_tree* t, F body, int64_t reps) {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HQ_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    g_capturing = true;
     const int rc = body();
+    g_capturing = false;
     const hipError_t ec = hipStreamEndCapture(s, &g);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     if (ec != hipSuccess)
         return fail(HQ_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
     hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     for (int64_t r = 0; e == hipSuccess && r < reps; ++r) e = hipGraphLaunch(ge, s);
-    if (ge) hipGraphExecDestroy(ge);
-    hipGraphDestroy(g);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(HQ_ERR_HIP, "graph replay failed: %s", hipGetErrorString(e));
     return HQ_OK;
 }
@@ -227,12 +229,86 @@ static int integrate_direct_graph(hq_tree* t, int scheme, double dt, int64_t nst
 }
 
 int integrate(hq_tree* t, int scheme, double dt, int64_t nsteps, double theta2, double G,
+              double eps2, int force);
+
+// Bunched mixed-precision integration with each step captured once and
+// replayed.  The first step is launched: it sizes the buffers and counts the
+// bunches that bound the captured walk's grid.  Replayed steps flag their
+// errors on the device (gstate); if any is set, the state saved at entry is
+// restored and all steps are launched instead, which reports the error the
+// launched path would report, or completes when the flag was only a grid bound.
+static int integrate_tree_graph(hq_tree* t, int scheme, double dt, int64_t nsteps,
+                                double theta2, double G, double eps2) {
+    int rc;
+    bool sorted = true;
+    hipStream_t s = t->stream;
+    const int64_t n = t->n;
+    if ((rc = graph_prepare(t))) return rc;
+    double* sv = t->gsave.as<double>();
+    double* st[6] = {t->x.as<double>(), t->y.as<double>(), t->z.as<double>(),
+                     t->px.as<double>(), t->py.as<double>(), t->pz.as<double>()};
+    for (int k = 0; k < 6; ++k)
+        HQ_CHECK_HIP(hipMemcpyAsync(sv + k * n, st[k], n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, s));
+    const double h = 0.5 * dt;
+    auto forces = [&]() { return forces_now(t, HQ_FORCE_BUNCHED, theta2, G, eps2, &sorted); };
+    if (scheme == HQ_SCHEME_VERLET) {
+        if ((rc = forces())) return rc;
+        kick_drift(t, true, h, 1, dt);
+        if ((rc = forces())) return rc;
+    } else {
+        if ((rc = forces())) return rc;
+        kick_drift(t, true, dt, 1, dt);
+    }
+    if ((rc = graph_prepare(t))) return rc;
+    k_fill_i64<<<1, 64, 0, s>>>(t->gstate.as<int64_t>(), 0, GS_SIZE);
+    t->graph_nbun = std::min<int64_t>(n, 2 * t->ngroups + 64);
+    t->graph_mode = true;
+    if (scheme == HQ_SCHEME_VERLET)
+        rc = replay(t, [&]() { kick_drift(t, true, h, 2, dt); return forces(); }, nsteps - 1);
+    else
+        rc = replay(t, [&]() {
+            int r = forces();
+            kick_drift(t, true, dt, 1, dt);
+            return r;
+        }, nsteps - 1);
+    t->graph_mode = false;
+    if (rc) return rc;
+    if (scheme == HQ_SCHEME_VERLET) kick_drift(t, true, h, 1, 0.0);
+    int64_t gs[GS_SIZE];
+    HQ_CHECK_HIP(hipMemcpyAsync(gs, t->gstate.as<int64_t>(), sizeof(gs), hipMemcpyDeviceToHost, s));
+    HQ_CHECK_HIP(hipStreamSynchronize(s));
+    HQ_CHECK_HIP(hipGetLastError());
+    if (gs[GS_STICKY] == 0) {
+        // the handle describes the last replayed step's tree, as after launched steps
+        t->nn = gs[GS_NN];
+        t->most_depth = 0;
+        for (int d = 0; d <= MAX_DEPTH + 1; ++d) {
+            t->level_start[d] = (int)gs[GS_LEVEL0 + d];
+            if (d > 0 && t->level_start[d] > t->level_start[d - 1]) t->most_depth = d - 1;
+        }
+        t->ngroups = gs[GS_NBUN];
+        return HQ_OK;
+    }
+    for (int k = 0; k < 6; ++k)
+        HQ_CHECK_HIP(hipMemcpyAsync(st[k], sv + k * n, n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, s));
+    t->graph = false;
+    rc = integrate(t, scheme, dt, nsteps, theta2, G, eps2, HQ_FORCE_BUNCHED);
+    t->graph = true;
+    return rc;
+}
+
+int integrate(hq_tree* t, int scheme, double dt, int64_t nsteps, double theta2, double G,
               double eps2, int force) {
     int rc;
     bool sorted = false;
     if (nsteps <= 0) return HQ_OK;
     if (force == HQ_FORCE_DIRECT && t->graph && nsteps >= 3)
         return integrate_direct_graph(t, scheme, dt, nsteps, G, eps2);
+    if (force == HQ_FORCE_BUNCHED && t->graph && nsteps >= 3 && t->precision == HQ_MIXED &&
+        t->fused)
+        return integrate_tree_graph(t, scheme, dt, nsteps, theta2, G, eps2);
     if (scheme == HQ_SCHEME_VERLET) {
         // integration.f90:137-171: F(q0); per step p += dt/2 F, q += dt p/m,
         // rebuild, F(q), p += dt/2 F.  The closing kick of a step and the

@@ -27,6 +27,9 @@ int fail(int code, const char* fmt, ...);
                               hipGetErrorString(e_), __FILE__, __LINE__);               \
     } while (0)
 
+// True while this thread records a stream capture (set by the integrator).
+bool capturing();
+
 // Simple owning device buffer that only grows (no free on shrink).
 struct DevBuf {
     void* p = nullptr;
@@ -37,6 +40,8 @@ struct DevBuf {
     ~DevBuf() { release(); }
     int ensure(size_t need) {
         if (need <= bytes) return HQ_OK;
+        if (capturing()) return fail(HQ_ERR_ARGS, "buffer of %zu bytes would grow to %zu inside a "
+                                     "captured step", bytes, need);
         if (p) hipFree(p);
         p = nullptr;
         bytes = 0;

@@ -63,7 +63,15 @@ struct hq_tree {
     hq::DevBuf px, py, pz, dblk, ework;
     std::vector<int> hbofs;  // host copy of dblk (per-system block prefix of the direct sum)
     bool have_state = false;
-    bool graph = true;       // integrate: replay captured direct-force steps
+    bool graph = true;       // integrate: replay captured steps (direct and bunched mixed forces)
+    // Graph mode: the build and the bunched walk take their counts (nodes,
+    // level bounds, bunches) from device memory and size their grids by upper
+    // bounds, so that a step can be captured.  gstate holds those counts and
+    // the errors of all replayed steps (GS_* slots); gsave the state before
+    // the replays, to rerun them launched if an error was flagged.
+    bool graph_mode = false;
+    int64_t graph_nbun = 0;  // grid of the walk in graph mode (upper bound of the bunches)
+    hq::DevBuf gstate, gsave;
     // status block: device + host mirror
     hq::DevBuf status;
     int64_t* hstatus = nullptr;
@@ -101,7 +109,22 @@ enum {
     ST_SIZE = 48
 };
 
+// graph-mode slots of hq_tree::gstate (int64)
+enum {
+    GS_STICKY = 0,     // errors of any replayed step: the GF_* bits
+    GS_NN = 1,         // node count, 0 when the build overflowed
+    GS_LEVEL0 = 2,     // level bounds (clamped) at GS_LEVEL0 + d, d = 0..26
+    GS_NBUN = 32,      // bunches of the last walk
+    GS_SIZE = 40
+};
+enum { GF_INPUT = 1, GF_CAPACITY = 2, GF_TIES = 4, GF_ROOTS = 8, GF_LOST = 16, GF_STACK = 32,
+       GF_BUNCHES = 64 };
+
 namespace hq {
+// set while a stream capture is open: buffers must not grow then
+extern thread_local bool g_capturing;
+__global__ void k_fill_i64(int64_t* __restrict__ p, int64_t v, int64_t n);
+__global__ void k_fill_i32(int* __restrict__ p, int v, int64_t n);
 int tree_build(hq_tree* t);
 int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2);
 int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int precision, bool bunched);
@@ -120,4 +143,6 @@ int direct_forces_systems(hq_tree* t, double G, double eps2);
 // forces of the current positions t->x/y/z into t->fx/fy/fz; *sorted tells
 // whether they are in tree order (map t->idx) or input order
 int forces_now(hq_tree* t, int force, double theta2, double G, double eps2, bool* sorted);
+// buffers at graph-mode sizes (before a capture)
+int graph_prepare(hq_tree* t);
 }  // namespace hq

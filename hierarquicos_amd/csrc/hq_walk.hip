@@ -487,8 +487,10 @@ __global__ __launch_bounds__(64) void k_eval_fp64(NodeView nv, const double* __r
 constexpr int REC = 24;
 
 __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeView nv,
-                                               float* __restrict__ rec) {
+                                               float* __restrict__ rec,
+                                               const int64_t* __restrict__ dnn) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (dnn) nn = *dnn;  // graph mode: the count is on the device
     if (k >= nn) return;
     const int64_t S = nv.stride;
     float r[REC];
@@ -754,12 +756,16 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    const float* __restrict__ mom,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz,
-                                                   int64_t* __restrict__ status) {
+                                                   int64_t* __restrict__ status,
+                                                   const int64_t* __restrict__ dnbun) {
     __shared__ int stk[SC];
     __shared__ float4 cring[RING], pring[RING];
     __shared__ float4 own[BUNCH];
-    const int gi = xcd_swizzle(blockIdx.x, gridDim.x);
-    if (gi >= nbunch) return;
+    // graph mode: the grid is an upper bound and the count is on the device;
+    // the swizzle spreads the blocks that have a bunch over all XCDs
+    if (dnbun) nbunch = (int)*dnbun;
+    if ((int)blockIdx.x >= nbunch) return;
+    const int gi = xcd_swizzle(blockIdx.x, nbunch);
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
     const int gf = bstart[gi], gl = bstart[gi + 1] - 1;
@@ -990,8 +996,10 @@ __global__ __launch_bounds__(256) void k_mark_groups(NodeView nv, const int* __r
 // that hold <= gs bodies each are packed greedily into bunches of <= gs bodies;
 // a tree with <= gs bodies is one bunch.  Marks each bunch's first particle.
 __global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, int gs,
-                                                      int* __restrict__ bmark) {
+                                                      int* __restrict__ bmark,
+                                                      const int64_t* __restrict__ dnn) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (dnn) nn = *dnn;
     if (k >= nn) return;
     const int cnt = nv.count[k];
     if (nv.depth[k] == 0 && cnt <= gs) {  // a root (one per system) small enough to bunch whole
@@ -1019,9 +1027,15 @@ __global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, i
 
 // split each super-group [sg[i], sg[i+1]) into ceil(c / B) bunches of near-equal
 // size (contiguous in tree order)
+// (graph mode: nsg from the device, *dnsg, and entries nsg .. len-1 of cnt zeroed)
 __global__ __launch_bounds__(256) void k_bunch_count(const int* __restrict__ sg, int nsg, int B,
-                                                     uint32_t* __restrict__ cnt) {
+                                                     uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ dnsg, int len) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dnsg) {
+        nsg = (int)*dnsg;
+        if (i >= nsg && i < len) cnt[i] = 0u;
+    }
     if (i >= nsg) return;
     const int c = sg[i + 1] - sg[i];
     cnt[i] = (uint32_t)((c + B - 1) / B);
@@ -1029,8 +1043,10 @@ __global__ __launch_bounds__(256) void k_bunch_count(const int* __restrict__ sg,
 
 __global__ __launch_bounds__(256) void k_bunch_emit(const int* __restrict__ sg, int nsg, int B,
                                                     const uint32_t* __restrict__ off,
-                                                    int* __restrict__ out) {
+                                                    int* __restrict__ out,
+                                                    const uint32_t* __restrict__ dnsg) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dnsg) nsg = (int)*dnsg;
     if (i >= nsg) return;
     const int c = sg[i + 1] - sg[i];
     const int nb = (c + B - 1) / B;
@@ -1064,7 +1080,42 @@ __global__ __launch_bounds__(256) void k_flag_write(const int* __restrict__ gmar
     if (f) out[pre + __popcll(m & lanemask_lt())] = gmark[i];
 }
 
+// Graph mode: a[*pos] = v (the end marker after a table whose length is on the device)
+__global__ void k_graph_set_end(int* __restrict__ a, const uint32_t* __restrict__ pos, int v) {
+    if (threadIdx.x == 0) a[*pos] = v;
+}
+
+// Graph mode: the bunch count for the walk's grid test, and a flag when the
+// grid (sized from a launched step) is too small for it.
+__global__ void k_graph_bunches(const uint32_t* __restrict__ nb, int64_t grid,
+                                int64_t* __restrict__ gs) {
+    if (threadIdx.x != 0) return;
+    gs[GS_NBUN] = *nb;
+    if ((int64_t)*nb > grid) gs[GS_STICKY] |= GF_BUNCHES;
+}
+
+// Graph mode: the walk's stack overflow, kept over replayed steps.
+__global__ void k_graph_walk_check(const int64_t* __restrict__ st, int64_t* __restrict__ gs) {
+    if (threadIdx.x == 0 && (st[ST_FLAGS] & 16)) gs[GS_STICKY] |= GF_STACK;
+}
+
 int tree_make_groups(hq_tree* t, bool bunched);
+
+// Buffers at the sizes graph mode uses (upper bounds), so that nothing grows
+// while a step is captured.
+int graph_prepare(hq_tree* t) {
+    int rc;
+    const int64_t n = t->n;
+    if ((rc = ensure_forces(t)) || (rc = t->gstate.ensure(GS_SIZE * sizeof(int64_t))) ||
+        (rc = t->gsave.ensure((size_t)6 * n * sizeof(double))) ||
+        (rc = t->bstart.ensure(((size_t)n + 1) * sizeof(int))) ||
+        (rc = t->gflag.ensure(((size_t)n + 1) * sizeof(int))) ||
+        (rc = t->groups.ensure(((size_t)n + 1) * sizeof(int))) ||
+        (rc = t->mom32.ensure((size_t)t->cap * REC * sizeof(float))) ||
+        (rc = t->scanpart.ensure(scan_part_size(2 * n + 65536) * sizeof(uint32_t))))
+        return rc;
+    return HQ_OK;
+}
 
 int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int precision,
                        bool bunched) {
@@ -1075,10 +1126,34 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
     if ((rc = tree_make_groups(t, bunched))) return rc;
     ngroups = (int)t->ngroups;
     const bool mixed = bunched && precision == HQ_MIXED;
+    if (t->graph_mode) {
+        // captured step: node and bunch counts stay on the device, grids are
+        // upper bounds (capacity, bunches of the launched step with margin)
+        if (!mixed || !t->fused) return fail(HQ_ERR_ARGS, "graph mode needs the fused mixed walk");
+        int64_t* gs = t->gstate.as<int64_t>();
+        if ((rc = t->mom32.ensure((size_t)t->cap * REC * sizeof(float)))) return rc;
+        k_mom32<<<grid_for(t->cap, 256), 256, 0, s>>>(t->cap, t->multipole, node_view(t),
+                                                      t->mom32.as<float>(), gs + GS_NN);
+        k_fill_i64<<<1, 64, 0, s>>>(t->status.as<int64_t>(), 0, ST_SIZE);
+        auto launch = [&](auto kern) {
+            kern<<<(int)t->graph_nbun, 64, 0, s>>>(
+                node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(), t->sy.as<double>(),
+                t->sz.as<double>(), t->sm.as<double>(), t->n, t->bstart.as<int>(),
+                (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf, t->mom32.as<float>(),
+                t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>(),
+                t->status.as<int64_t>(), gs + GS_NBUN);
+        };
+        if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
+        else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
+        else launch(k_walk_mixed<8, SCAP_FUSED>);
+        k_graph_walk_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), gs);
+        HQ_CHECK_HIP(hipGetLastError());
+        return HQ_OK;
+    }
     if (mixed) {
         if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
         k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
-                                                     t->mom32.as<float>());
+                                                     t->mom32.as<float>(), nullptr);
         if (t->fused) {
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
@@ -1088,7 +1163,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
                                             theta2, G, (float)eps2, t->leaf, t->mom32.as<float>(),
                                             t->fx.as<double>(), t->fy.as<double>(),
-                                            t->fz.as<double>(), t->status.as<int64_t>());
+                                            t->fz.as<double>(), t->status.as<int64_t>(), nullptr);
             };
             if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
             else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
@@ -1209,17 +1284,21 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     int rc;
     const int64_t n = t->n;
     hipStream_t s = t->stream;
-    if (t->lost > 0)
+    const bool gm = t->graph_mode;
+    if (!gm && t->lost > 0)  // graph mode: the build flags it (GF_LOST)
         return fail(HQ_ERR_LOST, "[prepare_groups] body lost: %lld particles at max depth",
                     (long long)t->lost);
     if ((rc = t->gflag.ensure(n * sizeof(int))) || (rc = t->groups.ensure((n + 1) * sizeof(int))))
         return rc;
-    HQ_CHECK_HIP(hipMemsetAsync(t->gflag.p, 0xFF, n * sizeof(int), s));
+    if (gm) k_fill_i32<<<grid_for(n, 256 * 8), 256, 0, s>>>(t->gflag.as<int>(), -1, n);
+    else HQ_CHECK_HIP(hipMemsetAsync(t->gflag.p, 0xFF, n * sizeof(int), s));
     const int B = std::min(t->group_size, BUNCH);
     const int S = std::max(B, t->super_size);
+    const int64_t nnb = gm ? t->cap : t->nn;  // graph mode: the count is on the device
+    const int64_t* dnn = gm ? t->gstate.as<int64_t>() + GS_NN : nullptr;
     if (bunched)
-        k_mark_bunches<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nn, S,
-                                                           t->gflag.as<int>());
+        k_mark_bunches<<<grid_for(nnb, 256), 256, 0, s>>>(node_view(t), nnb, S,
+                                                         t->gflag.as<int>(), dnn);
     else
         k_mark_groups<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nparent.as<int>(),
                                                            t->nn, t->group_size, t->gflag.as<int>());
@@ -1229,6 +1308,24 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), nb, t->scanpart, s))) return rc;
     k_flag_write<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>(),
                                     t->groups.as<int>());
+    if (gm) {
+        // super-groups -> bunches with the counts on the device: the super-group
+        // count bounded by n, the bunch table by n + 1
+        int* sg = t->groups.as<int>();
+        const uint32_t* dnsg = t->hist.as<uint32_t>() + nb;
+        if ((rc = t->gflag.ensure(((size_t)n + 1) * sizeof(int))) ||
+            (rc = t->bstart.ensure(((size_t)n + 1) * sizeof(int))))
+            return rc;
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(t->gflag.as<int>());
+        k_graph_set_end<<<1, 64, 0, s>>>(sg, dnsg, (int)n);
+        k_bunch_count<<<grid_for(n, 256), 256, 0, s>>>(sg, 0, B, cnt, dnsg, (int)n);
+        if ((rc = scan_excl_buf(cnt, n, t->scanpart, s))) return rc;
+        k_bunch_emit<<<grid_for(n, 256), 256, 0, s>>>(sg, 0, B, cnt, t->bstart.as<int>(), dnsg);
+        k_graph_set_end<<<1, 64, 0, s>>>(t->bstart.as<int>(), cnt + n, (int)n);
+        k_graph_bunches<<<1, 64, 0, s>>>(cnt + n, t->graph_nbun, t->gstate.as<int64_t>());
+        HQ_CHECK_HIP(hipGetLastError());
+        return HQ_OK;
+    }
     uint32_t total = 0;
     HQ_CHECK_HIP(hipMemcpyAsync(&total, t->hist.as<uint32_t>() + nb, sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s));
@@ -1242,13 +1339,14 @@ int tree_make_groups(hq_tree* t, bool bunched) {
         HQ_CHECK_HIP(hipMemcpyAsync(sg + nsg, &end, sizeof(int), hipMemcpyHostToDevice, s));
         if ((rc = t->gflag.ensure(((size_t)2 * nsg + 2 + n) * sizeof(int)))) return rc;
         uint32_t* cnt = reinterpret_cast<uint32_t*>(t->gflag.as<int>());
-        k_bunch_count<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt);
+        k_bunch_count<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt, nullptr, 0);
         if ((rc = scan_excl_buf(cnt, nsg, t->scanpart, s))) return rc;
         uint32_t nbun = 0;
         HQ_CHECK_HIP(hipMemcpyAsync(&nbun, cnt + nsg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HQ_CHECK_HIP(hipStreamSynchronize(s));
         if ((rc = t->bstart.ensure(((size_t)nbun + 1) * sizeof(int)))) return rc;
-        k_bunch_emit<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt, t->bstart.as<int>());
+        k_bunch_emit<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt, t->bstart.as<int>(),
+                                                        nullptr);
         HQ_CHECK_HIP(hipMemcpyAsync(t->bstart.as<int>() + nbun, &end, sizeof(int),
                                     hipMemcpyHostToDevice, s));
         HQ_CHECK_HIP(hipStreamSynchronize(s));

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from hierarquicos_amd import OctreeType, ics
+from hierarquicos_amd import HQError, OctreeType, ics
 from hierarquicos_amd import integrate as I
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +61,49 @@ def test_direct_graph_replay_matches_launched_steps(scheme):
         assert np.array_equal(qg, qe) and np.array_equal(pg, pe), (dt, nsteps)
         assert tg.energy(1.0, 1e-4).shape == (3, 7)   # launched work between graph runs
         q, p = qg, pg
+
+
+@pytest.mark.parametrize("scheme", ["verlet", "euler"])
+def test_bunched_graph_replay_matches_launched_steps(scheme):
+    """Bunched mixed-precision integration replays one captured step (node and
+    bunch counts on the device); launched steps give the same state bit for
+    bit, over several graph runs of a batch handle with launched work between
+    them.  Afterwards the handle describes the last step's tree."""
+    sizes = [3000, 700, 5000]
+    sys_ = [_plummer(k, 40 + i) for i, k in enumerate(sizes)]
+    m = np.concatenate([s[0] for s in sys_])
+    q = np.concatenate([s[1] + 15.0 * i for i, s in enumerate(sys_)])
+    p = np.concatenate([s[2] for s in sys_])
+    tg = OctreeType().pre_init(m, 11, counts=sizes, precision="mixed").tune(leaf=4)
+    te = OctreeType().pre_init(m, 11, counts=sizes, precision="mixed").tune(leaf=4, graph=False)
+    for dt, nsteps in ((1e-3, 6), (2e-3, 3), (5e-4, 9)):
+        tg.set_state(q, p).integrate(dt, nsteps, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
+        te.set_state(q, p).integrate(dt, nsteps, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
+        (qg, pg), (qe, pe) = tg.get_state(), te.get_state()
+        assert np.array_equal(qg, qe) and np.array_equal(pg, pe), (dt, nsteps)
+        assert tg.info()["nnodes"] == te.info()["nnodes"]
+        assert tg.energy(1.0, 1e-4).shape == (3, 7)   # launched work between graph runs
+        q, p = qg, pg
+
+
+def test_bunched_graph_error_reported_like_launched_steps():
+    """An error inside the replayed steps (two bodies meeting exactly at step
+    3, so lost at depth 25) is flagged on the device; the steps are then rerun
+    launched and return the launched path's error."""
+    n = 2048
+    m, q, v = ics.plummer(n, seed=9, velocities=True)
+    m = np.full(n, 1.0 / n)
+    q[0], q[1] = (-0.375, 0.1, 0.1), (0.375, 0.1, 0.1)
+    v[0], v[1] = (1.0, 0.0, 0.0), (-1.0, 0.0, 0.0)
+    p = v * m[:, None]
+    codes = []
+    for graph in (True, False):
+        t = OctreeType().pre_init(m, 11, precision="mixed").tune(graph=graph)
+        t.set_state(q, p)
+        with pytest.raises(HQError) as e:
+            t.integrate(0.125, 6, 0.25, 0.0, 0.0, scheme="verlet", force="bunched")
+        codes.append(e.value.code)
+    assert codes[0] == codes[1] == 3, codes
 
 
 @pytest.mark.parametrize("method,scheme", [(10, "verlet"), (11, "verlet"), (12, "euler")])
