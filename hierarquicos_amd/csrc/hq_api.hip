@@ -183,6 +183,9 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
         t->fmm_leaf = (int)value;
     } else if (key == HQ_TUNE_GRAPH) {
         t->graph = value != 0;
+    } else if (key == HQ_TUNE_SPLIT) {
+        if (value < 0 || value > 2) return fail(HQ_ERR_ARGS, "split mode %lld outside 0..2", (long long)value);
+        t->split = (int)value;
     } else if (key == HQ_TUNE_LEAF) {
         if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
         t->leaf = (int)value;
@@ -317,6 +320,7 @@ int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {
     info->max_cells = t->max_cells;
     info->max_parts = t->max_parts;
     info->max_stack = t->max_stack;
+    info->split_walks = t->split_walks;
     return HQ_OK;
 }
 

@@ -54,6 +54,11 @@ struct hq_tree {
     int fmm_leaf = 32;     // FMM: cells of <= fmm_leaf bodies are leaves (1..64)
     std::unique_ptr<hq::FmmState, hq::FmmFree> fmm;
     bool fused = true;     // mixed bunched walk as one fused kernel
+    int split = 1;         // split evaluation of the fused walk: 0 never, 1 few bunches, 2 always
+    int64_t split_lcap = 0;  // log entries per bunch (from the last walk's lists; 0 unknown)
+    int64_t split_walks = 0; // walks evaluated split (diagnostic)
+    bool graph_split = false;
+    hq::DevBuf slog, sdesc, snbat, spart;
     int64_t ngroups = 0;
     // walk scratch
     hq::DevBuf fx, fy, fz, counts, clist, plist, lstat;

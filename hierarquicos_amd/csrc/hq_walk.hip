@@ -701,10 +701,10 @@ __global__ __launch_bounds__(64) void k_eval_mixed(const double* __restrict__ sx
 // latency-bound walk of some wavefronts overlaps the arithmetic of others.
 constexpr int RING = 128;
 
-template <int MP>
+template <int MP, int MASK = RING - 1>
 __device__ __forceinline__ void load_rec(const float4* __restrict__ ring, int pos,
                                          const float* __restrict__ mom, float4& e, float r[REC]) {
-    e = ring[pos & (RING - 1)];
+    e = ring[pos & MASK];
     const int node = __builtin_amdgcn_readfirstlane(__float_as_int(e.w));
     const float4* __restrict__ rp = reinterpret_cast<const float4*>(mom + (int64_t)node * REC);
     constexpr int NQ = MP == 1 ? 1 : (MP == 4 ? 2 : REC / 4);
@@ -718,34 +718,78 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ ring, int po
     }
 }
 
+// One batch of <= 64 cells (entries head .. head+cnt-1 of a ring, or of a
+// batch log with MASK = -1) into acc, which starts at zero.
+template <int MP, int MASK = RING - 1>
+__device__ __forceinline__ void eval_cells_acc(const float4* __restrict__ ring, int head, int cnt,
+                                               const float* __restrict__ mom, float tx, float ty,
+                                               float tz, float eps2, float acc[3]) {
+    float4 eA, eB;
+    float rA[REC], rB[REC];
+    load_rec<MP, MASK>(ring, head, mom, eA, rA);
+    for (int j = 0; j < cnt; j += 2) {
+        const bool two = j + 1 < cnt;
+        if (two) load_rec<MP, MASK>(ring, head + j + 1, mom, eB, rB);
+        m2p32<MP>(eA, rA, tx, ty, tz, eps2, acc);
+        if (two) {
+            if (j + 2 < cnt) load_rec<MP, MASK>(ring, head + j + 2, mom, eA, rA);
+            m2p32<MP>(eB, rB, tx, ty, tz, eps2, acc);
+        }
+    }
+}
+
+template <int MASK = RING - 1>
+__device__ __forceinline__ void eval_parts_acc(const float4* __restrict__ ring, int head, int cnt,
+                                               float tx, float ty, float tz, float eps2,
+                                               float acc[3]) {
+    for (int j = 0; j < cnt; ++j) p2p32(ring[(head + j) & MASK], tx, ty, tz, eps2, acc);
+}
+
 template <int MP>
 __device__ __forceinline__ void eval_cells(const float4* __restrict__ ring, int head, int cnt,
                                            const float* __restrict__ mom, float tx, float ty,
                                            float tz, float eps2, Kahan3& sum) {
     float acc[3] = {0.f, 0.f, 0.f};
-    float4 eA, eB;
-    float rA[REC], rB[REC];
-    load_rec<MP>(ring, head, mom, eA, rA);
-    for (int j = 0; j < cnt; j += 2) {
-        const bool two = j + 1 < cnt;
-        if (two) load_rec<MP>(ring, head + j + 1, mom, eB, rB);
-        m2p32<MP>(eA, rA, tx, ty, tz, eps2, acc);
-        if (two) {
-            if (j + 2 < cnt) load_rec<MP>(ring, head + j + 2, mom, eA, rA);
-            m2p32<MP>(eB, rB, tx, ty, tz, eps2, acc);
-        }
-    }
+    eval_cells_acc<MP>(ring, head, cnt, mom, tx, ty, tz, eps2, acc);
     sum.add(acc);
 }
 
 __device__ __forceinline__ void eval_parts(const float4* __restrict__ ring, int head, int cnt,
                                            float tx, float ty, float tz, float eps2, Kahan3& sum) {
     float acc[3] = {0.f, 0.f, 0.f};
-    for (int j = 0; j < cnt; ++j) p2p32(ring[(head + j) & (RING - 1)], tx, ty, tz, eps2, acc);
+    eval_parts_acc(ring, head, cnt, tx, ty, tz, eps2, acc);
     sum.add(acc);
 }
 
-template <int MP, int SC>
+// Split evaluation of the mixed walk (for few bunches): the walk records each
+// bunch's batches, in the order the fused kernel evaluates them, into a log;
+// one wavefront per batch computes its partial sum; one wavefront per bunch
+// adds the partials in order with the same compensated sum.  Every float
+// operation is the fused kernel's, in its order, so the forces are the same
+// bit for bit.
+struct SplitOut {
+    float4* log;    // per bunch, lcap entries: the batches' cells and bodies in order
+    int2* desc;     // per bunch, bcap batches: x first entry, y count | kind << 16 (0 cells, 1 bodies)
+    int* nbat;      // batches per bunch (-1: its log overflowed)
+    int64_t lcap;
+    int bcap;
+};
+
+__device__ __forceinline__ void record_batch(const SplitOut& so, int gi, int kind,
+                                             const float4* __restrict__ ring, int head, int cnt,
+                                             int& lpos, int& nbat, bool& over, int lane) {
+    if (over || nbat >= so.bcap || lpos + cnt > so.lcap) {
+        over = true;
+        return;
+    }
+    float4* __restrict__ dst = so.log + (int64_t)gi * so.lcap + lpos;
+    for (int j = lane; j < cnt; j += 64) dst[j] = ring[(head + j) & (RING - 1)];
+    if (lane == 0) so.desc[(int64_t)gi * so.bcap + nbat] = make_int2(lpos, cnt | (kind << 16));
+    lpos += cnt;
+    ++nbat;
+}
+
+template <int MP, int SC, bool RECORD = false>
 __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __restrict__ g,
                                                    const double* __restrict__ sx,
                                                    const double* __restrict__ sy,
@@ -757,7 +801,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz,
                                                    int64_t* __restrict__ status,
-                                                   const int64_t* __restrict__ dnbun) {
+                                                   const int64_t* __restrict__ dnbun,
+                                                   SplitOut so) {
     __shared__ int stk[SC];
     __shared__ float4 cring[RING], pring[RING];
     __shared__ float4 own[BUNCH];
@@ -793,6 +838,16 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
     int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0;
     long long ntot_c = 0, ntot_p = 0;
     bool overflow = false;
+    int lpos = 0, nbat = 0;  // record mode: log position and batches
+    bool rover = false;      // record mode: the log overflowed
+    auto batch_cells = [&](int head, int cnt) {
+        if constexpr (RECORD) record_batch(so, gi, 0, cring, head, cnt, lpos, nbat, rover, lane);
+        else eval_cells<MP>(cring, head, cnt, mom, tx, ty, tz, eps2, sum);
+    };
+    auto batch_parts = [&](int head, int cnt) {
+        if constexpr (RECORD) record_batch(so, gi, 1, pring, head, cnt, lpos, nbat, rover, lane);
+        else eval_parts(pring, head, cnt, tx, ty, tz, eps2, sum);
+    };
     const int sys = nv.sys[gf];
     const double* __restrict__ gl2 = g + SG * sys + S_L2;
     if (lane == 0) stk[0] = sys;
@@ -849,7 +904,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                 mb &= mb - 1;
                 const int f = __shfl(bfirst, j, 64), c = __shfl(bcount, j, 64);
                 while (pcnt >= 64) {
-                    eval_parts(pring, phead, 64, tx, ty, tz, eps2, sum);
+                    batch_parts(phead, 64);
                     phead += 64;
                     pcnt -= 64;
                     ntot_p += 64;
@@ -884,39 +939,47 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         maxtop = top > maxtop ? top : maxtop;
         __builtin_amdgcn_wave_barrier();
         if (ccnt >= 64) {
-            eval_cells<MP>(cring, chead, 64, mom, tx, ty, tz, eps2, sum);
+            batch_cells(chead, 64);
             chead += 64;
             ccnt -= 64;
             ntot_c += 64;
         }
         if (pcnt >= 64) {
-            eval_parts(pring, phead, 64, tx, ty, tz, eps2, sum);
+            batch_parts(phead, 64);
             phead += 64;
             pcnt -= 64;
             ntot_p += 64;
         }
     }
     if (!overflow) {
-        if (ccnt > 0) eval_cells<MP>(cring, chead, ccnt, mom, tx, ty, tz, eps2, sum);
-        if (pcnt > 0) eval_parts(pring, phead, pcnt, tx, ty, tz, eps2, sum);
+        if (ccnt > 0) batch_cells(chead, ccnt);
+        if (pcnt > 0) batch_parts(phead, pcnt);
         ntot_c += ccnt;
         ntot_p += pcnt;
-        float acc[3] = {0.f, 0.f, 0.f};
-        for (int j = 0; j < nb; ++j) {
-            float4 e = own[j];
-            if (j == lane) e.w = 0.0f, e.x = tx + 1.0f;  // zero mass, non-zero distance
-            p2p32(e, tx, ty, tz, eps2, acc);
+        if constexpr (!RECORD) {
+            float acc[3] = {0.f, 0.f, 0.f};
+            for (int j = 0; j < nb; ++j) {
+                float4 e = own[j];
+                if (j == lane) e.w = 0.0f, e.x = tx + 1.0f;  // zero mass, non-zero distance
+                p2p32(e, tx, ty, tz, eps2, acc);
+            }
+            sum.add(acc);
+            if (lane < nb) {
+                const double gm = G * sm[i];
+                fx[i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
+                fy[i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
+                fz[i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
+            }
         }
-        sum.add(acc);
-        if (lane < nb) {
-            const double gm = G * sm[i];
-            fx[i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
-            fy[i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
-            fz[i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
-        }
+    }
+    if constexpr (RECORD) {
+        // every bunch's count is written (-1 on any overflow), so the later
+        // kernels never read batches this launch did not record
+        if (lane == 0) so.nbat[gi] = (rover || overflow) ? -1 : nbat;
     }
     if (lane == 0) {
         if (overflow) atomicOr((unsigned long long*)&status[ST_FLAGS], 16ull);
+        if (rover) atomicOr((unsigned long long*)&status[ST_FLAGS], 32ull);
         const unsigned long long unb = (unsigned long long)nb;
         atomicAdd((unsigned long long*)&status[ST_LISTC], (unsigned long long)ntot_c);
         atomicAdd((unsigned long long*)&status[ST_LISTP], (unsigned long long)ntot_p);
@@ -925,6 +988,103 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         atomicMax((long long*)&status[ST_NEEDC], (long long)ntot_c);
         atomicMax((long long*)&status[ST_NEEDP], (long long)ntot_p);
         atomicMax((long long*)&status[ST_MAXSTK], (long long)maxtop);
+    }
+}
+
+// Target coordinates of bunch gi, relative to the centre of its bounding box,
+// exactly as the walk computes them.
+struct BunchTargets {
+    int i, nb;
+    float tx, ty, tz;
+};
+
+__device__ __forceinline__ BunchTargets bunch_targets(const double* __restrict__ sx,
+                                                      const double* __restrict__ sy,
+                                                      const double* __restrict__ sz,
+                                                      const int* __restrict__ bstart, int gi,
+                                                      int lane) {
+    BunchTargets b;
+    const int gf = bstart[gi], gl = bstart[gi + 1] - 1;
+    b.nb = gl - gf + 1;
+    b.i = gf + (lane < b.nb ? lane : 0);
+    double v[3] = {sx[b.i], sy[b.i], sz[b.i]}, w[3] = {v[0], v[1], v[2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            v[c] = fmin(v[c], __shfl_xor(v[c], o, 64));
+            w[c] = fmax(w[c], __shfl_xor(w[c], o, 64));
+        }
+    }
+    const double bc[3] = {0.5 * (v[0] + w[0]), 0.5 * (v[1] + w[1]), 0.5 * (v[2] + w[2])};
+    b.tx = (float)(sx[b.i] - bc[0]);
+    b.ty = (float)(sy[b.i] - bc[1]);
+    b.tz = (float)(sz[b.i] - bc[2]);
+    return b;
+}
+
+// Split evaluation, step 2: block (gi, k) evaluates batch k of bunch gi.
+template <int MP>
+__global__ __launch_bounds__(64) void k_eval_batches(const double* __restrict__ sx,
+                                                     const double* __restrict__ sy,
+                                                     const double* __restrict__ sz,
+                                                     const int* __restrict__ bstart, int nbunch,
+                                                     const int64_t* __restrict__ dnbun,
+                                                     SplitOut so, const float* __restrict__ mom,
+                                                     float eps2, float4* __restrict__ part) {
+    if (dnbun) nbunch = (int)*dnbun;
+    const int gi = (int)(blockIdx.x / so.bcap), k = (int)(blockIdx.x % so.bcap);
+    if (gi >= nbunch || k >= so.nbat[gi]) return;
+    const int lane = __lane_id();
+    const BunchTargets b = bunch_targets(sx, sy, sz, bstart, gi, lane);
+    const int2 d = so.desc[(int64_t)gi * so.bcap + k];
+    const int first = d.x, cnt = d.y & 0xFFFF, kind = d.y >> 16;
+    const float4* __restrict__ lg = so.log + (int64_t)gi * so.lcap;
+    float acc[3] = {0.f, 0.f, 0.f};
+    if (kind == 0) eval_cells_acc<MP, -1>(lg, first, cnt, mom, b.tx, b.ty, b.tz, eps2, acc);
+    else eval_parts_acc<-1>(lg, first, cnt, b.tx, b.ty, b.tz, eps2, acc);
+    part[((int64_t)gi * so.bcap + k) * 64 + lane] = make_float4(acc[0], acc[1], acc[2], 0.0f);
+}
+
+// Split evaluation, step 3: the partials of bunch gi in batch order, then the
+// bunch's own bodies, in the walk kernel's compensated sum.
+__global__ __launch_bounds__(64) void k_combine(const double* __restrict__ sx,
+                                                const double* __restrict__ sy,
+                                                const double* __restrict__ sz,
+                                                const double* __restrict__ sm,
+                                                const int* __restrict__ bstart, int nbunch,
+                                                const int64_t* __restrict__ dnbun, SplitOut so,
+                                                const float4* __restrict__ part, double G,
+                                                float eps2, double* __restrict__ fx,
+                                                double* __restrict__ fy, double* __restrict__ fz) {
+    __shared__ float4 own[BUNCH];
+    if (dnbun) nbunch = (int)*dnbun;
+    const int gi = blockIdx.x;
+    if (gi >= nbunch) return;
+    const int lane = __lane_id();
+    const BunchTargets b = bunch_targets(sx, sy, sz, bstart, gi, lane);
+    const float tx = b.tx, ty = b.ty, tz = b.tz;
+    own[lane] = make_float4(tx, ty, tz, lane < b.nb ? (float)sm[b.i] : 0.0f);
+    __syncthreads();
+    Kahan3 sum;
+    const int nbat = so.nbat[gi];
+    for (int k = 0; k < nbat; ++k) {
+        const float4 a = part[((int64_t)gi * so.bcap + k) * 64 + lane];
+        float acc[3] = {a.x, a.y, a.z};
+        sum.add(acc);
+    }
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int j = 0; j < b.nb; ++j) {
+        float4 e = own[j];
+        if (j == lane) e.w = 0.0f, e.x = tx + 1.0f;  // zero mass, non-zero distance
+        p2p32(e, tx, ty, tz, eps2, acc);
+    }
+    sum.add(acc);
+    if (lane < b.nb) {
+        const double gm = G * sm[b.i];
+        fx[b.i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
+        fy[b.i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
+        fz[b.i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
     }
 }
 
@@ -1094,12 +1254,76 @@ __global__ void k_graph_bunches(const uint32_t* __restrict__ nb, int64_t grid,
     if ((int64_t)*nb > grid) gs[GS_STICKY] |= GF_BUNCHES;
 }
 
-// Graph mode: the walk's stack overflow, kept over replayed steps.
+// Graph mode: the walk's stack (or split log) overflow, kept over replayed steps.
 __global__ void k_graph_walk_check(const int64_t* __restrict__ st, int64_t* __restrict__ gs) {
-    if (threadIdx.x == 0 && (st[ST_FLAGS] & 16)) gs[GS_STICKY] |= GF_STACK;
+    if (threadIdx.x == 0 && (st[ST_FLAGS] & (16 | 32))) gs[GS_STICKY] |= GF_STACK;
 }
 
 int tree_make_groups(hq_tree* t, bool bunched);
+
+// Split evaluation (record, per-batch partials, combine) is used for few
+// bunches, once a walk has measured the list lengths (split_lcap).
+constexpr int SPLIT_MAX = 2048;                      // bunches (auto mode)
+constexpr double SPLIT_MEM_MAX = 1024.0 * 1024 * 1024;  // bytes of log and partials
+
+static int split_bcap(int64_t lcap) { return (int)(lcap / 64 + 3); }
+
+static bool use_split(const hq_tree* t, int64_t nbunch) {
+    if (t->split == 0 || t->split_lcap <= 0) return false;
+    if (t->split == 1 && nbunch > SPLIT_MAX) return false;
+    const double bcap = split_bcap(t->split_lcap);
+    const double bytes = (double)nbunch * (t->split_lcap * 16.0 + bcap * (8.0 + 64 * 16.0) + 4.0);
+    return bytes <= SPLIT_MEM_MAX;
+}
+
+static void note_lists(hq_tree* t) {  // log size for the next split walk, from this walk's lists
+    const int64_t need = t->hstatus[ST_NEEDC] + t->hstatus[ST_NEEDP];
+    t->split_lcap = (need * 5 / 4 + 128 + 63) / 64 * 64;
+}
+
+static int split_buffers(hq_tree* t, int64_t nbunch) {
+    int rc;
+    const int64_t lcap = t->split_lcap, bcap = split_bcap(lcap);
+    if ((rc = t->slog.ensure((size_t)nbunch * lcap * sizeof(float4))) ||
+        (rc = t->sdesc.ensure((size_t)nbunch * bcap * sizeof(int2))) ||
+        (rc = t->snbat.ensure((size_t)nbunch * sizeof(int))) ||
+        (rc = t->spart.ensure((size_t)nbunch * bcap * 64 * sizeof(float4))))
+        return rc;
+    return HQ_OK;
+}
+
+// The three launches of a split mixed walk over grid bunches (the count on the
+// device when dnbun is given).
+static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta2, double G,
+                        double eps2) {
+    int rc;
+    if ((rc = split_buffers(t, grid))) return rc;
+    hipStream_t s = t->stream;
+    const SplitOut so{t->slog.as<float4>(), t->sdesc.as<int2>(), t->snbat.as<int>(),
+                      t->split_lcap, split_bcap(t->split_lcap)};
+    auto record = [&](auto kern) {
+        kern<<<grid, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
+                                 t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
+                                 t->bstart.as<int>(), grid, theta2, G, (float)eps2, t->leaf,
+                                 t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
+                                 t->fz.as<double>(), t->status.as<int64_t>(), dnbun, so);
+    };
+    auto eval = [&](auto kern) {
+        kern<<<grid * so.bcap, 64, 0, s>>>(t->sx.as<double>(), t->sy.as<double>(),
+                                           t->sz.as<double>(), t->bstart.as<int>(), grid, dnbun,
+                                           so, t->mom32.as<float>(), (float)eps2,
+                                           t->spart.as<float4>());
+    };
+    if (t->multipole == 1) record(k_walk_mixed<1, SCAP_FUSED, true>), eval(k_eval_batches<1>);
+    else if (t->multipole == 4) record(k_walk_mixed<4, SCAP_FUSED, true>), eval(k_eval_batches<4>);
+    else record(k_walk_mixed<8, SCAP_FUSED, true>), eval(k_eval_batches<8>);
+    k_combine<<<grid, 64, 0, s>>>(t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(),
+                                  t->sm.as<double>(), t->bstart.as<int>(), grid, dnbun, so,
+                                  t->spart.as<float4>(), G, (float)eps2, t->fx.as<double>(),
+                                  t->fy.as<double>(), t->fz.as<double>());
+    HQ_CHECK_HIP(hipGetLastError());
+    return HQ_OK;
+}
 
 // Buffers at the sizes graph mode uses (upper bounds), so that nothing grows
 // while a step is captured.
@@ -1114,6 +1338,8 @@ int graph_prepare(hq_tree* t) {
         (rc = t->mom32.ensure((size_t)t->cap * REC * sizeof(float))) ||
         (rc = t->scanpart.ensure(scan_part_size(2 * n + 65536) * sizeof(uint32_t))))
         return rc;
+    t->graph_split = use_split(t, t->graph_nbun);
+    if (t->graph_split && (rc = split_buffers(t, t->graph_nbun))) return rc;
     return HQ_OK;
 }
 
@@ -1135,17 +1361,21 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         k_mom32<<<grid_for(t->cap, 256), 256, 0, s>>>(t->cap, t->multipole, node_view(t),
                                                       t->mom32.as<float>(), gs + GS_NN);
         k_fill_i64<<<1, 64, 0, s>>>(t->status.as<int64_t>(), 0, ST_SIZE);
-        auto launch = [&](auto kern) {
-            kern<<<(int)t->graph_nbun, 64, 0, s>>>(
-                node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(), t->sy.as<double>(),
-                t->sz.as<double>(), t->sm.as<double>(), t->n, t->bstart.as<int>(),
-                (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf, t->mom32.as<float>(),
-                t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>(),
-                t->status.as<int64_t>(), gs + GS_NBUN);
-        };
-        if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
-        else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
-        else launch(k_walk_mixed<8, SCAP_FUSED>);
+        if (t->graph_split) {
+            if ((rc = launch_split(t, (int)t->graph_nbun, gs + GS_NBUN, theta2, G, eps2))) return rc;
+        } else {
+            auto launch = [&](auto kern) {
+                kern<<<(int)t->graph_nbun, 64, 0, s>>>(
+                    node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
+                    t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
+                    t->bstart.as<int>(), (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf,
+                    t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
+                    t->fz.as<double>(), t->status.as<int64_t>(), gs + GS_NBUN, SplitOut{});
+            };
+            if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
+            else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
+            else launch(k_walk_mixed<8, SCAP_FUSED>);
+        }
         k_graph_walk_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), gs);
         HQ_CHECK_HIP(hipGetLastError());
         return HQ_OK;
@@ -1154,6 +1384,31 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
         k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
                                                      t->mom32.as<float>(), nullptr);
+        if (t->fused && use_split(t, ngroups)) {
+            HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+            HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
+            if ((rc = launch_split(t, ngroups, nullptr, theta2, G, eps2))) return rc;
+            HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
+            if ((rc = read_status(t))) return rc;
+            HQ_CHECK_HIP(hipGetLastError());
+            const int64_t fl = t->hstatus[ST_FLAGS];
+            note_lists(t);
+            if (!(fl & (16 | 32))) {
+                ++t->split_walks;
+                hipEventElapsedTime(&t->ms_eval, t->ev[5], t->ev[8]);
+                t->ms_lists = 0.0f;
+                t->ms_walk = t->ms_eval;
+                t->list_cells = t->hstatus[ST_LISTC];
+                t->list_parts = t->hstatus[ST_LISTP];
+                t->exec_cells = t->hstatus[ST_EXECC];
+                t->exec_parts = t->hstatus[ST_EXECP];
+                t->max_cells = t->hstatus[ST_NEEDC];
+                t->max_parts = t->hstatus[ST_NEEDP];
+                t->max_stack = t->hstatus[ST_MAXSTK];
+                return HQ_OK;
+            }
+            // the log or the stack overflowed: the fused walk below
+        }
         if (t->fused) {
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
@@ -1163,7 +1418,8 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
                                             theta2, G, (float)eps2, t->leaf, t->mom32.as<float>(),
                                             t->fx.as<double>(), t->fy.as<double>(),
-                                            t->fz.as<double>(), t->status.as<int64_t>(), nullptr);
+                                            t->fz.as<double>(), t->status.as<int64_t>(), nullptr,
+                                            SplitOut{});
             };
             if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
             else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
@@ -1172,6 +1428,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             if ((rc = read_status(t))) return rc;
             HQ_CHECK_HIP(hipGetLastError());
             if (!(t->hstatus[ST_FLAGS] & 16)) {
+                note_lists(t);
                 hipEventElapsedTime(&t->ms_eval, t->ev[5], t->ev[8]);
                 t->ms_lists = 0.0f;
                 t->ms_walk = t->ms_eval;

@@ -145,12 +145,16 @@ class OctreeType:
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
     def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None,
-             graph=None):
+             graph=None, split=None):
         """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
         bucket (cells of <= leaf bodies enter as their bodies) of the bunched
-        walk; FMM leaf size; graph: whether integrate() with direct forces
-        replays one captured step (default True)."""
+        walk; FMM leaf size; graph: whether integrate() replays one captured
+        step (default True); split: 0 never, 1 (default) with few bunches, 2
+        always -- the fused walk's batches evaluated one wavefront each (same
+        results)."""
         lib = L.lib()
+        if split is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_SPLIT, int(split)))
         if graph is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_GRAPH, int(bool(graph))))
         if fmm_leaf is not None:

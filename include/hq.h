@@ -116,6 +116,10 @@ int hq_tree_destroy(hq_tree *t);
                               HQ_FORCE_BUNCHED at HQ_MIXED in the fused walk, captures one step
                               in a HIP graph and replays it (same results); 0: launches every
                               step */
+#define HQ_TUNE_SPLIT 7    /* fused mixed walk: 1 (default) with few bunches (<= 2048) records
+                              each bunch's batches and evaluates them one wavefront per batch,
+                              then sums them in order (same results as the fused kernel, more
+                              parallel); 0 never; 2 whenever the logs fit */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
 /* Cartesian FMM of order 4 or 6 on the built tree: Dehnen's method carried to
@@ -232,6 +236,7 @@ typedef struct {
     int64_t max_cells;     /* longest cell list of the last grouped walk */
     int64_t max_parts;     /* longest particle list */
     int64_t max_stack;     /* deepest walk stack */
+    int64_t split_walks;   /* mixed walks of this handle evaluated split (HQ_TUNE_SPLIT) */
 } hq_tree_info_t;
 int hq_tree_info(const hq_tree *t, hq_tree_info_t *info);
 

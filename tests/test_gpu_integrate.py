@@ -66,16 +66,19 @@ def test_direct_graph_replay_matches_launched_steps(scheme):
 @pytest.mark.parametrize("scheme", ["verlet", "euler"])
 def test_bunched_graph_replay_matches_launched_steps(scheme):
     """Bunched mixed-precision integration replays one captured step (node and
-    bunch counts on the device); launched steps give the same state bit for
-    bit, over several graph runs of a batch handle with launched work between
-    them.  Afterwards the handle describes the last step's tree."""
+    bunch counts on the device, split walk); launched steps with the fused walk
+    give the same state bit for bit, over several graph runs of a batch handle
+    with launched work between them.  Afterwards the handle describes the last
+    step's tree."""
     sizes = [3000, 700, 5000]
     sys_ = [_plummer(k, 40 + i) for i, k in enumerate(sizes)]
     m = np.concatenate([s[0] for s in sys_])
     q = np.concatenate([s[1] + 15.0 * i for i, s in enumerate(sys_)])
     p = np.concatenate([s[2] for s in sys_])
+    # graph replays with the split walk (few bunches) against launched fused walks
     tg = OctreeType().pre_init(m, 11, counts=sizes, precision="mixed").tune(leaf=4)
-    te = OctreeType().pre_init(m, 11, counts=sizes, precision="mixed").tune(leaf=4, graph=False)
+    te = OctreeType().pre_init(m, 11, counts=sizes, precision="mixed").tune(leaf=4, graph=False,
+                                                                            split=0)
     for dt, nsteps in ((1e-3, 6), (2e-3, 3), (5e-4, 9)):
         tg.set_state(q, p).integrate(dt, nsteps, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
         te.set_state(q, p).integrate(dt, nsteps, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")

@@ -70,3 +70,21 @@ def test_autotune_picks_a_valid_setting():
     Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
     F = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
     assert O.rel_errors(F[sample], Fd)[0] < 1e-4
+
+
+@pytest.mark.parametrize("method,leaf", [(10, 0), (11, 4), (12, 2)])
+def test_split_walk_bitidentical_to_fused(method, leaf):
+    """The split evaluation (walk records its batches; one wavefront per batch;
+    partials summed in order) gives the fused kernel's forces bit for bit."""
+    m, q, _ = ics.plummer(20000, seed=5)
+    x, y, z = (np.ascontiguousarray(q[:, c]) for c in range(3))
+    ta = OctreeType().pre_init(m, method, precision="mixed").tune(leaf=leaf, split=0)
+    tb = OctreeType().pre_init(m, method, precision="mixed").tune(leaf=leaf, split=2)
+    ta.init(x, y, z)
+    tb.init(x, y, z)
+    Fa = ta.forces_bunched(0.25, 1.0, 1e-4, "mixed")
+    Fb1 = tb.forces_bunched(0.25, 1.0, 1e-4, "mixed")   # measures the lists (fused)
+    Fb2 = tb.forces_bunched(0.25, 1.0, 1e-4, "mixed")   # split
+    assert ta.info()["split_walks"] == 0 and tb.info()["split_walks"] == 1
+    assert np.array_equal(Fa, Fb1) and np.array_equal(Fa, Fb2)
+    assert tb.info()["exec_cells"] == ta.info()["exec_cells"]
