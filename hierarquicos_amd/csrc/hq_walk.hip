@@ -1240,8 +1240,8 @@ __global__ __launch_bounds__(256) void k_flag_write(const int* __restrict__ gmar
     if (f) out[pre + __popcll(m & lanemask_lt())] = gmark[i];
 }
 
-// Graph mode: a[*pos] = v (the end marker after a table whose length is on the device)
-__global__ void k_graph_set_end(int* __restrict__ a, const uint32_t* __restrict__ pos, int v) {
+// a[*pos] = v: the end marker after a table whose length is on the device
+__global__ void k_set_end(int* __restrict__ a, const uint32_t* __restrict__ pos, int v) {
     if (threadIdx.x == 0) a[*pos] = v;
 }
 
@@ -1565,22 +1565,31 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), nb, t->scanpart, s))) return rc;
     k_flag_write<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>(),
                                     t->groups.as<int>());
-    if (gm) {
+    if (bunched) {
         // super-groups -> bunches with the counts on the device: the super-group
-        // count bounded by n, the bunch table by n + 1
+        // count bounded by n, the bunch table by n + 1 (one read-back at the
+        // end when launched, none in graph mode)
         int* sg = t->groups.as<int>();
         const uint32_t* dnsg = t->hist.as<uint32_t>() + nb;
         if ((rc = t->gflag.ensure(((size_t)n + 1) * sizeof(int))) ||
             (rc = t->bstart.ensure(((size_t)n + 1) * sizeof(int))))
             return rc;
         uint32_t* cnt = reinterpret_cast<uint32_t*>(t->gflag.as<int>());
-        k_graph_set_end<<<1, 64, 0, s>>>(sg, dnsg, (int)n);
+        k_set_end<<<1, 64, 0, s>>>(sg, dnsg, (int)n);
         k_bunch_count<<<grid_for(n, 256), 256, 0, s>>>(sg, 0, B, cnt, dnsg, (int)n);
         if ((rc = scan_excl_buf(cnt, n, t->scanpart, s))) return rc;
         k_bunch_emit<<<grid_for(n, 256), 256, 0, s>>>(sg, 0, B, cnt, t->bstart.as<int>(), dnsg);
-        k_graph_set_end<<<1, 64, 0, s>>>(t->bstart.as<int>(), cnt + n, (int)n);
-        k_graph_bunches<<<1, 64, 0, s>>>(cnt + n, t->graph_nbun, t->gstate.as<int64_t>());
+        k_set_end<<<1, 64, 0, s>>>(t->bstart.as<int>(), cnt + n, (int)n);
+        if (gm) {
+            k_graph_bunches<<<1, 64, 0, s>>>(cnt + n, t->graph_nbun, t->gstate.as<int64_t>());
+            HQ_CHECK_HIP(hipGetLastError());
+            return HQ_OK;
+        }
+        uint32_t nbun = 0;
+        HQ_CHECK_HIP(hipMemcpyAsync(&nbun, cnt + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HQ_CHECK_HIP(hipStreamSynchronize(s));
         HQ_CHECK_HIP(hipGetLastError());
+        t->ngroups = nbun;
         return HQ_OK;
     }
     uint32_t total = 0;
@@ -1588,27 +1597,6 @@ int tree_make_groups(hq_tree* t, bool bunched) {
                                 hipMemcpyDeviceToHost, s));
     HQ_CHECK_HIP(hipStreamSynchronize(s));
     t->ngroups = total;
-    if (bunched) {
-        // super-groups -> bunches of <= B particles
-        const int end = (int)n;
-        const int nsg = (int)total;
-        int* sg = t->groups.as<int>();
-        HQ_CHECK_HIP(hipMemcpyAsync(sg + nsg, &end, sizeof(int), hipMemcpyHostToDevice, s));
-        if ((rc = t->gflag.ensure(((size_t)2 * nsg + 2 + n) * sizeof(int)))) return rc;
-        uint32_t* cnt = reinterpret_cast<uint32_t*>(t->gflag.as<int>());
-        k_bunch_count<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt, nullptr, 0);
-        if ((rc = scan_excl_buf(cnt, nsg, t->scanpart, s))) return rc;
-        uint32_t nbun = 0;
-        HQ_CHECK_HIP(hipMemcpyAsync(&nbun, cnt + nsg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HQ_CHECK_HIP(hipStreamSynchronize(s));
-        if ((rc = t->bstart.ensure(((size_t)nbun + 1) * sizeof(int)))) return rc;
-        k_bunch_emit<<<grid_for(nsg, 256), 256, 0, s>>>(sg, nsg, B, cnt, t->bstart.as<int>(),
-                                                        nullptr);
-        HQ_CHECK_HIP(hipMemcpyAsync(t->bstart.as<int>() + nbun, &end, sizeof(int),
-                                    hipMemcpyHostToDevice, s));
-        HQ_CHECK_HIP(hipStreamSynchronize(s));
-        t->ngroups = nbun;
-    }
     return HQ_OK;
 }
 
