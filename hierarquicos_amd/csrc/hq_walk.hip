@@ -804,8 +804,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    const int64_t* __restrict__ dnbun,
                                                    SplitOut so) {
     __shared__ int stk[SC];
-    __shared__ float4 cring[RING], pring[RING];
-    __shared__ float4 own[BUNCH];
+    __shared__ float4 cring[RING], pring[RING];  // pring also holds the bunch's own bodies at the end
     // graph mode: the grid is an upper bound and the count is on the device;
     // the swizzle spreads the blocks that have a bunch over all XCDs
     if (dnbun) nbunch = (int)*dnbun;
@@ -833,7 +832,6 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
     const double bc[3] = {0.5 * (bmin[0] + bmax[0]), 0.5 * (bmin[1] + bmax[1]),
                           0.5 * (bmin[2] + bmax[2])};
     const float tx = (float)(sx[i] - bc[0]), ty = (float)(sy[i] - bc[1]), tz = (float)(sz[i] - bc[2]);
-    own[lane] = make_float4(tx, ty, tz, lane < nb ? (float)sm[i] : 0.0f);
     Kahan3 sum;
     int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0;
     long long ntot_c = 0, ntot_p = 0;
@@ -957,9 +955,13 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         ntot_c += ccnt;
         ntot_p += pcnt;
         if constexpr (!RECORD) {
+            // the rings are drained: the particle ring takes the bunch's bodies
+            __builtin_amdgcn_wave_barrier();
+            pring[lane] = make_float4(tx, ty, tz, lane < nb ? (float)sm[i] : 0.0f);
+            __builtin_amdgcn_wave_barrier();
             float acc[3] = {0.f, 0.f, 0.f};
             for (int j = 0; j < nb; ++j) {
-                float4 e = own[j];
+                float4 e = pring[j];
                 if (j == lane) e.w = 0.0f, e.x = tx + 1.0f;  // zero mass, non-zero distance
                 p2p32(e, tx, ty, tz, eps2, acc);
             }
