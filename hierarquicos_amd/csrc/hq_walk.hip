@@ -719,20 +719,23 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ ring, int po
 }
 
 // One batch of <= 64 cells (entries head .. head+cnt-1 of a ring, or of a
-// batch log with MASK = -1) into acc, which starts at zero.
+// batch log with MASK = -1) into acc, which starts at zero.  Ring batches
+// start at multiples of 64 (heads advance by whole batches), so a batch never
+// wraps: it is read from one base with immediate offsets.
 template <int MP, int MASK = RING - 1>
 __device__ __forceinline__ void eval_cells_acc(const float4* __restrict__ ring, int head, int cnt,
                                                const float* __restrict__ mom, float tx, float ty,
                                                float tz, float eps2, float acc[3]) {
+    const float4* __restrict__ b = ring + (head & MASK);
     float4 eA, eB;
     float rA[REC], rB[REC];
-    load_rec<MP, MASK>(ring, head, mom, eA, rA);
+    load_rec<MP, -1>(b, 0, mom, eA, rA);
     for (int j = 0; j < cnt; j += 2) {
         const bool two = j + 1 < cnt;
-        if (two) load_rec<MP, MASK>(ring, head + j + 1, mom, eB, rB);
+        if (two) load_rec<MP, -1>(b, j + 1, mom, eB, rB);
         m2p32<MP>(eA, rA, tx, ty, tz, eps2, acc);
         if (two) {
-            if (j + 2 < cnt) load_rec<MP, MASK>(ring, head + j + 2, mom, eA, rA);
+            if (j + 2 < cnt) load_rec<MP, -1>(b, j + 2, mom, eA, rA);
             m2p32<MP>(eB, rB, tx, ty, tz, eps2, acc);
         }
     }
@@ -742,7 +745,8 @@ template <int MASK = RING - 1>
 __device__ __forceinline__ void eval_parts_acc(const float4* __restrict__ ring, int head, int cnt,
                                                float tx, float ty, float tz, float eps2,
                                                float acc[3]) {
-    for (int j = 0; j < cnt; ++j) p2p32(ring[(head + j) & MASK], tx, ty, tz, eps2, acc);
+    const float4* __restrict__ b = ring + (head & MASK);
+    for (int j = 0; j < cnt; ++j) p2p32(b[j], tx, ty, tz, eps2, acc);
 }
 
 template <int MP>
