@@ -530,11 +530,6 @@ __global__ void k_levels(const uint32_t* __restrict__ boff, int nblk, int64_t* _
 }
 
 // ------------------------------------------------------------------ moments
-// reference child slot (0-based octant index, :527-531) of a digit 4xb+2yb+zb
-__device__ __forceinline__ int ref_slot(uint32_t dig) {
-    return (int)((dig >> 2) & 1u) + 2 * (int)(1u - ((dig >> 1) & 1u)) + 4 * (int)(1u - (dig & 1u));
-}
-
 // Arguments of the upward pass (node store and sorted particles).
 struct UpArgs {
     int multipole;
@@ -588,31 +583,53 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
             qz = qz / mass;
         }
     } else {
-        int slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-        int cs = ncstart[k], ce = ncstart[k + 1];
-        for (int c = cs; c < ce; ++c) slot[ref_slot(ndigit[c])] = c;
+        // children in the reference's order r = 0..7 (octant index
+        // 1 + x + 2(1-y) + 4(1-z), :527-531, of digit 4x + 2y + z): the present digits as a
+        // mask; the child of digit d is the (number of present digits below
+        // d)-th of the contiguous range.  Every child's record is loaded up
+        // front (absent slots read the first child, unused), so the loads are
+        // in flight together; the sums below keep the reference's order.
+        const int cs = ncstart[k], ce = ncstart[k + 1];
+        unsigned present = 0u;
+        for (int c = cs; c < ce; ++c) present |= 1u << ndigit[c];
+        int cidx[8];
+        double mcv[8], cxv[8], cyv[8], czv[8];
+        int cnv[8];
+#pragma unroll
         for (int r = 0; r < 8; ++r) {
-            int c = slot[r];
-            if (c < 0) continue;
-            double mc = nmass[c];
+            const unsigned d = 4u * (unsigned)(r & 1) + 2u * (1u - ((r >> 1) & 1u)) +
+                               (1u - ((r >> 2) & 1u));
+            cidx[r] = ((present >> d) & 1u) ? cs + __popc(present & ((1u << d) - 1u)) : -1;
+            const int c = cidx[r] >= 0 ? cidx[r] : cs;
+            mcv[r] = nmass[c];
+            cxv[r] = ncomx[c];
+            cyv[r] = ncomy[c];
+            czv[r] = ncomz[c];
+            cnv[r] = ncount[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (cidx[r] < 0) continue;
+            double mc = mcv[r];
             mass = mass + mc;
-            qx = qx + mc * ncomx[c];
-            qy = qy + mc * ncomy[c];
-            qz = qz + mc * ncomz[c];
-            count += ncount[c];
+            qx = qx + mc * cxv[r];
+            qy = qy + mc * cyv[r];
+            qz = qz + mc * czv[r];
+            count += cnv[r];
         }
         qx = qx / mass;
         qy = qy / mass;
         qz = qz / mass;
         if (multipole >= 4) {
             const bool oct = multipole > 4;
+#pragma unroll
             for (int r = 0; r < 8; ++r) {
-                int c = slot[r];
+                const int c = cidx[r];
                 if (c < 0) continue;
-                double pm = nmass[c];
-                double dxi = ncomx[c] - qx;
-                double dyi = ncomy[c] - qy;
-                double dzi = ncomz[c] - qz;
+                double pm = mcv[r];
+                double dxi = cxv[r] - qx;
+                double dyi = cyv[r] - qy;
+                double dzi = czv[r] - qz;
                 bool twig = ntype[c] == 2;
                 double Qc[6];
                 for (int j = 0; j < 6; ++j) Qc[j] = nquad[(int64_t)j * nn + c];
