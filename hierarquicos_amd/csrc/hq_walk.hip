@@ -19,6 +19,12 @@
 //   k_eval_mixed     the same sums in FP32 on recentred coordinates with a
 //                    lean (algebraically identical) multipole evaluation and
 //                    block-compensated (Kahan) accumulation
+//   k_walk_mixed     the mixed walk fused: one wavefront per bunch walks and
+//                    evaluates 64-entry batches from LDS rings as they fill
+//                    (record mode: writes the batches to a log instead)
+//   k_eval_batches,  split evaluation for few bunches: one wavefront per
+//   k_combine        logged batch, then the partials summed in batch order
+//                    (the fused kernel's arithmetic, bit for bit)
 //   k_scatter        sorted forces -> caller order and layout
 #include <math.h>
 
