@@ -192,12 +192,16 @@ __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
                                               uint64_t* __restrict__ khi,
                                               uint32_t* __restrict__ klo,
                                               int* __restrict__ idx,
-                                              uint64_t* __restrict__ khio) {
+                                              uint64_t* __restrict__ khio,
+                                              const double* __restrict__ m,
+                                              double4* __restrict__ rec,
+                                              uint2* __restrict__ aux) {
 #pragma clang fp contract(off)
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     double px = x[p], py = y[p], pz = z[p];
-    const double* g = sg + SG * (sysid ? sysid[p] : 0);
+    const int sp = sysid ? sysid[p] : 0;
+    const double* g = sg + SG * sp;
     double cx = g[S_CX], cy = g[S_CY], cz = g[S_CZ], h = g[S_HALF];
     uint64_t hi = 0;
     uint32_t lo = 0;
@@ -219,6 +223,10 @@ __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
     klo[p] = lo;
     idx[p] = (int)p;
     if (khio) khio[p] = hi;
+    // one packed record per particle, so that the gather after the sort reads
+    // one line per particle instead of one per array
+    rec[p] = make_double4(px, py, pz, m[p]);
+    aux[p] = make_uint2(lo, (uint32_t)sp);
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -339,25 +347,23 @@ __global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __re
 }
 
 __global__ __launch_bounds__(256) void k_gather(const int* __restrict__ idx,
-                                                const double* __restrict__ x,
-                                                const double* __restrict__ y,
-                                                const double* __restrict__ z,
-                                                const double* __restrict__ m,
-                                                const uint32_t* __restrict__ klo, int64_t n,
+                                                const double4* __restrict__ rec,
+                                                const uint2* __restrict__ aux, int64_t n,
                                                 double* __restrict__ sx, double* __restrict__ sy,
                                                 double* __restrict__ sz, double* __restrict__ sm,
                                                 uint32_t* __restrict__ sklo,
-                                                const int* __restrict__ sysid,
                                                 int* __restrict__ ssys) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int p = idx[i];
-    sx[i] = x[p];
-    sy[i] = y[p];
-    sz[i] = z[p];
-    sm[i] = m[p];
-    sklo[i] = klo[p];
-    ssys[i] = sysid ? sysid[p] : 0;
+    const int p = idx[i];
+    const double4 r = rec[p];
+    const uint2 a = aux[p];
+    sx[i] = r.x;
+    sy[i] = r.y;
+    sz[i] = r.z;
+    sm[i] = r.w;
+    sklo[i] = a.x;
+    ssys[i] = (int)a.y;
 }
 
 // A batch is sorted by octant key, then stably by system: the second sort's
@@ -861,7 +867,8 @@ int tree_build(hq_tree* t) {
     int rc;
     size_t n8 = n * sizeof(double);
     if ((rc = t->geom.ensure((G_SYS + (size_t)SG * nsys) * sizeof(double)))) return rc;
-    if ((rc = t->khi.ensure(n * 8)) || (rc = t->klo.ensure(n * 4)) || (rc = t->idx.ensure(n * 4)))
+    if ((rc = t->khi.ensure(n * 8)) || (rc = t->klo.ensure(n * 4)) || (rc = t->idx.ensure(n * 4)) ||
+        (rc = t->prec.ensure(n * 32)) || (rc = t->paux.ensure(n * 8)))
         return rc;
     if ((rc = t->sx.ensure(n8)) || (rc = t->sy.ensure(n8)) || (rc = t->sz.ensure(n8)) ||
         (rc = t->sm.ensure(n8)) || (rc = t->sklo.ensure(n * 4)) || (rc = t->lcp.ensure(n)) ||
@@ -919,7 +926,9 @@ int tree_build(hq_tree* t) {
                                             t->z.as<double>(), n, sg, sysid,
                                             t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
                                             t->idx.as<int>(),
-                                            batch ? t->khio.as<uint64_t>() : nullptr);
+                                            batch ? t->khio.as<uint64_t>() : nullptr,
+                                            t->m.as<double>(), t->prec.as<double4>(),
+                                            t->paux.as<uint2>());
     HQ_CHECK_HIP(mark(1));
     // 3. sort: by octant key; a batch then stably by system
     if ((rc = radix_sort(t))) return rc;
@@ -938,12 +947,11 @@ int tree_build(hq_tree* t) {
     k_tie_fix<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
                                                t->idx.as<int>(), sysid, n,
                                                t->status.as<int64_t>());
-    k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->x.as<double>(),
-                                              t->y.as<double>(), t->z.as<double>(),
-                                              t->m.as<double>(), t->klo.as<uint32_t>(), n,
-                                              t->sx.as<double>(), t->sy.as<double>(),
-                                              t->sz.as<double>(), t->sm.as<double>(),
-                                              t->sklo.as<uint32_t>(), sysid, t->ssys.as<int>());
+    k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->prec.as<double4>(),
+                                              t->paux.as<uint2>(), n, t->sx.as<double>(),
+                                              t->sy.as<double>(), t->sz.as<double>(),
+                                              t->sm.as<double>(), t->sklo.as<uint32_t>(),
+                                              t->ssys.as<int>());
     HQ_CHECK_HIP(mark(2));
     // 4. nodes
     k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),
