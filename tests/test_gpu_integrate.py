@@ -89,6 +89,25 @@ def test_bunched_graph_replay_matches_launched_steps(scheme):
         q, p = qg, pg
 
 
+def test_bunched_graph_replay_tiny_systems():
+    """Graph replay and split walk with systems of 1, 2, 3 and 65 bodies in one
+    batch (single-leaf roots, bunches smaller than a wavefront)."""
+    sizes = [1, 2, 3, 65, 400]
+    sys_ = [_plummer(k, 60 + i) for i, k in enumerate(sizes)]
+    m = np.concatenate([s[0] for s in sys_])
+    q = np.concatenate([s[1] + 9.0 * i for i, s in enumerate(sys_)])
+    p = np.concatenate([s[2] for s in sys_])
+    for scheme in ("verlet", "euler"):
+        tg = OctreeType().pre_init(m, 12, counts=sizes, precision="mixed")
+        te = OctreeType().pre_init(m, 12, counts=sizes, precision="mixed").tune(graph=False,
+                                                                                split=0)
+        tg.set_state(q, p).integrate(1e-3, 5, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
+        te.set_state(q, p).integrate(1e-3, 5, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
+        (qg, pg), (qe, pe) = tg.get_state(), te.get_state()
+        assert np.array_equal(qg, qe) and np.array_equal(pg, pe), scheme
+        assert np.all(np.isfinite(qg))
+
+
 def test_bunched_graph_error_reported_like_launched_steps():
     """An error inside the replayed steps (two bodies meeting exactly at step
     3, so lost at depth 25) is flagged on the device; the steps are then rerun
