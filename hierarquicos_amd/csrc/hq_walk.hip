@@ -484,12 +484,17 @@ __global__ __launch_bounds__(64) void k_eval_fp64(NodeView nv, const double* __r
 }
 
 // ------------------------------------------------------------------ mixed evaluation
-// Node moments as FP32 records (24 floats): M, Qxx,Qyy,Qzz,Qxy,Qxz,Qyz,
-// -1.5*trQ, T' = -1.5*(Sum m d r^2) (3), O (10, octree_default.f90 :831-843
-// order), pad.  The oct M2P of :2032-2082 is rewritten with
-// V = O:dd (V_x = dx^2 O1 + dy^2 O2 + dz^2 O3 + 2 dx dy O4 + 2 dx dz O7 +
-// 2 dy dz O10, cyclic) and d.V = the 17.5-term bracket, which is the same
-// polynomial grouped for fewer operations.
+// Node moments as FP32 records of 24 floats, pre-scaled for the M2P below:
+//   r[0]      M
+//   r[1..6]   Q~ = -3 (Qxx, Qyy, Qzz, Qxy, Qxz, Qyz)   (raw second moments)
+//   r[7]      tq = -1.5 trQ
+//   r[8..10]  T~ = -1.5 T, T_i = sum_j O_ijj   (octupole only)
+//   r[11..23] O~ = 7.5 x (O1..O9, O10, 2 O10, 2 O8, 2 O6)  (octupole only;
+//             O1..O10 in octree_default.f90 :831-843 order: xxx, xyy, xzz,
+//             xxy, yyy, yzz, xxz, yyz, zzz, xyz)
+// The moments are the raw (not traceless) ones: with softening the raw
+// expansion is the exact Taylor series of the softened potential, which the
+// traceless one is not.
 constexpr int REC = 24;
 
 __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeView nv,
@@ -504,14 +509,19 @@ __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeVi
     for (int j = 0; j < REC; ++j) r[j] = 0.0f;
     r[0] = (float)nv.mass[k];
     if (multipole >= 4) {
-        double tr = 0.0;
-        for (int j = 0; j < 6; ++j) r[1 + j] = (float)nv.quad[j * S + k];
-        tr = nv.quad[0 * S + k] + nv.quad[1 * S + k] + nv.quad[2 * S + k];
-        r[7] = (float)(-1.5 * tr);
+        double q[6];
+        for (int j = 0; j < 6; ++j) q[j] = nv.quad[j * S + k];
+        for (int j = 0; j < 6; ++j) r[1 + j] = (float)(-3.0 * q[j]);
+        r[7] = (float)(-1.5 * (q[0] + q[1] + q[2]));
     }
     if (multipole > 4) {
-        for (int j = 0; j < 3; ++j) r[8 + j] = (float)(-1.5 * nv.oct[(11 + j) * S + k]);
-        for (int j = 0; j < 10; ++j) r[11 + j] = (float)nv.oct[j * S + k];
+        double o[14];
+        for (int j = 0; j < 14; ++j) o[j] = nv.oct[j * S + k];
+        for (int j = 0; j < 3; ++j) r[8 + j] = (float)(-1.5 * o[11 + j]);
+        for (int j = 0; j < 10; ++j) r[11 + j] = (float)(7.5 * o[j]);
+        r[21] = (float)(15.0 * o[9]);
+        r[22] = (float)(15.0 * o[7]);
+        r[23] = (float)(15.0 * o[5]);
     }
     float4* o = reinterpret_cast<float4*>(rec + k * REC);
 #pragma unroll
@@ -531,6 +541,15 @@ struct Kahan3 {
     }
 };
 
+// Force of one cell on one target, d = com - target, s^2 = |d|^2 + eps^2,
+// with r_k = s^-k.  The reference's expansion (octree_default.f90 :2032-2082)
+//   acc = d [M r3 - 1.5 trQ r5 + 7.5 r7 (d.Qd + d.T) - 17.5 r9 d.(O:dd)]
+//         + r5 (-3 Qd - 1.5 T) + 7.5 r7 O:dd
+// is grouped, with the record's scaled moments (w = Q~d + T~, v = O~:dd), as
+//   acc = B d + r5 (w + v / s^2),
+//   B   = r3 (M + (tq - 2.5 (d.(w + T~) + (14/15) d.v / s^2) / s^2) / s^2)
+// which is the same polynomial in fewer operations (68 VALU for the octupole
+// term, 32 for the quadrupole).
 template <int MP>
 __device__ __forceinline__ void m2p32(const float4 e, const float* __restrict__ r, float tx,
                                       float ty, float tz, float eps2, float acc[3]) {
@@ -539,46 +558,40 @@ __device__ __forceinline__ void m2p32(const float4 e, const float* __restrict__ 
     const float ri = __frsqrt_rn(s2);
     const float ri2 = ri * ri;
     const float r3 = ri * ri2;
-    float cd = r[0] * r3;  // M / s^3
     if (MP == 1) {
+        const float cd = r[0] * r3;  // M / s^3
         acc[0] = fmaf(cd, dx, acc[0]);
         acc[1] = fmaf(cd, dy, acc[1]);
         acc[2] = fmaf(cd, dz, acc[2]);
         return;
     }
     const float r5 = r3 * ri2;
-    const float r7 = r5 * ri2;
-    const float qx = fmaf(r[1], dx, fmaf(r[4], dy, r[5] * dz));
-    const float qy = fmaf(r[4], dx, fmaf(r[2], dy, r[6] * dz));
-    const float qz = fmaf(r[5], dx, fmaf(r[6], dy, r[3] * dz));
-    float dq = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
-    cd = fmaf(r[7], r5, cd);  // - 1.5 trQ / s^5
-    float wx = -3.0f * qx, wy = -3.0f * qy, wz = -3.0f * qz;
-    const float c7 = 7.5f * r7;
+    // w = Q~ d (+ T~)
+    const float t0 = MP == 8 ? r[8] : 0.0f, t1 = MP == 8 ? r[9] : 0.0f, t2 = MP == 8 ? r[10] : 0.0f;
+    float wx = fmaf(r[1], dx, fmaf(r[4], dy, fmaf(r[5], dz, t0)));
+    float wy = fmaf(r[4], dx, fmaf(r[2], dy, fmaf(r[6], dz, t1)));
+    float wz = fmaf(r[5], dx, fmaf(r[6], dy, fmaf(r[3], dz, t2)));
+    float inner;  // the bracket that multiplies -2.5 / s^4
     if (MP == 8) {
+        const float a = fmaf(dx, wx + t0, fmaf(dy, wy + t1, dz * (wz + t2)));
         const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
         const float dx2 = dx + dx;
-        const float xy = dx2 * dy, xz = dx2 * dz, yz = (dy + dy) * dz;
-        // V = O:dd with O1..O10 at r[11..20]
-        const float vx = fmaf(xx, r[11], fmaf(yy, r[12], fmaf(zz, r[13], fmaf(xy, r[14], fmaf(xz, r[17], yz * r[20])))));
-        const float vy = fmaf(xx, r[14], fmaf(yy, r[15], fmaf(zz, r[16], fmaf(xy, r[12], fmaf(xz, r[20], yz * r[18])))));
-        const float vz = fmaf(xx, r[17], fmaf(yy, r[18], fmaf(zz, r[19], fmaf(xy, r[20], fmaf(xz, r[13], yz * r[16])))));
-        const float dv = fmaf(dx, vx, fmaf(dy, vy, dz * vz));
-        // T' = -1.5 T at r[8..10]:  d.T = -(d.T')/1.5
-        const float dT = fmaf(dx, r[8], fmaf(dy, r[9], dz * r[10])) * (-2.0f / 3.0f);
-        dq = dq + dT;
-        cd = fmaf(-17.5f * r7 * ri2, dv, cd);
-        wx += r[8];
-        wy += r[9];
-        wz += r[10];
-        acc[0] = fmaf(c7, vx, acc[0]);
-        acc[1] = fmaf(c7, vy, acc[1]);
-        acc[2] = fmaf(c7, vz, acc[2]);
+        const float xy2 = dx2 * dy, xz2 = dx2 * dz, yz = dy * dz;
+        const float vx = fmaf(xx, r[11], fmaf(yy, r[12], fmaf(zz, r[13], fmaf(xy2, r[14], fmaf(xz2, r[17], yz * r[21])))));
+        const float vy = fmaf(xx, r[14], fmaf(yy, r[15], fmaf(zz, r[16], fmaf(xy2, r[12], fmaf(xz2, r[20], yz * r[22])))));
+        const float vz = fmaf(xx, r[17], fmaf(yy, r[18], fmaf(zz, r[19], fmaf(xy2, r[20], fmaf(xz2, r[13], yz * r[23])))));
+        const float b = fmaf(dx, vx, fmaf(dy, vy, dz * vz));
+        inner = fmaf((14.0f / 15.0f) * ri2, b, a);
+        wx = fmaf(ri2, vx, wx);
+        wy = fmaf(ri2, vy, wy);
+        wz = fmaf(ri2, vz, wz);
+    } else {
+        inner = fmaf(dx, wx, fmaf(dy, wy, dz * wz));
     }
-    cd = fmaf(c7, dq, cd);
-    acc[0] = fmaf(cd, dx, fmaf(r5, wx, acc[0]));
-    acc[1] = fmaf(cd, dy, fmaf(r5, wy, acc[1]));
-    acc[2] = fmaf(cd, dz, fmaf(r5, wz, acc[2]));
+    const float B = r3 * fmaf(ri2, fmaf(-2.5f * ri2, inner, r[7]), r[0]);
+    acc[0] = fmaf(B, dx, fmaf(r5, wx, acc[0]));
+    acc[1] = fmaf(B, dy, fmaf(r5, wy, acc[1]));
+    acc[2] = fmaf(B, dz, fmaf(r5, wz, acc[2]));
 }
 
 __device__ __forceinline__ void p2p32(const float4 e, float tx, float ty, float tz, float eps2,
