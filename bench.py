@@ -296,6 +296,9 @@ def main():
     ap.add_argument("--unfused", action="store_true", help="lists and evaluation as two kernels")
     ap.add_argument("--leaf", type=int, default=4,
                     help="leaf bucket: cells of <= leaf bodies enter the lists as their bodies")
+    ap.add_argument("--far", type=float, default=6.0,
+                    help="far-field factor: accepted cells at least this many bunch radii from "
+                         "the bunch centre enter its local expansion (0: off)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick bunch / super-group / leaf bucket with the autotuner first")
     ap.add_argument("--sample", type=int, default=4096)
@@ -351,7 +354,8 @@ def main():
     dx, dy, dz = (DeviceArray.from_numpy(np.ascontiguousarray(q[:, k])) for k in range(3))
     dF = DeviceArray((3 * N,))
     tree = OctreeType().pre_init(dm, args.method, args.bunch, precision="mixed")
-    tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused, leaf=args.leaf)
+    tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused, leaf=args.leaf,
+              far=args.far)
     tuned = None
     if args.autotune:
         tree.init(dx, dy, dz)
@@ -479,7 +483,7 @@ def main():
                    "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
                    "walk": (f"bunched Barnes-1990 lists, <= {args.bunch} particles per bunch, "
                             f"cut from sibling subtrees packed to <= {args.super}, leaf bucket "
-                            f"{args.leaf}"),
+                            f"{args.leaf}, far field beyond {args.far:g} bunch radii"),
                    "autotuned": tuned},
         "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
         "ms_per_step_median": q50, "ms_per_step_iqr": [q25, q75],
@@ -492,7 +496,8 @@ def main():
         "walk_stats": {"bunches": info["ngroups"], "list_cells": info["list_cells"],
                        "list_parts": info["list_parts"], "exec_cells": info["exec_cells"],
                        "exec_parts": info["exec_parts"], "max_cells": info["max_cells"],
-                       "max_parts": info["max_parts"], "max_stack": info["max_stack"]},
+                       "max_parts": info["max_parts"], "max_stack": info["max_stack"],
+                       "far_cells": info["far_cells"]},
         "roofline": roof,
         "cpu_baseline": base,
     }

@@ -186,6 +186,9 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
     } else if (key == HQ_TUNE_SPLIT) {
         if (value < 0 || value > 2) return fail(HQ_ERR_ARGS, "split mode %lld outside 0..2", (long long)value);
         t->split = (int)value;
+    } else if (key == HQ_TUNE_FAR) {
+        if (value < 0 || value > 1000) return fail(HQ_ERR_ARGS, "far factor %lld outside 0..1000", (long long)value);
+        t->far = (int)value;
     } else if (key == HQ_TUNE_LEAF) {
         if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
         t->leaf = (int)value;
@@ -305,6 +308,7 @@ int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {
     info->multipole = t->multipole;
     info->list_cells = t->list_cells;
     info->list_parts = t->list_parts;
+    info->far_cells = t->far_cells;
     info->n_p2p = t->n_p2p;
     info->n_m2p = t->n_m2p;
     info->ms_build = t->ms_build;

@@ -56,10 +56,13 @@ struct hq_tree {
     std::unique_ptr<hq::FmmState, hq::FmmFree> fmm;
     bool fused = true;     // mixed bunched walk as one fused kernel
     int split = 1;         // split evaluation of the fused walk: 0 never, 1 few bunches, 2 always
+    int far = 60;          // far-field factor of the fused mixed walk, in tenths (0: off)
+    int64_t far_cells = 0; // cells of the last walk taken through the far-field expansion
     int64_t split_lcap = 0;  // log entries per bunch (from the last walk's lists; 0 unknown)
     int64_t split_walks = 0; // walks evaluated split (diagnostic)
     bool graph_split = false;
     hq::DevBuf slog, sdesc, snbat, spart;
+    hq::DevBuf flist, fnum;  // far-field lists of the mixed walk: node indices and counts per bunch
     int64_t ngroups = 0;
     // walk scratch
     hq::DevBuf fx, fy, fz, counts, clist, plist, lstat;
@@ -112,6 +115,7 @@ enum {
     ST_EXECC = 39,     // executed target-cell interactions of the last grouped walk
     ST_EXECP = 40,     // executed target-particle interactions
     ST_MAXSTK = 41,    // deepest walk stack of the last grouped walk
+    ST_LISTF = 42,     // cells of the mixed walk taken through the far-field expansion
     ST_SIZE = 48
 };
 

@@ -31,6 +31,7 @@
 #include <algorithm>
 
 #include "hq_tree.h"
+#include "hq_local.h"
 
 namespace hq {
 
@@ -605,6 +606,72 @@ __device__ __forceinline__ void p2p32(const float4 e, float tx, float ty, float 
     acc[2] = fmaf(f, dz, acc[2]);
 }
 
+// Sum of v over the wavefront, returned to every lane.  Row shifts within the
+// four 16-lane rows, then row broadcasts, in a fixed order: the rounding does
+// not depend on timing, so every kernel that calls it on the same values gets
+// the same bits.  All 64 lanes must be active.
+__device__ __forceinline__ float wave_sum(float v) {
+    int x = __float_as_int(v);
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false)));
+    return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
+// Radius of a bunch for its far-field expansion: the largest distance of its
+// bodies (lanes < nb) from the bunch centre, in the FP32 offsets the walk
+// uses, at least 1e-30.
+__device__ __forceinline__ float bunch_radius(float tx, float ty, float tz, int lane, int nb) {
+    float r2 = lane < nb ? fmaf(tx, tx, fmaf(ty, ty, tz * tz)) : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o, 64));
+    return fmaxf(__fsqrt_rn(r2), 1e-30f);
+}
+
+// One batch of <= 64 far cells through the bunch's local expansion: lane j
+// forms the expansion of its cell (node >= 0; -1 on idle lanes) about the
+// bunch centre bc, the wavefront sums the 55 coefficients, and every lane
+// evaluates the sum at its target (tx, ty, tz relative to bc).  acc starts at
+// zero.  All 64 lanes must be active.
+template <int MP>
+__device__ __forceinline__ void eval_far_acc(int node, const double* __restrict__ cx,
+                                             const double* __restrict__ cy,
+                                             const double* __restrict__ cz,
+                                             const float* __restrict__ mom, const double bc[3],
+                                             float h, float hinv, float tx, float ty, float tz,
+                                             float eps2, float acc[3]) {
+#pragma clang fp contract(off)
+    float r[REC];
+#pragma unroll
+    for (int j = 0; j < REC; ++j) r[j] = 0.0f;
+    float Rx = 1.0f, Ry = 1.0f, Rz = 1.0f;  // idle lane: no mass, finite geometry
+    if (node >= 0) {
+        Rx = (float)(bc[0] - cx[node]);
+        Ry = (float)(bc[1] - cy[node]);
+        Rz = (float)(bc[2] - cz[node]);
+        const float4* __restrict__ rp = reinterpret_cast<const float4*>(mom + (int64_t)node * REC);
+        constexpr int NQ = MP == 1 ? 1 : (MP == 4 ? 2 : REC / 4);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const float4 v4 = rp[q];
+            r[4 * q] = v4.x;
+            r[4 * q + 1] = v4.y;
+            r[4 * q + 2] = v4.z;
+            r[4 * q + 3] = v4.w;
+        }
+    }
+    float L[NLOC];  // the wavefront's sums: the same in every lane
+    local_m2l<MP>(Rx, Ry, Rz, eps2, h, r, [&](int j, float v) { L[j] = wave_sum(v); });
+    float a[3];
+    local_l2p(L, tx * hinv, ty * hinv, tz * hinv, a);
+    acc[0] = -hinv * a[0];
+    acc[1] = -hinv * a[1];
+    acc[2] = -hinv * a[2];
+}
+
 constexpr int KBLK = 32;  // interactions per compensated partial sum
 
 template <int MP>
@@ -820,6 +887,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    const double* __restrict__ sm, int64_t n,
                                                    const int* __restrict__ bstart, int nbunch,
                                                    double theta2, double G, float eps2, int lbucket,
+                                                   float far2, int* __restrict__ flist,
+                                                   int* __restrict__ fnum, int fcap,
                                                    const float* __restrict__ mom,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz,
@@ -855,8 +924,14 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
     const double bc[3] = {0.5 * (bmin[0] + bmax[0]), 0.5 * (bmin[1] + bmax[1]),
                           0.5 * (bmin[2] + bmax[2])};
     const float tx = (float)(sx[i] - bc[0]), ty = (float)(sy[i] - bc[1]), tz = (float)(sz[i] - bc[2]);
+    // far field: accepted cells whose centre of mass is at least sqrt(far2)
+    // bunch radii from the bunch centre go to the bunch's far list for k_far
+    // (at most fcap of them; any more are evaluated here like the others)
+    const float hb = bunch_radius(tx, ty, tz, lane, nb);
+    const double farR2 = far2 > 0.0f ? (double)far2 * ((double)hb * (double)hb) : INFINITY;
+    int* __restrict__ fl = flist + (int64_t)gi * fcap;
     Kahan3 sum;
-    int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0;
+    int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0, nfar = 0;
     long long ntot_c = 0, ntot_p = 0;
     bool overflow = false;
     int lpos = 0, nbat = 0;  // record mode: log position and batches
@@ -904,6 +979,12 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                     }
                     const int depth = nv.depth[k];
                     action = (gl2[depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                    if (action == 3) {
+                        double R2 = 0.0;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) R2 = fma(cc[c] - bc[c], cc[c] - bc[c], R2);
+                        if (R2 >= farR2) action = 5;  // far cell, if its list has room
+                    }
                 }
             }
         }
@@ -939,6 +1020,15 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                 pcnt += c;
                 __builtin_amdgcn_wave_barrier();
             }
+        }
+        {
+            const uint64_t mfar = __ballot(action == 5);
+            if (action == 5) {
+                const int slot = nfar + __popcll(mfar & lt);
+                if (slot < fcap) fl[slot] = k;
+                else action = 3;
+            }
+            nfar = min(nfar + __popcll(mfar), fcap);
         }
         const uint64_t mc = __ballot(action == 3);
         if (action == 3)
@@ -1003,11 +1093,13 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         if (lane == 0) so.nbat[gi] = (rover || overflow) ? -1 : nbat;
     }
     if (lane == 0) {
+        fnum[gi] = nfar;
         if (overflow) atomicOr((unsigned long long*)&status[ST_FLAGS], 16ull);
         if (rover) atomicOr((unsigned long long*)&status[ST_FLAGS], 32ull);
         const unsigned long long unb = (unsigned long long)nb;
-        atomicAdd((unsigned long long*)&status[ST_LISTC], (unsigned long long)ntot_c);
+        atomicAdd((unsigned long long*)&status[ST_LISTC], (unsigned long long)(ntot_c + nfar));
         atomicAdd((unsigned long long*)&status[ST_LISTP], (unsigned long long)ntot_p);
+        atomicAdd((unsigned long long*)&status[ST_LISTF], (unsigned long long)nfar);
         atomicAdd((unsigned long long*)&status[ST_EXECC], (unsigned long long)ntot_c * unb);
         atomicAdd((unsigned long long*)&status[ST_EXECP], (unsigned long long)ntot_p * unb + unb * (unb - 1));
         atomicMax((long long*)&status[ST_NEEDC], (long long)ntot_c);
@@ -1021,6 +1113,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
 struct BunchTargets {
     int i, nb;
     float tx, ty, tz;
+    double bc[3];
+    float h, hinv;  // radius for the far-field expansion, and its inverse
 };
 
 __device__ __forceinline__ BunchTargets bunch_targets(const double* __restrict__ sx,
@@ -1045,6 +1139,9 @@ __device__ __forceinline__ BunchTargets bunch_targets(const double* __restrict__
     b.tx = (float)(sx[b.i] - bc[0]);
     b.ty = (float)(sy[b.i] - bc[1]);
     b.tz = (float)(sz[b.i] - bc[2]);
+    for (int c = 0; c < 3; ++c) b.bc[c] = bc[c];
+    b.h = bunch_radius(b.tx, b.ty, b.tz, lane, b.nb);
+    b.hinv = 1.0f / b.h;
     return b;
 }
 
@@ -1110,6 +1207,48 @@ __global__ __launch_bounds__(64) void k_combine(const double* __restrict__ sx,
         fx[b.i] = gm * ((double)sum.s[0] - (double)sum.c[0]);
         fy[b.i] = gm * ((double)sum.s[1] - (double)sum.c[1]);
         fz[b.i] = gm * ((double)sum.s[2] - (double)sum.c[2]);
+    }
+}
+
+// Far field of the mixed walk: one wavefront per bunch takes its far list 64
+// cells at a time through the local expansion (eval_far_acc), sums the
+// batches with the walk's compensated sum and adds G m times the sum to the
+// force the walk wrote.  Runs after the walk (fused, or split and combined).
+template <int MP>
+__global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
+                                            const double* __restrict__ sy,
+                                            const double* __restrict__ sz,
+                                            const double* __restrict__ sm,
+                                            const int* __restrict__ bstart, int nbunch,
+                                            const int64_t* __restrict__ dnbun,
+                                            const int* __restrict__ flist,
+                                            const int* __restrict__ fnum, int fcap,
+                                            const double* __restrict__ ccx,
+                                            const double* __restrict__ ccy,
+                                            const double* __restrict__ ccz,
+                                            const float* __restrict__ mom, double G, float eps2,
+                                            double* __restrict__ fx, double* __restrict__ fy,
+                                            double* __restrict__ fz) {
+    if (dnbun) nbunch = (int)*dnbun;
+    if ((int)blockIdx.x >= nbunch) return;
+    const int gi = xcd_swizzle(blockIdx.x, nbunch);
+    const int nf = min(fnum[gi], fcap);
+    if (nf <= 0) return;
+    const int lane = __lane_id();
+    const BunchTargets b = bunch_targets(sx, sy, sz, bstart, gi, lane);
+    const int* __restrict__ fl = flist + (int64_t)gi * fcap;
+    Kahan3 sum;
+    for (int base = 0; base < nf; base += 64) {
+        const int node = base + lane < nf ? fl[base + lane] : -1;
+        float acc[3];
+        eval_far_acc<MP>(node, ccx, ccy, ccz, mom, b.bc, b.h, b.hinv, b.tx, b.ty, b.tz, eps2, acc);
+        sum.add(acc);
+    }
+    if (lane < b.nb) {
+        const double gm = G * sm[b.i];
+        fx[b.i] = fx[b.i] + gm * ((double)sum.s[0] - (double)sum.c[0]);
+        fy[b.i] = fy[b.i] + gm * ((double)sum.s[1] - (double)sum.c[1]);
+        fz[b.i] = fz[b.i] + gm * ((double)sum.s[2] - (double)sum.c[2]);
     }
 }
 
@@ -1293,6 +1432,39 @@ constexpr double SPLIT_MEM_MAX = 1024.0 * 1024 * 1024;  // bytes of log and part
 
 static int split_bcap(int64_t lcap) { return (int)(lcap / 64 + 3); }
 
+// Square of the far-field factor of the mixed walk (0: no far field).
+static float far2(const hq_tree* t) { return (float)(t->far * t->far) / 100.0f; }
+
+// Far cells a bunch's list holds (more are evaluated in the walk as ordinary
+// cells); 0 when the far field is off.
+constexpr int FAR_CAP = 4096;
+static int far_cap(const hq_tree* t) { return t->far > 0 ? FAR_CAP : 0; }
+
+static int far_buffers(hq_tree* t, int64_t nbunch) {
+    int rc;
+    if ((rc = t->fnum.ensure((size_t)(nbunch + 1) * sizeof(int))) ||
+        (rc = t->flist.ensure((size_t)(nbunch + 1) * (far_cap(t) + 1) * sizeof(int))))
+        return rc;
+    return HQ_OK;
+}
+
+// The far field of a mixed walk over grid bunches (after the walk).
+static void launch_far(hq_tree* t, int grid, const int64_t* dnbun, double G, double eps2) {
+    if (t->far <= 0) return;
+    auto launch = [&](auto kern) {
+        kern<<<grid, 64, 0, t->stream>>>(t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(),
+                                         t->sm.as<double>(), t->bstart.as<int>(), grid, dnbun,
+                                         t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
+                                         t->ncomx.as<double>(), t->ncomy.as<double>(),
+                                         t->ncomz.as<double>(), t->mom32.as<float>(), G,
+                                         (float)eps2, t->fx.as<double>(), t->fy.as<double>(),
+                                         t->fz.as<double>());
+    };
+    if (t->multipole == 1) launch(k_far<1>);
+    else if (t->multipole == 4) launch(k_far<4>);
+    else launch(k_far<8>);
+}
+
 static bool use_split(const hq_tree* t, int64_t nbunch) {
     if (t->split == 0 || t->split_lcap <= 0) return false;
     if (t->split == 1 && nbunch > SPLIT_MAX) return false;
@@ -1322,7 +1494,7 @@ static int split_buffers(hq_tree* t, int64_t nbunch) {
 static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta2, double G,
                         double eps2) {
     int rc;
-    if ((rc = split_buffers(t, grid))) return rc;
+    if ((rc = split_buffers(t, grid)) || (rc = far_buffers(t, grid))) return rc;
     hipStream_t s = t->stream;
     const SplitOut so{t->slog.as<float4>(), t->sdesc.as<int2>(), t->snbat.as<int>(),
                       t->split_lcap, split_bcap(t->split_lcap)};
@@ -1330,6 +1502,7 @@ static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta
         kern<<<grid, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                  t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                                  t->bstart.as<int>(), grid, theta2, G, (float)eps2, t->leaf,
+                                 far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                                  t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
                                  t->fz.as<double>(), t->status.as<int64_t>(), dnbun, so);
     };
@@ -1346,6 +1519,7 @@ static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta
                                   t->sm.as<double>(), t->bstart.as<int>(), grid, dnbun, so,
                                   t->spart.as<float4>(), G, (float)eps2, t->fx.as<double>(),
                                   t->fy.as<double>(), t->fz.as<double>());
+    launch_far(t, grid, dnbun, G, eps2);
     HQ_CHECK_HIP(hipGetLastError());
     return HQ_OK;
 }
@@ -1365,6 +1539,7 @@ int graph_prepare(hq_tree* t) {
         return rc;
     t->graph_split = use_split(t, t->graph_nbun);
     if (t->graph_split && (rc = split_buffers(t, t->graph_nbun))) return rc;
+    if ((rc = far_buffers(t, t->graph_nbun))) return rc;
     return HQ_OK;
 }
 
@@ -1394,12 +1569,14 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                     node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                     t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                     t->bstart.as<int>(), (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf,
+                    far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                     t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
                     t->fz.as<double>(), t->status.as<int64_t>(), gs + GS_NBUN, SplitOut{});
             };
             if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
             else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
             else launch(k_walk_mixed<8, SCAP_FUSED>);
+            launch_far(t, (int)t->graph_nbun, gs + GS_NBUN, G, eps2);
         }
         k_graph_walk_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), gs);
         HQ_CHECK_HIP(hipGetLastError());
@@ -1424,6 +1601,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 t->ms_lists = 0.0f;
                 t->ms_walk = t->ms_eval;
                 t->list_cells = t->hstatus[ST_LISTC];
+                t->far_cells = t->hstatus[ST_LISTF];
                 t->list_parts = t->hstatus[ST_LISTP];
                 t->exec_cells = t->hstatus[ST_EXECC];
                 t->exec_parts = t->hstatus[ST_EXECP];
@@ -1435,20 +1613,23 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             // the log or the stack overflowed: the fused walk below
         }
         if (t->fused) {
+            if ((rc = far_buffers(t, ngroups))) return rc;
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
             auto launch = [&](auto kern) {
                 kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                             t->sy.as<double>(), t->sz.as<double>(),
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
-                                            theta2, G, (float)eps2, t->leaf, t->mom32.as<float>(),
-                                            t->fx.as<double>(), t->fy.as<double>(),
+                                            theta2, G, (float)eps2, t->leaf, far2(t),
+                                            t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
+                                            t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
                                             t->fz.as<double>(), t->status.as<int64_t>(), nullptr,
                                             SplitOut{});
             };
             if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
             else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
             else launch(k_walk_mixed<8, SCAP_FUSED>);
+            launch_far(t, ngroups, nullptr, G, eps2);
             HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
             if ((rc = read_status(t))) return rc;
             HQ_CHECK_HIP(hipGetLastError());
@@ -1458,6 +1639,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 t->ms_lists = 0.0f;
                 t->ms_walk = t->ms_eval;
                 t->list_cells = t->hstatus[ST_LISTC];
+                t->far_cells = t->hstatus[ST_LISTF];
                 t->list_parts = t->hstatus[ST_LISTP];
                 t->exec_cells = t->hstatus[ST_EXECC];
                 t->exec_parts = t->hstatus[ST_EXECP];
@@ -1521,6 +1703,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         }
         t->list_cells = t->hstatus[ST_LISTC];
         t->list_parts = t->hstatus[ST_LISTP];
+        t->far_cells = t->hstatus[ST_LISTF];
         t->exec_cells = t->hstatus[ST_EXECC];
         t->exec_parts = t->hstatus[ST_EXECP];
         t->max_cells = t->hstatus[ST_NEEDC];

@@ -120,6 +120,12 @@ int hq_tree_destroy(hq_tree *t);
                               each bunch's batches and evaluates them one wavefront per batch,
                               then sums them in order (same results as the fused kernel, more
                               parallel); 0 never; 2 whenever the logs fit */
+#define HQ_TUNE_FAR 8      /* mixed bunched walk, fused or split: an accepted cell whose centre
+                              of mass lies at least value/10 bunch radii from the bunch centre
+                              enters the bunch's local expansion (one M2L per cell, orders 5 / 3
+                              / 2 for its monopole / quadrupole / octupole terms, evaluated once
+                              per body) instead of one M2P per body; 0..1000, default 60; 0 off.
+                              The two-kernel walk (HQ_TUNE_FUSED 0) has no far field */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
 /* Cartesian FMM of order 4 or 6 on the built tree: Dehnen's method carried to
@@ -237,6 +243,7 @@ typedef struct {
     int64_t max_parts;     /* longest particle list */
     int64_t max_stack;     /* deepest walk stack */
     int64_t split_walks;   /* mixed walks of this handle evaluated split (HQ_TUNE_SPLIT) */
+    int64_t far_cells;     /* cells of the last mixed walk taken through the far field (HQ_TUNE_FAR) */
 } hq_tree_info_t;
 int hq_tree_info(const hq_tree *t, hq_tree_info_t *info);
 

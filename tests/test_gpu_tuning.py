@@ -52,8 +52,9 @@ def test_buckets_do_not_increase_error():
 
 
 def test_fused_and_two_kernel_agree_with_buckets():
+    # the two-kernel walk has no far field: compare with it switched off
     m, x, y, z, t = _tree(30000, 23)
-    t.tune(leaf=6)
+    t.tune(leaf=6, far=0)
     a = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
     t.tune(fused=False)
     b = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
@@ -88,3 +89,26 @@ def test_split_walk_bitidentical_to_fused(method, leaf):
     assert ta.info()["split_walks"] == 0 and tb.info()["split_walks"] == 1
     assert np.array_equal(Fa, Fb1) and np.array_equal(Fa, Fb2)
     assert tb.info()["exec_cells"] == ta.info()["exec_cells"]
+
+
+@pytest.mark.parametrize("method", [10, 11, 12])
+def test_far_field_error_budget(method):
+    """Far cells through each bunch's local expansion (the default) against the
+    same walk with every cell evaluated per body: the extra error is a small
+    part of the multipole error, and both stay within the budget against the
+    FP64 direct sum."""
+    m, x, y, z, t = _tree(200000, 25)
+    sample = np.arange(0, 200000, 101)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    t2 = OctreeType().pre_init(m, method, precision="mixed").init(x, y, z)
+    t2.tune(far=6.0)
+    Ff = t2.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    assert t2.info()["far_cells"] > 0
+    t2.tune(far=0)
+    Fn = t2.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    assert t2.info()["far_cells"] == 0
+    e_far, _ = O.rel_errors(Ff[sample], Fd)
+    e_near, _ = O.rel_errors(Fn[sample], Fd)
+    d, _ = O.rel_errors(Ff[sample], Fn[sample])
+    assert d < 0.35 * e_near, (d, e_near)
+    assert e_far < 1.05 * e_near + 1e-6, (e_far, e_near)
