@@ -631,18 +631,14 @@ __device__ __forceinline__ float bunch_radius(float tx, float ty, float tz, int 
     return fmaxf(__fsqrt_rn(r2), 1e-30f);
 }
 
-// One batch of <= 64 far cells through the bunch's local expansion: lane j
-// forms the expansion of its cell (node >= 0; -1 on idle lanes) about the
-// bunch centre bc, the wavefront sums the 55 coefficients, and every lane
-// evaluates the sum at its target (tx, ty, tz relative to bc).  acc starts at
-// zero.  All 64 lanes must be active.
+// Local expansion of this lane's far cell (node >= 0; -1 adds nothing) about
+// the bunch centre bc, added to the lane's coefficients L (scaled by h^|n|).
 template <int MP>
-__device__ __forceinline__ void eval_far_acc(int node, const double* __restrict__ cx,
-                                             const double* __restrict__ cy,
-                                             const double* __restrict__ cz,
-                                             const float* __restrict__ mom, const double bc[3],
-                                             float h, float hinv, float tx, float ty, float tz,
-                                             float eps2, float acc[3]) {
+__device__ __forceinline__ void far_m2l(int node, const double* __restrict__ cx,
+                                        const double* __restrict__ cy,
+                                        const double* __restrict__ cz,
+                                        const float* __restrict__ mom, const double bc[3], float h,
+                                        float eps2, float L[NLOC]) {
 #pragma clang fp contract(off)
     float r[REC];
 #pragma unroll
@@ -663,13 +659,7 @@ __device__ __forceinline__ void eval_far_acc(int node, const double* __restrict_
             r[4 * q + 3] = v4.w;
         }
     }
-    float L[NLOC];  // the wavefront's sums: the same in every lane
-    local_m2l<MP>(Rx, Ry, Rz, eps2, h, r, [&](int j, float v) { L[j] = wave_sum(v); });
-    float a[3];
-    local_l2p(L, tx * hinv, ty * hinv, tz * hinv, a);
-    acc[0] = -hinv * a[0];
-    acc[1] = -hinv * a[1];
-    acc[2] = -hinv * a[2];
+    local_m2l<MP>(Rx, Ry, Rz, eps2, h, r, [&](int j, float v) { L[j] = L[j] + v; });
 }
 
 constexpr int KBLK = 32;  // interactions per compensated partial sum
@@ -965,10 +955,6 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                     action = leaf ? 0 : 1;
                 } else if (leaf) {
                     action = 2;
-                } else if (l - f + 1 <= lbucket) {
-                    action = 4;  // small cell: its bodies, exactly
-                    bfirst = f;
-                    bcount = l - f + 1;
                 } else {
                     const double cc[3] = {nv.cx[k], nv.cy[k], nv.cz[k]};
                     double dist2 = 0.0;
@@ -984,6 +970,11 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
 #pragma unroll
                         for (int c = 0; c < 3; ++c) R2 = fma(cc[c] - bc[c], cc[c] - bc[c], R2);
                         if (R2 >= farR2) action = 5;  // far cell, if its list has room
+                    }
+                    if (action != 5 && l - f + 1 <= lbucket) {
+                        action = 4;  // small cell, not far: its bodies, exactly
+                        bfirst = f;
+                        bcount = l - f + 1;
                     }
                 }
             }
@@ -1210,10 +1201,11 @@ __global__ __launch_bounds__(64) void k_combine(const double* __restrict__ sx,
     }
 }
 
-// Far field of the mixed walk: one wavefront per bunch takes its far list 64
-// cells at a time through the local expansion (eval_far_acc), sums the
-// batches with the walk's compensated sum and adds G m times the sum to the
-// force the walk wrote.  Runs after the walk (fused, or split and combined).
+// Far field of the mixed walk: one wavefront per bunch forms the local
+// expansion of its far list (far_m2l, lane-parallel over the cells), sums it
+// over the wavefront, evaluates it at its bodies and adds G m times the result
+// to the force the walk wrote.  Runs after the walk (fused, or split and
+// combined).
 template <int MP>
 __global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
                                             const double* __restrict__ sy,
@@ -1237,18 +1229,24 @@ __global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
     const int lane = __lane_id();
     const BunchTargets b = bunch_targets(sx, sy, sz, bstart, gi, lane);
     const int* __restrict__ fl = flist + (int64_t)gi * fcap;
-    Kahan3 sum;
+    // lane j takes cells j, j + 64, ... into its own coefficients; the
+    // wavefront sums them once and every lane evaluates the sum at its body
+    float L[NLOC];
+#pragma unroll
+    for (int j = 0; j < NLOC; ++j) L[j] = 0.0f;
     for (int base = 0; base < nf; base += 64) {
         const int node = base + lane < nf ? fl[base + lane] : -1;
-        float acc[3];
-        eval_far_acc<MP>(node, ccx, ccy, ccz, mom, b.bc, b.h, b.hinv, b.tx, b.ty, b.tz, eps2, acc);
-        sum.add(acc);
+        far_m2l<MP>(node, ccx, ccy, ccz, mom, b.bc, b.h, eps2, L);
     }
+#pragma unroll
+    for (int j = 0; j < NLOC; ++j) L[j] = wave_sum(L[j]);
+    float a[3];
+    local_l2p(L, b.tx * b.hinv, b.ty * b.hinv, b.tz * b.hinv, a);
     if (lane < b.nb) {
         const double gm = G * sm[b.i];
-        fx[b.i] = fx[b.i] + gm * ((double)sum.s[0] - (double)sum.c[0]);
-        fy[b.i] = fy[b.i] + gm * ((double)sum.s[1] - (double)sum.c[1]);
-        fz[b.i] = fz[b.i] + gm * ((double)sum.s[2] - (double)sum.c[2]);
+        fx[b.i] = fx[b.i] + gm * (double)(-b.hinv * a[0]);
+        fy[b.i] = fy[b.i] + gm * (double)(-b.hinv * a[1]);
+        fz[b.i] = fz[b.i] + gm * (double)(-b.hinv * a[2]);
     }
 }
 
