@@ -296,7 +296,7 @@ def main():
     ap.add_argument("--unfused", action="store_true", help="lists and evaluation as two kernels")
     ap.add_argument("--leaf", type=int, default=4,
                     help="leaf bucket: cells of <= leaf bodies enter the lists as their bodies")
-    ap.add_argument("--far", type=float, default=6.0,
+    ap.add_argument("--far", type=float, default=5.0,
                     help="far-field factor: accepted cells at least this many bunch radii from "
                          "the bunch centre enter its local expansion (0: off)")
     ap.add_argument("--autotune", action="store_true",

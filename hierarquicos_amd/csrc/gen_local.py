@@ -29,7 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "hq_local.h")
 
 PMONO, PQUAD, POCT = 5, 3, 2
-KMAX = 5
+KMAX = max(PMONO, PQUAD + 2, POCT + 3)
 
 
 def midx(n):

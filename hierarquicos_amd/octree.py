@@ -152,7 +152,7 @@ class OctreeType:
         step (default True); split: 0 never, 1 (default) with few bunches, 2
         always -- the fused walk's batches evaluated one wavefront each (same
         results); far: the mixed walk's far-field factor in bunch radii (default
-        6.0; 0 off): farther accepted cells enter the bunch's local expansion."""
+        5.0; 0 off): farther accepted cells enter the bunch's local expansion."""
         lib = L.lib()
         if far is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FAR, int(round(10 * float(far)))))

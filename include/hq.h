@@ -124,7 +124,7 @@ int hq_tree_destroy(hq_tree *t);
                               of mass lies at least value/10 bunch radii from the bunch centre
                               enters the bunch's local expansion (one M2L per cell, orders 5 / 3
                               / 2 for its monopole / quadrupole / octupole terms, evaluated once
-                              per body) instead of one M2P per body; 0..1000, default 60; 0 off.
+                              per body) instead of one M2P per body; 0..1000, default 50; 0 off.
                               The two-kernel walk (HQ_TUNE_FUSED 0) has no far field */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 

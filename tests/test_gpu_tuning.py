@@ -101,7 +101,7 @@ def test_far_field_error_budget(method):
     sample = np.arange(0, 200000, 101)
     Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
     t2 = OctreeType().pre_init(m, method, precision="mixed").init(x, y, z)
-    t2.tune(far=6.0)
+    t2.tune(far=5.0)
     Ff = t2.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
     assert t2.info()["far_cells"] > 0
     t2.tune(far=0)
