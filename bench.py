@@ -446,14 +446,20 @@ def main():
 
     eval_ms = float(np.mean(evals))
     mp = {10: 1, 11: 4, 12: 8}[args.method]
-    kernel = f"k_eval_mixed<{mp}>" if args.unfused else f"k_walk_mixed<{mp},1536>"
+    if args.unfused:
+        kernel = f"k_eval_mixed<{mp}>"
+    elif args.far > 0:   # the walk and its far-field kernel, timed together
+        kernel = f"k_walk_mixed<{mp},1536> + k_far<{mp}>"
+    else:
+        kernel = f"k_walk_mixed<{mp},1536>"
     order = {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method]
     workload = f"{args.profile}_n{N}_{order}_theta{args.theta}_eps{args.eps}"
     traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
     tf = os.path.join(REPO, "profiles", f"traffic_k_walk_mixed_{workload.split('_theta')[0]}.json")
     if os.path.exists(tf) and not args.unfused:
         rec = json.load(open(tf))
-        if rec.get("workload") == workload and rec.get("leaf", 0) == args.leaf:
+        if (rec.get("workload") == workload and rec.get("leaf", 0) == args.leaf
+                and rec.get("far", 0.0) == args.far):
             traffic = rec["hbm_bytes_per_launch"]
     roof = None
     if flops_alg is not None:
@@ -465,7 +471,8 @@ def main():
                 "flops_per_launch": flops_alg, "kernel_ms": eval_ms,
                 "list_entries": {"cells": info["list_cells"], "particles": info["list_parts"]},
                 "executed_interactions": {"cells": info["exec_cells"],
-                                          "particles": info["exec_parts"]},
+                                          "particles": info["exec_parts"],
+                                          "far_cells_m2l": info["far_cells"]},
                 "executed_tflops": (FLOPS_M2P[info["multipole"]] * info["exec_cells"]
                                     + FLOPS_P2P * info["exec_parts"]) / (eval_ms * 1e-3) / 1e12,
                 "bunches": info["ngroups"],
