@@ -610,7 +610,7 @@ __device__ __forceinline__ void p2p32(const float4 e, float tx, float ty, float 
 // four 16-lane rows, then row broadcasts, in a fixed order: the rounding does
 // not depend on timing, so every kernel that calls it on the same values gets
 // the same bits.  All 64 lanes must be active.
-__device__ __forceinline__ float wave_sum(float v) {
+__device__ __forceinline__ float wave_sum_dpp(float v) {
     int x = __float_as_int(v);
     x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false)));
     x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false)));
@@ -1239,7 +1239,7 @@ __global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
         far_m2l<MP>(node, ccx, ccy, ccz, mom, b.bc, b.h, eps2, L);
     }
 #pragma unroll
-    for (int j = 0; j < NLOC; ++j) L[j] = wave_sum(L[j]);
+    for (int j = 0; j < NLOC; ++j) L[j] = wave_sum_dpp(L[j]);
     float a[3];
     local_l2p(L, b.tx * b.hinv, b.ty * b.hinv, b.tz * b.hinv, a);
     if (lane < b.nb) {
