@@ -45,6 +45,7 @@ struct hq_tree {
     hq::DevBuf ndepth, ntype, nfirst, ncount, ncstart, nnext, nparent, ndigit;
     hq::DevBuf nmass, ncomx, ncomy, ncomz, nquad, noct, ncx, ncy, ncz, nkhi, nklo;
     hq::DevBuf mom32;  // float moment records for the mixed-precision walk
+    hq::DevBuf wnode;  // packed per-node records for the mixed walk's traversal
     int level_start[hq::MAX_DEPTH + 2] = {};
     int most_depth = 0;
     int64_t lost = 0;

@@ -498,12 +498,34 @@ __global__ __launch_bounds__(64) void k_eval_fp64(NodeView nv, const double* __r
 // traceless one is not.
 constexpr int REC = 24;
 
+// The per-node data the mixed walk's traversal reads, in one 48-byte record
+// (three 16-byte loads per candidate node instead of nine scattered ones).
+struct __align__(16) WalkNode {
+    double cx, cy, cz, mass;   // centre of mass, mass
+    int first, count, cstart;  // bodies [first, first + count), first child
+    int meta;                  // children | type << 4 | depth << 12
+};
+
 __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeView nv,
                                                float* __restrict__ rec,
+                                               WalkNode* __restrict__ wn,
                                                const int64_t* __restrict__ dnn) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (dnn) nn = *dnn;  // graph mode: the count is on the device
     if (k >= nn) return;
+    {
+        WalkNode w;
+        w.cx = nv.cx[k];
+        w.cy = nv.cy[k];
+        w.cz = nv.cz[k];
+        w.mass = nv.mass[k];
+        w.first = nv.first[k];
+        w.count = nv.count[k];
+        w.cstart = nv.cstart[k];
+        const int nch = nv.cstart[k + 1] - w.cstart;
+        w.meta = (nch & 15) | ((int)(uint8_t)nv.type[k] << 4) | ((int)nv.depth[k] << 12);
+        wn[k] = w;
+    }
     const int64_t S = nv.stride;
     float r[REC];
 #pragma unroll
@@ -879,6 +901,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    double theta2, double G, float eps2, int lbucket,
                                                    float far2, int* __restrict__ flist,
                                                    int* __restrict__ fnum, int fcap,
+                                                   const WalkNode* __restrict__ wn,
                                                    const float* __restrict__ mom,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz,
@@ -943,27 +966,36 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         const int base = top - take;
         const int k = lane < take ? stk[base + lane] : -1;
         top = base;
-        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell, 4 bucket
+        int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell, 4 bucket, 5 far
         int bfirst = 0, bcount = 0;
+        double cc[3] = {0.0, 0.0, 0.0};
+        int kfirst = 0, kcs = 0, knch = 0;
         if (k >= 0) {
-            const double mass = nv.mass[k];
-            if (mass != 0.0) {
-                const int f = nv.first[k];
-                const int l = f + nv.count[k] - 1;
-                const bool leaf = nv.type[k] == 1;
+            const double2 w0 = reinterpret_cast<const double2*>(wn + k)[0];
+            const double2 w2 = reinterpret_cast<const double2*>(wn + k)[1];
+            const int4 w1 = reinterpret_cast<const int4*>(wn + k)[2];
+            cc[0] = w0.x;
+            cc[1] = w0.y;
+            cc[2] = w2.x;
+            kfirst = w1.x;
+            kcs = w1.z;
+            knch = w1.w & 15;
+            if (w2.y != 0.0) {
+                const int f = w1.x;
+                const int l = f + w1.y - 1;
+                const bool leaf = ((w1.w >> 4) & 0xff) == 1;
                 if (f <= gl && l >= gf) {
                     action = leaf ? 0 : 1;
                 } else if (leaf) {
                     action = 2;
                 } else {
-                    const double cc[3] = {nv.cx[k], nv.cy[k], nv.cz[k]};
                     double dist2 = 0.0;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const double dd = fmax(fmax(bmin[c] - cc[c], 0.0), cc[c] - bmax[c]);
                         dist2 = fma(dd, dd, dist2);
                     }
-                    const int depth = nv.depth[k];
+                    const int depth = w1.w >> 12;
                     action = (gl2[depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
                     if (action == 3) {
                         double R2 = 0.0;
@@ -981,7 +1013,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         }
         const uint64_t mp = __ballot(action == 2);
         if (action == 2) {
-            const int p = nv.first[k];
+            const int p = kfirst;
             pring[(phead + pcnt + __popcll(mp & lt)) & (RING - 1)] =
                 make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]), (float)(sz[p] - bc[2]),
                             (float)sm[p]);
@@ -1024,11 +1056,11 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         const uint64_t mc = __ballot(action == 3);
         if (action == 3)
             cring[(chead + ccnt + __popcll(mc & lt)) & (RING - 1)] =
-                make_float4((float)(nv.cx[k] - bc[0]), (float)(nv.cy[k] - bc[1]),
-                            (float)(nv.cz[k] - bc[2]), __int_as_float(k));
+                make_float4((float)(cc[0] - bc[0]), (float)(cc[1] - bc[1]),
+                            (float)(cc[2] - bc[2]), __int_as_float(k));
         ccnt += __popcll(mc);
-        const int cs = action == 1 ? nv.cstart[k] : 0;
-        const int nch = action == 1 ? nv.cstart[k + 1] - cs : 0;
+        const int cs = action == 1 ? kcs : 0;
+        const int nch = action == 1 ? knch : 0;
         const int incl = wave_incl_scan(nch);
         const int tot = __shfl(incl, 63, 64);
         if (top + tot > SC) {
@@ -1501,7 +1533,8 @@ static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta
                                  t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                                  t->bstart.as<int>(), grid, theta2, G, (float)eps2, t->leaf,
                                  far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
-                                 t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
+                                 t->wnode.as<WalkNode>(), t->mom32.as<float>(),
+                                 t->fx.as<double>(), t->fy.as<double>(),
                                  t->fz.as<double>(), t->status.as<int64_t>(), dnbun, so);
     };
     auto eval = [&](auto kern) {
@@ -1533,6 +1566,7 @@ int graph_prepare(hq_tree* t) {
         (rc = t->gflag.ensure(((size_t)n + 1) * sizeof(int))) ||
         (rc = t->groups.ensure(((size_t)n + 1) * sizeof(int))) ||
         (rc = t->mom32.ensure((size_t)t->cap * REC * sizeof(float))) ||
+        (rc = t->wnode.ensure((size_t)t->cap * sizeof(WalkNode))) ||
         (rc = t->scanpart.ensure(scan_part_size(2 * n + 65536) * sizeof(uint32_t))))
         return rc;
     t->graph_split = use_split(t, t->graph_nbun);
@@ -1556,8 +1590,10 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         if (!mixed || !t->fused) return fail(HQ_ERR_ARGS, "graph mode needs the fused mixed walk");
         int64_t* gs = t->gstate.as<int64_t>();
         if ((rc = t->mom32.ensure((size_t)t->cap * REC * sizeof(float)))) return rc;
+        if ((rc = t->wnode.ensure((size_t)t->cap * sizeof(WalkNode)))) return rc;
         k_mom32<<<grid_for(t->cap, 256), 256, 0, s>>>(t->cap, t->multipole, node_view(t),
-                                                      t->mom32.as<float>(), gs + GS_NN);
+                                                      t->mom32.as<float>(),
+                                                      t->wnode.as<WalkNode>(), gs + GS_NN);
         k_fill_i64<<<1, 64, 0, s>>>(t->status.as<int64_t>(), 0, ST_SIZE);
         if (t->graph_split) {
             if ((rc = launch_split(t, (int)t->graph_nbun, gs + GS_NBUN, theta2, G, eps2))) return rc;
@@ -1568,7 +1604,8 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                     t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                     t->bstart.as<int>(), (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf,
                     far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
-                    t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
+                    t->wnode.as<WalkNode>(), t->mom32.as<float>(), t->fx.as<double>(),
+                    t->fy.as<double>(),
                     t->fz.as<double>(), t->status.as<int64_t>(), gs + GS_NBUN, SplitOut{});
             };
             if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
@@ -1582,8 +1619,10 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
     }
     if (mixed) {
         if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
+        if ((rc = t->wnode.ensure((size_t)t->nn * sizeof(WalkNode)))) return rc;
         k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
-                                                     t->mom32.as<float>(), nullptr);
+                                                     t->mom32.as<float>(),
+                                                     t->wnode.as<WalkNode>(), nullptr);
         if (t->fused && use_split(t, ngroups)) {
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
@@ -1620,7 +1659,8 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
                                             theta2, G, (float)eps2, t->leaf, far2(t),
                                             t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
-                                            t->mom32.as<float>(), t->fx.as<double>(), t->fy.as<double>(),
+                                            t->wnode.as<WalkNode>(), t->mom32.as<float>(),
+                                            t->fx.as<double>(), t->fy.as<double>(),
                                             t->fz.as<double>(), t->status.as<int64_t>(), nullptr,
                                             SplitOut{});
             };
