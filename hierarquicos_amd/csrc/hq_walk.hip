@@ -643,6 +643,18 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
+// Inclusive prefix sum of v over the wavefront (lane l gets the sum of lanes
+// 0..l), by the same row shifts and broadcasts.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_incl_scan_dpp(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
 // Radius of a bunch for its far-field expansion: the largest distance of its
 // bodies (lanes < nb) from the bunch centre, in the FP32 offsets the walk
 // uses, at least 1e-30.
@@ -1061,8 +1073,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         ccnt += __popcll(mc);
         const int cs = action == 1 ? kcs : 0;
         const int nch = action == 1 ? knch : 0;
-        const int incl = wave_incl_scan(nch);
-        const int tot = __shfl(incl, 63, 64);
+        const int incl = wave_incl_scan_dpp(nch);
+        const int tot = __builtin_amdgcn_readlane(incl, 63);
         if (top + tot > SC) {
             overflow = true;
             break;
