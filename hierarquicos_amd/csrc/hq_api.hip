@@ -299,6 +299,7 @@ int hq_tree_fmm(hq_tree* t, int order, double theta2, double G, double eps2, int
 
 int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {
     if (!t || !info) return fail(HQ_ERR_ARGS, "null argument");
+    collect_build_times(const_cast<hq_tree*>(t));  // the last build's phase times, if pending
     memset(info, 0, sizeof(*info));
     info->n = t->n;
     info->nnodes = t->nn;

@@ -799,10 +799,22 @@ static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
     return HQ_OK;
 }
 
+void collect_build_times(hq_tree* t) {
+    if (!t->build_timed || capturing()) return;
+    t->build_timed = false;
+    if (hipEventSynchronize(t->ev[4]) != hipSuccess) return;
+    hipEventElapsedTime(&t->ms_keys, t->ev[0], t->ev[1]);
+    hipEventElapsedTime(&t->ms_sort, t->ev[1], t->ev[2]);
+    hipEventElapsedTime(&t->ms_nodes, t->ev[2], t->ev[3]);
+    hipEventElapsedTime(&t->ms_upward, t->ev[3], t->ev[4]);
+    hipEventElapsedTime(&t->ms_build, t->ev[0], t->ev[4]);
+}
+
 int read_status(hq_tree* t) {
     HQ_CHECK_HIP(hipMemcpyAsync(t->hstatus, t->status.p, ST_SIZE * sizeof(int64_t),
                                 hipMemcpyDeviceToHost, t->stream));
     HQ_CHECK_HIP(hipStreamSynchronize(t->stream));
+    collect_build_times(t);
     return HQ_OK;
 }
 
@@ -861,6 +873,8 @@ static int radix_sort(hq_tree* t) {
 }
 
 int tree_build(hq_tree* t) {
+    // the previous build's phase times, before this build records its events
+    if (!capturing()) collect_build_times(t);
     const int64_t n = t->n;
     const int nsys = t->nsys;
     const bool batch = nsys > 1;
@@ -999,13 +1013,9 @@ int tree_build(hq_tree* t) {
         if (b > a) k_upward<<<grid_for(b - a, 128), 128, 0, s>>>(a, b, d, ua);
     }
     HQ_CHECK_HIP(hipEventRecord(t->ev[4], s));
-    HQ_CHECK_HIP(hipStreamSynchronize(s));
     HQ_CHECK_HIP(hipGetLastError());
-    hipEventElapsedTime(&t->ms_keys, t->ev[0], t->ev[1]);
-    hipEventElapsedTime(&t->ms_sort, t->ev[1], t->ev[2]);
-    hipEventElapsedTime(&t->ms_nodes, t->ev[2], t->ev[3]);
-    hipEventElapsedTime(&t->ms_upward, t->ev[3], t->ev[4]);
-    hipEventElapsedTime(&t->ms_build, t->ev[0], t->ev[4]);
+    // no wait here: the phase times are read at the next synchronisation
+    t->build_timed = true;
     t->built = true;
     return HQ_OK;
 }

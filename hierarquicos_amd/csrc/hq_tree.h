@@ -57,6 +57,7 @@ struct hq_tree {
     std::unique_ptr<hq::FmmState, hq::FmmFree> fmm;
     bool fused = true;     // mixed bunched walk as one fused kernel
     int split = 1;         // split evaluation of the fused walk: 0 never, 1 few bunches, 2 always
+    bool build_timed = false;   // the last launched build's events await reading
     int far = 50;          // far-field factor of the fused mixed walk, in tenths (0: off)
     int64_t far_cells = 0; // cells of the last walk taken through the far-field expansion
     int64_t split_lcap = 0;  // log entries per bunch (from the last walk's lists; 0 unknown)
@@ -132,6 +133,58 @@ enum { GF_INPUT = 1, GF_CAPACITY = 2, GF_TIES = 4, GF_ROOTS = 8, GF_LOST = 16, G
        GF_BUNCHES = 64 };
 
 namespace hq {
+// The mixed walk's per-node records (written by k_mom32 before each mixed walk):
+//  - FP32 moments, REC floats, pre-scaled for the M2P: r[0] M; r[1..6] -3 Q
+//    (xx, yy, zz, xy, xz, yz, raw second moments); r[7] -1.5 trQ; r[8..10]
+//    -1.5 T, T_i = sum_j O_ijj; r[11..23] 7.5 x (O1..O9, O10, 2 O10, 2 O8,
+//    2 O6) with O1..O10 in octree_default.f90 :831-843 order (xxx, xyy, xzz,
+//    xxy, yyy, yzz, xxz, yyz, zzz, xyz).  Raw (not traceless) moments: with
+//    softening only they give the exact Taylor series of the softened
+//    potential.
+//  - WalkNode: what the traversal reads of a candidate node, 48 bytes.
+constexpr int REC = 24;
+struct __align__(16) WalkNode {
+    double cx, cy, cz, mass;   // centre of mass, mass
+    int first, count, cstart;  // bodies [first, first + count), first child
+    int meta;                  // children | type << 4 | depth << 12
+};
+
+__device__ __forceinline__ void store_walk_records(int64_t k, int multipole, double mass,
+                                                   double cx, double cy, double cz, int first,
+                                                   int count, int cstart, int nch, int type,
+                                                   int depth, const double* Q, const double* O,
+                                                   float* __restrict__ rec,
+                                                   WalkNode* __restrict__ wn) {
+    float r[REC];
+#pragma unroll
+    for (int j = 0; j < REC; ++j) r[j] = 0.0f;
+    r[0] = (float)mass;
+    if (multipole >= 4) {
+        for (int j = 0; j < 6; ++j) r[1 + j] = (float)(-3.0 * Q[j]);
+        r[7] = (float)(-1.5 * (Q[0] + Q[1] + Q[2]));
+    }
+    if (multipole > 4) {
+        for (int j = 0; j < 3; ++j) r[8 + j] = (float)(-1.5 * O[11 + j]);
+        for (int j = 0; j < 10; ++j) r[11 + j] = (float)(7.5 * O[j]);
+        r[21] = (float)(15.0 * O[9]);
+        r[22] = (float)(15.0 * O[7]);
+        r[23] = (float)(15.0 * O[5]);
+    }
+    float4* o = reinterpret_cast<float4*>(rec + k * REC);
+#pragma unroll
+    for (int j = 0; j < REC / 4; ++j) o[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+    WalkNode w;
+    w.cx = cx;
+    w.cy = cy;
+    w.cz = cz;
+    w.mass = mass;
+    w.first = first;
+    w.count = count;
+    w.cstart = cstart;
+    w.meta = (nch & 15) | ((type & 0xff) << 4) | (depth << 12);
+    wn[k] = w;
+}
+
 // set while a stream capture is open: buffers must not grow then
 extern thread_local bool g_capturing;
 __global__ void k_fill_i64(int64_t* __restrict__ p, int64_t v, int64_t n);
@@ -143,6 +196,8 @@ int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2);
 int tree_forces_fmm(hq_tree* t, int order, double theta, double G, double eps2, bool mfma, int S);
 int tree_scatter_forces(hq_tree* t, double* F, int layout, int where);
 int read_status(hq_tree* t);
+// the last launched build's phase times from its events (the stream has passed them)
+void collect_build_times(hq_tree* t);
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
                      DevBuf& hist, DevBuf& part, hipStream_t s);
 int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s);

@@ -485,27 +485,8 @@ __global__ __launch_bounds__(64) void k_eval_fp64(NodeView nv, const double* __r
 }
 
 // ------------------------------------------------------------------ mixed evaluation
-// Node moments as FP32 records of 24 floats, pre-scaled for the M2P below:
-//   r[0]      M
-//   r[1..6]   Q~ = -3 (Qxx, Qyy, Qzz, Qxy, Qxz, Qyz)   (raw second moments)
-//   r[7]      tq = -1.5 trQ
-//   r[8..10]  T~ = -1.5 T, T_i = sum_j O_ijj   (octupole only)
-//   r[11..23] O~ = 7.5 x (O1..O9, O10, 2 O10, 2 O8, 2 O6)  (octupole only;
-//             O1..O10 in octree_default.f90 :831-843 order: xxx, xyy, xzz,
-//             xxy, yyy, yzz, xxz, yyz, zzz, xyz)
-// The moments are the raw (not traceless) ones: with softening the raw
-// expansion is the exact Taylor series of the softened potential, which the
-// traceless one is not.
-constexpr int REC = 24;
-
-// The per-node data the mixed walk's traversal reads, in one 48-byte record
-// (three 16-byte loads per candidate node instead of nine scattered ones).
-struct __align__(16) WalkNode {
-    double cx, cy, cz, mass;   // centre of mass, mass
-    int first, count, cstart;  // bodies [first, first + count), first child
-    int meta;                  // children | type << 4 | depth << 12
-};
-
+// The FP32 moment records and the packed traversal records (REC, WalkNode,
+// store_walk_records in hq_tree.h), from the node store, before each mixed walk.
 __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeView nv,
                                                float* __restrict__ rec,
                                                WalkNode* __restrict__ wn,
@@ -513,42 +494,16 @@ __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeVi
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (dnn) nn = *dnn;  // graph mode: the count is on the device
     if (k >= nn) return;
-    {
-        WalkNode w;
-        w.cx = nv.cx[k];
-        w.cy = nv.cy[k];
-        w.cz = nv.cz[k];
-        w.mass = nv.mass[k];
-        w.first = nv.first[k];
-        w.count = nv.count[k];
-        w.cstart = nv.cstart[k];
-        const int nch = nv.cstart[k + 1] - w.cstart;
-        w.meta = (nch & 15) | ((int)(uint8_t)nv.type[k] << 4) | ((int)nv.depth[k] << 12);
-        wn[k] = w;
-    }
     const int64_t S = nv.stride;
-    float r[REC];
-#pragma unroll
-    for (int j = 0; j < REC; ++j) r[j] = 0.0f;
-    r[0] = (float)nv.mass[k];
-    if (multipole >= 4) {
-        double q[6];
-        for (int j = 0; j < 6; ++j) q[j] = nv.quad[j * S + k];
-        for (int j = 0; j < 6; ++j) r[1 + j] = (float)(-3.0 * q[j]);
-        r[7] = (float)(-1.5 * (q[0] + q[1] + q[2]));
-    }
-    if (multipole > 4) {
-        double o[14];
-        for (int j = 0; j < 14; ++j) o[j] = nv.oct[j * S + k];
-        for (int j = 0; j < 3; ++j) r[8 + j] = (float)(-1.5 * o[11 + j]);
-        for (int j = 0; j < 10; ++j) r[11 + j] = (float)(7.5 * o[j]);
-        r[21] = (float)(15.0 * o[9]);
-        r[22] = (float)(15.0 * o[7]);
-        r[23] = (float)(15.0 * o[5]);
-    }
-    float4* o = reinterpret_cast<float4*>(rec + k * REC);
-#pragma unroll
-    for (int j = 0; j < REC / 4; ++j) o[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+    double Q[6] = {0, 0, 0, 0, 0, 0}, O[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (multipole >= 4)
+        for (int j = 0; j < 6; ++j) Q[j] = nv.quad[j * S + k];
+    if (multipole > 4)
+        for (int j = 0; j < 14; ++j) O[j] = nv.oct[j * S + k];
+    const int cs = nv.cstart[k];
+    store_walk_records(k, multipole, nv.mass[k], nv.cx[k], nv.cy[k], nv.cz[k], nv.first[k],
+                       nv.count[k], cs, nv.cstart[k + 1] - cs, (int)(uint8_t)nv.type[k],
+                       (int)nv.depth[k], Q, O, rec, wn);
 }
 
 struct Kahan3 {
