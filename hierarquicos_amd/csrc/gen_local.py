@@ -118,7 +118,6 @@ def render(lines, lang):
                         lambda m: "MP %s ? %s : 0.0f;" % (">= 4" if m.group(1) == "4" else "== 8",
                                                           m.group(2)), ln)
             ln = ln.replace("RSQRT(", "__frsqrt_rn(")
-            ln = ln.replace("OUT(", "out(")
         out.append(ln)
     return out
 
@@ -132,11 +131,11 @@ def gen_m2l():
     w = out.append
     w("// Local coefficients of one source cell (this lane's) about the bunch centre.")
     w("// R = bunch centre - cell centre of mass, h = bunch radius (> 0), r = the")
-    w("// cell's FP32 record; out(i, h^|n| L_n) for the multi-index n = LOC_n[i], each")
-    w("// coefficient as soon as it is formed.")
-    w("template <int MP, class Out>")
+    w("// cell's FP32 record; adds h^|n| L_n to L[i] for the multi-index n = LOC_n[i]")
+    w("// (the addition folded into each coefficient's last multiply-add).")
+    w("template <int MP>")
     w("__device__ __forceinline__ void local_m2l(float Rx, float Ry, float Rz, float eps2, float h,")
-    w("                                          const float* __restrict__ r, Out&& out) {")
+    w("                                          const float* __restrict__ r, float* __restrict__ L) {")
     w("#pragma clang fp contract(off)")
     w("    const float rho = fmaf(Rx, Rx, fmaf(Ry, Ry, fmaf(Rz, Rz, eps2)));")
     w("    const float ri = RSQRT(rho);")
@@ -222,14 +221,14 @@ def gen_m2l():
         if o <= POCT:
             for m in OCT:
                 parts.append(("P%d%d%d" % m, "D%d%d%d" % add(n, m), "o"))
-        mono_term = "t%d * D%d%d%d" % ((o,) + n)
+        mono_term = "fmaf(t%d, D%d%d%d, L[%d])" % ((o,) + n + (idx,))
         if not parts:
-            w("    OUT(%d, %s);" % (idx, mono_term))
+            w("    L[%d] = %s;" % (idx, mono_term))
             continue
         expr = None
         for p, d, kind in parts:
             expr = "%s * %s" % (p, d) if expr is None else "fmaf(%s, %s, %s)" % (p, d, expr)
-        w("    OUT(%d, fmaf(s%d, %s, %s));" % (idx, o, expr, mono_term))
+        w("    L[%d] = fmaf(s%d, %s, %s);" % (idx, o, expr, mono_term))
     w("}")
     return out
 
@@ -293,7 +292,6 @@ def python_source():
                     or t.startswith("float ") or t == "}":
                 continue
             t = t.replace("const float ", "").rstrip(";")
-            t = re.sub(r"^OUT\((\d+), (.*)\)$", r"L[\1] = \2", t)
             t = re.sub(r"@([^@]+)@", r"\1", t)
             t = re.sub(r"SEL(\d)\((.*)\)$",
                        lambda m: "(%s) if MP %s else 0.0" % (m.group(2), ">= 4" if m.group(1) == "4"

@@ -8,11 +8,11 @@ constexpr int NLOC = 55;  // local coefficients, 1 <= |n| <= 5
 
 // Local coefficients of one source cell (this lane's) about the bunch centre.
 // R = bunch centre - cell centre of mass, h = bunch radius (> 0), r = the
-// cell's FP32 record; out(i, h^|n| L_n) for the multi-index n = LOC_n[i], each
-// coefficient as soon as it is formed.
-template <int MP, class Out>
+// cell's FP32 record; adds h^|n| L_n to L[i] for the multi-index n = LOC_n[i]
+// (the addition folded into each coefficient's last multiply-add).
+template <int MP>
 __device__ __forceinline__ void local_m2l(float Rx, float Ry, float Rz, float eps2, float h,
-                                          const float* __restrict__ r, Out&& out) {
+                                          const float* __restrict__ r, float* __restrict__ L) {
 #pragma clang fp contract(off)
     const float rho = fmaf(Rx, Rx, fmaf(Ry, Ry, fmaf(Rz, Rz, eps2)));
     const float ri = __frsqrt_rn(rho);
@@ -162,61 +162,61 @@ __device__ __forceinline__ void local_m2l(float Rx, float Ry, float Rz, float ep
     const float t3 = t2 * sg;
     const float t4 = t3 * sg;
     const float t5 = t4 * sg;
-    out(0, fmaf(s1, fmaf(P111, D211, fmaf(P003, D103, fmaf(P021, D121, fmaf(P201, D301, fmaf(P012, D112, fmaf(P030, D130, fmaf(P210, D310, fmaf(P102, D202, fmaf(P120, D220, fmaf(P300, D400, fmaf(P011, D111, fmaf(P101, D201, fmaf(P110, D210, fmaf(P002, D102, fmaf(P020, D120, P200 * D300))))))))))))))), t1 * D100));
-    out(1, fmaf(s1, fmaf(P111, D121, fmaf(P003, D013, fmaf(P021, D031, fmaf(P201, D211, fmaf(P012, D022, fmaf(P030, D040, fmaf(P210, D220, fmaf(P102, D112, fmaf(P120, D130, fmaf(P300, D310, fmaf(P011, D021, fmaf(P101, D111, fmaf(P110, D120, fmaf(P002, D012, fmaf(P020, D030, P200 * D210))))))))))))))), t1 * D010));
-    out(2, fmaf(s1, fmaf(P111, D112, fmaf(P003, D004, fmaf(P021, D022, fmaf(P201, D202, fmaf(P012, D013, fmaf(P030, D031, fmaf(P210, D211, fmaf(P102, D103, fmaf(P120, D121, fmaf(P300, D301, fmaf(P011, D012, fmaf(P101, D102, fmaf(P110, D111, fmaf(P002, D003, fmaf(P020, D021, P200 * D201))))))))))))))), t1 * D001));
-    out(3, fmaf(s2, fmaf(P111, D311, fmaf(P003, D203, fmaf(P021, D221, fmaf(P201, D401, fmaf(P012, D212, fmaf(P030, D230, fmaf(P210, D410, fmaf(P102, D302, fmaf(P120, D320, fmaf(P300, D500, fmaf(P011, D211, fmaf(P101, D301, fmaf(P110, D310, fmaf(P002, D202, fmaf(P020, D220, P200 * D400))))))))))))))), t2 * D200));
-    out(4, fmaf(s2, fmaf(P111, D221, fmaf(P003, D113, fmaf(P021, D131, fmaf(P201, D311, fmaf(P012, D122, fmaf(P030, D140, fmaf(P210, D320, fmaf(P102, D212, fmaf(P120, D230, fmaf(P300, D410, fmaf(P011, D121, fmaf(P101, D211, fmaf(P110, D220, fmaf(P002, D112, fmaf(P020, D130, P200 * D310))))))))))))))), t2 * D110));
-    out(5, fmaf(s2, fmaf(P111, D212, fmaf(P003, D104, fmaf(P021, D122, fmaf(P201, D302, fmaf(P012, D113, fmaf(P030, D131, fmaf(P210, D311, fmaf(P102, D203, fmaf(P120, D221, fmaf(P300, D401, fmaf(P011, D112, fmaf(P101, D202, fmaf(P110, D211, fmaf(P002, D103, fmaf(P020, D121, P200 * D301))))))))))))))), t2 * D101));
-    out(6, fmaf(s2, fmaf(P111, D131, fmaf(P003, D023, fmaf(P021, D041, fmaf(P201, D221, fmaf(P012, D032, fmaf(P030, D050, fmaf(P210, D230, fmaf(P102, D122, fmaf(P120, D140, fmaf(P300, D320, fmaf(P011, D031, fmaf(P101, D121, fmaf(P110, D130, fmaf(P002, D022, fmaf(P020, D040, P200 * D220))))))))))))))), t2 * D020));
-    out(7, fmaf(s2, fmaf(P111, D122, fmaf(P003, D014, fmaf(P021, D032, fmaf(P201, D212, fmaf(P012, D023, fmaf(P030, D041, fmaf(P210, D221, fmaf(P102, D113, fmaf(P120, D131, fmaf(P300, D311, fmaf(P011, D022, fmaf(P101, D112, fmaf(P110, D121, fmaf(P002, D013, fmaf(P020, D031, P200 * D211))))))))))))))), t2 * D011));
-    out(8, fmaf(s2, fmaf(P111, D113, fmaf(P003, D005, fmaf(P021, D023, fmaf(P201, D203, fmaf(P012, D014, fmaf(P030, D032, fmaf(P210, D212, fmaf(P102, D104, fmaf(P120, D122, fmaf(P300, D302, fmaf(P011, D013, fmaf(P101, D103, fmaf(P110, D112, fmaf(P002, D004, fmaf(P020, D022, P200 * D202))))))))))))))), t2 * D002));
-    out(9, fmaf(s3, fmaf(P011, D311, fmaf(P101, D401, fmaf(P110, D410, fmaf(P002, D302, fmaf(P020, D320, P200 * D500))))), t3 * D300));
-    out(10, fmaf(s3, fmaf(P011, D221, fmaf(P101, D311, fmaf(P110, D320, fmaf(P002, D212, fmaf(P020, D230, P200 * D410))))), t3 * D210));
-    out(11, fmaf(s3, fmaf(P011, D212, fmaf(P101, D302, fmaf(P110, D311, fmaf(P002, D203, fmaf(P020, D221, P200 * D401))))), t3 * D201));
-    out(12, fmaf(s3, fmaf(P011, D131, fmaf(P101, D221, fmaf(P110, D230, fmaf(P002, D122, fmaf(P020, D140, P200 * D320))))), t3 * D120));
-    out(13, fmaf(s3, fmaf(P011, D122, fmaf(P101, D212, fmaf(P110, D221, fmaf(P002, D113, fmaf(P020, D131, P200 * D311))))), t3 * D111));
-    out(14, fmaf(s3, fmaf(P011, D113, fmaf(P101, D203, fmaf(P110, D212, fmaf(P002, D104, fmaf(P020, D122, P200 * D302))))), t3 * D102));
-    out(15, fmaf(s3, fmaf(P011, D041, fmaf(P101, D131, fmaf(P110, D140, fmaf(P002, D032, fmaf(P020, D050, P200 * D230))))), t3 * D030));
-    out(16, fmaf(s3, fmaf(P011, D032, fmaf(P101, D122, fmaf(P110, D131, fmaf(P002, D023, fmaf(P020, D041, P200 * D221))))), t3 * D021));
-    out(17, fmaf(s3, fmaf(P011, D023, fmaf(P101, D113, fmaf(P110, D122, fmaf(P002, D014, fmaf(P020, D032, P200 * D212))))), t3 * D012));
-    out(18, fmaf(s3, fmaf(P011, D014, fmaf(P101, D104, fmaf(P110, D113, fmaf(P002, D005, fmaf(P020, D023, P200 * D203))))), t3 * D003));
-    out(19, t4 * D400);
-    out(20, t4 * D310);
-    out(21, t4 * D301);
-    out(22, t4 * D220);
-    out(23, t4 * D211);
-    out(24, t4 * D202);
-    out(25, t4 * D130);
-    out(26, t4 * D121);
-    out(27, t4 * D112);
-    out(28, t4 * D103);
-    out(29, t4 * D040);
-    out(30, t4 * D031);
-    out(31, t4 * D022);
-    out(32, t4 * D013);
-    out(33, t4 * D004);
-    out(34, t5 * D500);
-    out(35, t5 * D410);
-    out(36, t5 * D401);
-    out(37, t5 * D320);
-    out(38, t5 * D311);
-    out(39, t5 * D302);
-    out(40, t5 * D230);
-    out(41, t5 * D221);
-    out(42, t5 * D212);
-    out(43, t5 * D203);
-    out(44, t5 * D140);
-    out(45, t5 * D131);
-    out(46, t5 * D122);
-    out(47, t5 * D113);
-    out(48, t5 * D104);
-    out(49, t5 * D050);
-    out(50, t5 * D041);
-    out(51, t5 * D032);
-    out(52, t5 * D023);
-    out(53, t5 * D014);
-    out(54, t5 * D005);
+    L[0] = fmaf(s1, fmaf(P111, D211, fmaf(P003, D103, fmaf(P021, D121, fmaf(P201, D301, fmaf(P012, D112, fmaf(P030, D130, fmaf(P210, D310, fmaf(P102, D202, fmaf(P120, D220, fmaf(P300, D400, fmaf(P011, D111, fmaf(P101, D201, fmaf(P110, D210, fmaf(P002, D102, fmaf(P020, D120, P200 * D300))))))))))))))), fmaf(t1, D100, L[0]));
+    L[1] = fmaf(s1, fmaf(P111, D121, fmaf(P003, D013, fmaf(P021, D031, fmaf(P201, D211, fmaf(P012, D022, fmaf(P030, D040, fmaf(P210, D220, fmaf(P102, D112, fmaf(P120, D130, fmaf(P300, D310, fmaf(P011, D021, fmaf(P101, D111, fmaf(P110, D120, fmaf(P002, D012, fmaf(P020, D030, P200 * D210))))))))))))))), fmaf(t1, D010, L[1]));
+    L[2] = fmaf(s1, fmaf(P111, D112, fmaf(P003, D004, fmaf(P021, D022, fmaf(P201, D202, fmaf(P012, D013, fmaf(P030, D031, fmaf(P210, D211, fmaf(P102, D103, fmaf(P120, D121, fmaf(P300, D301, fmaf(P011, D012, fmaf(P101, D102, fmaf(P110, D111, fmaf(P002, D003, fmaf(P020, D021, P200 * D201))))))))))))))), fmaf(t1, D001, L[2]));
+    L[3] = fmaf(s2, fmaf(P111, D311, fmaf(P003, D203, fmaf(P021, D221, fmaf(P201, D401, fmaf(P012, D212, fmaf(P030, D230, fmaf(P210, D410, fmaf(P102, D302, fmaf(P120, D320, fmaf(P300, D500, fmaf(P011, D211, fmaf(P101, D301, fmaf(P110, D310, fmaf(P002, D202, fmaf(P020, D220, P200 * D400))))))))))))))), fmaf(t2, D200, L[3]));
+    L[4] = fmaf(s2, fmaf(P111, D221, fmaf(P003, D113, fmaf(P021, D131, fmaf(P201, D311, fmaf(P012, D122, fmaf(P030, D140, fmaf(P210, D320, fmaf(P102, D212, fmaf(P120, D230, fmaf(P300, D410, fmaf(P011, D121, fmaf(P101, D211, fmaf(P110, D220, fmaf(P002, D112, fmaf(P020, D130, P200 * D310))))))))))))))), fmaf(t2, D110, L[4]));
+    L[5] = fmaf(s2, fmaf(P111, D212, fmaf(P003, D104, fmaf(P021, D122, fmaf(P201, D302, fmaf(P012, D113, fmaf(P030, D131, fmaf(P210, D311, fmaf(P102, D203, fmaf(P120, D221, fmaf(P300, D401, fmaf(P011, D112, fmaf(P101, D202, fmaf(P110, D211, fmaf(P002, D103, fmaf(P020, D121, P200 * D301))))))))))))))), fmaf(t2, D101, L[5]));
+    L[6] = fmaf(s2, fmaf(P111, D131, fmaf(P003, D023, fmaf(P021, D041, fmaf(P201, D221, fmaf(P012, D032, fmaf(P030, D050, fmaf(P210, D230, fmaf(P102, D122, fmaf(P120, D140, fmaf(P300, D320, fmaf(P011, D031, fmaf(P101, D121, fmaf(P110, D130, fmaf(P002, D022, fmaf(P020, D040, P200 * D220))))))))))))))), fmaf(t2, D020, L[6]));
+    L[7] = fmaf(s2, fmaf(P111, D122, fmaf(P003, D014, fmaf(P021, D032, fmaf(P201, D212, fmaf(P012, D023, fmaf(P030, D041, fmaf(P210, D221, fmaf(P102, D113, fmaf(P120, D131, fmaf(P300, D311, fmaf(P011, D022, fmaf(P101, D112, fmaf(P110, D121, fmaf(P002, D013, fmaf(P020, D031, P200 * D211))))))))))))))), fmaf(t2, D011, L[7]));
+    L[8] = fmaf(s2, fmaf(P111, D113, fmaf(P003, D005, fmaf(P021, D023, fmaf(P201, D203, fmaf(P012, D014, fmaf(P030, D032, fmaf(P210, D212, fmaf(P102, D104, fmaf(P120, D122, fmaf(P300, D302, fmaf(P011, D013, fmaf(P101, D103, fmaf(P110, D112, fmaf(P002, D004, fmaf(P020, D022, P200 * D202))))))))))))))), fmaf(t2, D002, L[8]));
+    L[9] = fmaf(s3, fmaf(P011, D311, fmaf(P101, D401, fmaf(P110, D410, fmaf(P002, D302, fmaf(P020, D320, P200 * D500))))), fmaf(t3, D300, L[9]));
+    L[10] = fmaf(s3, fmaf(P011, D221, fmaf(P101, D311, fmaf(P110, D320, fmaf(P002, D212, fmaf(P020, D230, P200 * D410))))), fmaf(t3, D210, L[10]));
+    L[11] = fmaf(s3, fmaf(P011, D212, fmaf(P101, D302, fmaf(P110, D311, fmaf(P002, D203, fmaf(P020, D221, P200 * D401))))), fmaf(t3, D201, L[11]));
+    L[12] = fmaf(s3, fmaf(P011, D131, fmaf(P101, D221, fmaf(P110, D230, fmaf(P002, D122, fmaf(P020, D140, P200 * D320))))), fmaf(t3, D120, L[12]));
+    L[13] = fmaf(s3, fmaf(P011, D122, fmaf(P101, D212, fmaf(P110, D221, fmaf(P002, D113, fmaf(P020, D131, P200 * D311))))), fmaf(t3, D111, L[13]));
+    L[14] = fmaf(s3, fmaf(P011, D113, fmaf(P101, D203, fmaf(P110, D212, fmaf(P002, D104, fmaf(P020, D122, P200 * D302))))), fmaf(t3, D102, L[14]));
+    L[15] = fmaf(s3, fmaf(P011, D041, fmaf(P101, D131, fmaf(P110, D140, fmaf(P002, D032, fmaf(P020, D050, P200 * D230))))), fmaf(t3, D030, L[15]));
+    L[16] = fmaf(s3, fmaf(P011, D032, fmaf(P101, D122, fmaf(P110, D131, fmaf(P002, D023, fmaf(P020, D041, P200 * D221))))), fmaf(t3, D021, L[16]));
+    L[17] = fmaf(s3, fmaf(P011, D023, fmaf(P101, D113, fmaf(P110, D122, fmaf(P002, D014, fmaf(P020, D032, P200 * D212))))), fmaf(t3, D012, L[17]));
+    L[18] = fmaf(s3, fmaf(P011, D014, fmaf(P101, D104, fmaf(P110, D113, fmaf(P002, D005, fmaf(P020, D023, P200 * D203))))), fmaf(t3, D003, L[18]));
+    L[19] = fmaf(t4, D400, L[19]);
+    L[20] = fmaf(t4, D310, L[20]);
+    L[21] = fmaf(t4, D301, L[21]);
+    L[22] = fmaf(t4, D220, L[22]);
+    L[23] = fmaf(t4, D211, L[23]);
+    L[24] = fmaf(t4, D202, L[24]);
+    L[25] = fmaf(t4, D130, L[25]);
+    L[26] = fmaf(t4, D121, L[26]);
+    L[27] = fmaf(t4, D112, L[27]);
+    L[28] = fmaf(t4, D103, L[28]);
+    L[29] = fmaf(t4, D040, L[29]);
+    L[30] = fmaf(t4, D031, L[30]);
+    L[31] = fmaf(t4, D022, L[31]);
+    L[32] = fmaf(t4, D013, L[32]);
+    L[33] = fmaf(t4, D004, L[33]);
+    L[34] = fmaf(t5, D500, L[34]);
+    L[35] = fmaf(t5, D410, L[35]);
+    L[36] = fmaf(t5, D401, L[36]);
+    L[37] = fmaf(t5, D320, L[37]);
+    L[38] = fmaf(t5, D311, L[38]);
+    L[39] = fmaf(t5, D302, L[39]);
+    L[40] = fmaf(t5, D230, L[40]);
+    L[41] = fmaf(t5, D221, L[41]);
+    L[42] = fmaf(t5, D212, L[42]);
+    L[43] = fmaf(t5, D203, L[43]);
+    L[44] = fmaf(t5, D140, L[44]);
+    L[45] = fmaf(t5, D131, L[45]);
+    L[46] = fmaf(t5, D122, L[46]);
+    L[47] = fmaf(t5, D113, L[47]);
+    L[48] = fmaf(t5, D104, L[48]);
+    L[49] = fmaf(t5, D050, L[49]);
+    L[50] = fmaf(t5, D041, L[50]);
+    L[51] = fmaf(t5, D032, L[51]);
+    L[52] = fmaf(t5, D023, L[52]);
+    L[53] = fmaf(t5, D014, L[53]);
+    L[54] = fmaf(t5, D005, L[54]);
 }
 
 // Acceleration sum_n L_(n+e_i) u^n / n! at u = (target - bunch centre) / h from the

@@ -648,7 +648,7 @@ __device__ __forceinline__ void far_m2l(int node, const double* __restrict__ cx,
             r[4 * q + 3] = v4.w;
         }
     }
-    local_m2l<MP>(Rx, Ry, Rz, eps2, h, r, [&](int j, float v) { L[j] = L[j] + v; });
+    local_m2l<MP>(Rx, Ry, Rz, eps2, h, r, L);
 }
 
 constexpr int KBLK = 32;  // interactions per compensated partial sum
