@@ -352,12 +352,14 @@ __global__ __launch_bounds__(256) void k_gather(const int* __restrict__ idx,
                                                 double* __restrict__ sx, double* __restrict__ sy,
                                                 double* __restrict__ sz, double* __restrict__ sm,
                                                 uint32_t* __restrict__ sklo,
-                                                int* __restrict__ ssys) {
+                                                int* __restrict__ ssys,
+                                                double4* __restrict__ sbody) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = idx[i];
     const double4 r = rec[p];
     const uint2 a = aux[p];
+    sbody[i] = r;
     sx[i] = r.x;
     sy[i] = r.y;
     sz[i] = r.z;
@@ -882,7 +884,8 @@ int tree_build(hq_tree* t) {
     size_t n8 = n * sizeof(double);
     if ((rc = t->geom.ensure((G_SYS + (size_t)SG * nsys) * sizeof(double)))) return rc;
     if ((rc = t->khi.ensure(n * 8)) || (rc = t->klo.ensure(n * 4)) || (rc = t->idx.ensure(n * 4)) ||
-        (rc = t->prec.ensure(n * 32)) || (rc = t->paux.ensure(n * 8)))
+        (rc = t->prec.ensure(n * 32)) || (rc = t->paux.ensure(n * 8)) ||
+        (rc = t->sbody.ensure(n * 32)))
         return rc;
     if ((rc = t->sx.ensure(n8)) || (rc = t->sy.ensure(n8)) || (rc = t->sz.ensure(n8)) ||
         (rc = t->sm.ensure(n8)) || (rc = t->sklo.ensure(n * 4)) || (rc = t->lcp.ensure(n)) ||
@@ -965,7 +968,7 @@ int tree_build(hq_tree* t) {
                                               t->paux.as<uint2>(), n, t->sx.as<double>(),
                                               t->sy.as<double>(), t->sz.as<double>(),
                                               t->sm.as<double>(), t->sklo.as<uint32_t>(),
-                                              t->ssys.as<int>());
+                                              t->ssys.as<int>(), t->sbody.as<double4>());
     HQ_CHECK_HIP(mark(2));
     // 4. nodes
     k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),

@@ -40,6 +40,7 @@ struct hq_tree {
     hq::DevBuf prec, paux;  // packed (x, y, z, m) and (key_lo, system) per input particle
     // sorted particles (tree order)
     hq::DevBuf sx, sy, sz, sm, sklo, lcp;
+    hq::DevBuf sbody;  // sorted bodies as packed (x, y, z, m) records
     // node store (level order)
     int64_t nn = 0, cap = 0;
     hq::DevBuf ndepth, ntype, nfirst, ncount, ncstart, nnext, nparent, ndigit;

@@ -869,6 +869,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    float far2, int* __restrict__ flist,
                                                    int* __restrict__ fnum, int fcap,
                                                    const WalkNode* __restrict__ wn,
+                                                   const double4* __restrict__ sb,
                                                    const float* __restrict__ mom,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz,
@@ -980,10 +981,10 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         }
         const uint64_t mp = __ballot(action == 2);
         if (action == 2) {
-            const int p = kfirst;
+            const double4 bp = sb[kfirst];
             pring[(phead + pcnt + __popcll(mp & lt)) & (RING - 1)] =
-                make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]), (float)(sz[p] - bc[2]),
-                            (float)sm[p]);
+                make_float4((float)(bp.x - bc[0]), (float)(bp.y - bc[1]), (float)(bp.z - bc[2]),
+                            (float)bp.w);
         }
         pcnt += __popcll(mp);
         {
@@ -1002,10 +1003,10 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                     ntot_p += 64;
                 }
                 if (lane < c) {
-                    const int p = f + lane;
+                    const double4 bp = sb[f + lane];
                     pring[(phead + pcnt + lane) & (RING - 1)] =
-                        make_float4((float)(sx[p] - bc[0]), (float)(sy[p] - bc[1]),
-                                    (float)(sz[p] - bc[2]), (float)sm[p]);
+                        make_float4((float)(bp.x - bc[0]), (float)(bp.y - bc[1]),
+                                    (float)(bp.z - bc[2]), (float)bp.w);
                 }
                 pcnt += c;
                 __builtin_amdgcn_wave_barrier();
@@ -1500,7 +1501,7 @@ static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta
                                  t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                                  t->bstart.as<int>(), grid, theta2, G, (float)eps2, t->leaf,
                                  far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
-                                 t->wnode.as<WalkNode>(), t->mom32.as<float>(),
+                                 t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(),
                                  t->fx.as<double>(), t->fy.as<double>(),
                                  t->fz.as<double>(), t->status.as<int64_t>(), dnbun, so);
     };
@@ -1571,7 +1572,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                     t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                     t->bstart.as<int>(), (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf,
                     far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
-                    t->wnode.as<WalkNode>(), t->mom32.as<float>(), t->fx.as<double>(),
+                    t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(), t->fx.as<double>(),
                     t->fy.as<double>(),
                     t->fz.as<double>(), t->status.as<int64_t>(), gs + GS_NBUN, SplitOut{});
             };
@@ -1626,7 +1627,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
                                             theta2, G, (float)eps2, t->leaf, far2(t),
                                             t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
-                                            t->wnode.as<WalkNode>(), t->mom32.as<float>(),
+                                            t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(),
                                             t->fx.as<double>(), t->fy.as<double>(),
                                             t->fz.as<double>(), t->status.as<int64_t>(), nullptr,
                                             SplitOut{});
