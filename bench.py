@@ -299,6 +299,9 @@ def main():
     ap.add_argument("--far", type=float, default=5.0,
                     help="far-field factor: accepted cells at least this many bunch radii from "
                          "the bunch centre enter its local expansion (0: off)")
+    ap.add_argument("--far-inner", type=float, default=3.0,
+                    help="inner far field: accepted cells between this many and --far bunch "
+                         "radii take a higher-order local expansion (0: off)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick bunch / super-group / leaf bucket with the autotuner first")
     ap.add_argument("--sample", type=int, default=4096)
@@ -355,7 +358,7 @@ def main():
     dF = DeviceArray((3 * N,))
     tree = OctreeType().pre_init(dm, args.method, args.bunch, precision="mixed")
     tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused, leaf=args.leaf,
-              far=args.far)
+              far=args.far, far_inner=args.far_inner)
     tuned = None
     if args.autotune:
         tree.init(dx, dy, dz)
@@ -448,8 +451,10 @@ def main():
     mp = {10: 1, 11: 4, 12: 8}[args.method]
     if args.unfused:
         kernel = f"k_eval_mixed<{mp}>"
+    elif args.far > 0 and 0 < args.far_inner < args.far:   # the walk and both far-field tiers
+        kernel = f"k_walk_mixed<{mp},1536> + k_far<{mp},5> + k_far<{mp},6>"
     elif args.far > 0:   # the walk and its far-field kernel, timed together
-        kernel = f"k_walk_mixed<{mp},1536> + k_far<{mp}>"
+        kernel = f"k_walk_mixed<{mp},1536> + k_far<{mp},5>"
     else:
         kernel = f"k_walk_mixed<{mp},1536>"
     order = {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method]
@@ -459,7 +464,8 @@ def main():
     if os.path.exists(tf) and not args.unfused:
         rec = json.load(open(tf))
         if (rec.get("workload") == workload and rec.get("leaf", 0) == args.leaf
-                and rec.get("far", 0.0) == args.far):
+                and rec.get("far", 0.0) == args.far
+                and rec.get("far_inner", 0.0) == args.far_inner):
             traffic = rec["hbm_bytes_per_launch"]
     roof = None
     if flops_alg is not None:
@@ -478,6 +484,13 @@ def main():
                 "bunches": info["ngroups"],
                 "longest_lists": {"cells": info["max_cells"], "particles": info["max_parts"],
                                   "stack": info["max_stack"]}}
+    if args.far <= 0:
+        far_text = "no far field"
+    elif 0 < args.far_inner < args.far:
+        far_text = (f"far field from {args.far_inner:g} bunch radii (local expansion of order 6, "
+                    f"order 5 beyond {args.far:g})")
+    else:
+        far_text = f"far field beyond {args.far:g} bunch radii (local expansion of order 5)"
     line = {
         "metric": "particles/s per force eval at N=1M, rms rel err <=1e-4 vs FP64 direct",
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
@@ -490,7 +503,7 @@ def main():
                    "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
                    "walk": (f"bunched Barnes-1990 lists, <= {args.bunch} particles per bunch, "
                             f"cut from sibling subtrees packed to <= {args.super}, leaf bucket "
-                            f"{args.leaf}, far field beyond {args.far:g} bunch radii"),
+                            f"{args.leaf}, " + far_text),
                    "autotuned": tuned},
         "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
         "ms_per_step_median": q50, "ms_per_step_iqr": [q25, q75],

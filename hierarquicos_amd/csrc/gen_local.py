@@ -16,10 +16,12 @@ A source cell with mass M and raw moments M_m about its centre of mass
 R = (bunch centre) - (centre of mass) gives the potential's local
 coefficients  L_n = -sum_m (-1)^|m| / m! M_m D_(n+m)(R),  so that near the
 bunch centre  phi(c + d) = sum_n L_n d^n / n!  and the acceleration is
-a_i = -sum_n L_(n+e_i) d^n / n!.  Orders: the monopole term to |n| <= 5, the
-quadrupole to |n| <= 3, the octupole to |n| <= 2 (the derivative order is 5
-in all three).  The coefficients are stored scaled by h^|n| (h the bunch
-radius), so every stored value is bounded and the evaluation uses d / h.
+a_i = -sum_n L_(n+e_i) d^n / n!.  Two sets of orders are generated: p5 takes
+the monopole term to |n| <= 5, the quadrupole to |n| <= 3 and the octupole to
+|n| <= 2 (derivative order 5 in all three; 55 coefficients); p6 takes them to
+6, 4 and 3 (derivative order 6; 83 coefficients).  The coefficients are
+stored scaled by h^|n| (h the bunch radius), so every stored value is bounded
+and the evaluation uses d / h.
 """
 import os
 import re
@@ -28,8 +30,7 @@ from fractions import Fraction
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "hq_local.h")
 
-PMONO, PQUAD, POCT = 5, 3, 2
-KMAX = max(PMONO, PQUAD + 2, POCT + 3)
+ORDERS = {5: (5, 3, 2), 6: (6, 4, 3)}  # name -> (monopole, quadrupole, octupole) local orders
 
 
 def midx(n):
@@ -122,11 +123,19 @@ def render(lines, lang):
     return out
 
 
-LOC = [n for o in range(1, KMAX + 1) for n in midx(o)]  # stored coefficients
-LIDX = {n: i for i, n in enumerate(LOC)}
+def orders(p):
+    """(PMONO, PQUAD, POCT, KMAX, LOC, LIDX) of the order set p."""
+    pmono, pquad, poct = ORDERS[p]
+    kmax = max(pmono, pquad + 2, poct + 3)
+    loc = [n for o in range(1, kmax + 1) for n in midx(o)]  # stored coefficients
+    return pmono, pquad, poct, kmax, loc, {n: i for i, n in enumerate(loc)}
 
 
-def gen_m2l():
+LOC = orders(5)[4]
+
+
+def gen_m2l(p):
+    PMONO, PQUAD, POCT, KMAX, LOC, LIDX = orders(p)
     out = []
     w = out.append
     w("// Local coefficients of one source cell (this lane's) about the bunch centre.")
@@ -134,8 +143,8 @@ def gen_m2l():
     w("// cell's FP32 record; adds h^|n| L_n to L[i] for the multi-index n = LOC_n[i]")
     w("// (the addition folded into each coefficient's last multiply-add).")
     w("template <int MP>")
-    w("__device__ __forceinline__ void local_m2l(float Rx, float Ry, float Rz, float eps2, float h,")
-    w("                                          const float* __restrict__ r, float* __restrict__ L) {")
+    w("__device__ __forceinline__ void local_m2l_p%d(float Rx, float Ry, float Rz, float eps2, float h," % p)
+    w("                                             const float* __restrict__ r, float* __restrict__ L) {")
     w("#pragma clang fp contract(off)")
     w("    const float rho = fmaf(Rx, Rx, fmaf(Ry, Ry, fmaf(Rz, Rz, eps2)));")
     w("    const float ri = RSQRT(rho);")
@@ -233,14 +242,15 @@ def gen_m2l():
     return out
 
 
-def gen_l2p():
+def gen_l2p(p):
+    PMONO, PQUAD, POCT, KMAX, LOC, LIDX = orders(p)
     out = []
     w = out.append
     w("")
     w("// Acceleration sum_n L_(n+e_i) u^n / n! at u = (target - bunch centre) / h from the")
-    w("// scaled coefficients L of local_m2l; the caller multiplies by -1/h.")
-    w("__device__ __forceinline__ void local_l2p(const float* __restrict__ L, float ux, float uy,")
-    w("                                          float uz, float acc[3]) {")
+    w("// scaled coefficients L of local_m2l_p%d; the caller multiplies by -1/h." % p)
+    w("__device__ __forceinline__ void local_l2p_p%d(const float* __restrict__ L, float ux, float uy," % p)
+    w("                                             float uz, float acc[3]) {")
     w("#pragma clang fp contract(off)")
     names = {(0, 0, 0): None, (1, 0, 0): "ux", (0, 1, 0): "uy", (0, 0, 1): "uz"}
     # u^n / n! = u^(n-e_i)/(n-e_i)! * u_i / n_i
@@ -280,9 +290,9 @@ def gen_l2p():
     return out
 
 
-def python_source():
-    """The two functions as Python (for CPU tests): m2l(MP, Rx, Ry, Rz, eps2, h, r, L,
-    fmaf, RSQRT) fills L; l2p(L, ux, uy, uz, acc, fmaf) fills acc."""
+def python_source(p=5):
+    """The two functions of order set p as Python (for CPU tests): m2l(MP, Rx, Ry, Rz,
+    eps2, h, r, L, fmaf, RSQRT) adds into L; l2p(L, ux, uy, uz, acc, fmaf) fills acc."""
     def body(lines):
         out = []
         for ln in lines:
@@ -298,8 +308,8 @@ def python_source():
                                                              else "== 8"), t)
             out.append("    " + t)
         return out
-    src = ["def m2l(MP, Rx, Ry, Rz, eps2, h, r, L, fmaf, RSQRT):"] + body(gen_m2l())
-    src += ["", "def l2p(L, ux, uy, uz, acc, fmaf):"] + body(gen_l2p())
+    src = ["def m2l(MP, Rx, Ry, Rz, eps2, h, r, L, fmaf, RSQRT):"] + body(gen_m2l(p))
+    src += ["", "def l2p(L, ux, uy, uz, acc, fmaf):"] + body(gen_l2p(p))
     return "\n".join(src) + "\n"
 
 
@@ -308,13 +318,19 @@ def main():
              "#pragma once",
              "",
              "namespace hq {",
-             "",
-             "constexpr int NLOC = %d;  // local coefficients, 1 <= |n| <= %d" % (len(LOC), KMAX),
-             "// LOC_n[i] = (nx, ny, nz): " + " ".join("%d%d%d" % n for n in LOC),
              ""]
-    lines += render(gen_m2l(), "c")
-    lines += render(gen_l2p(), "c")
-    lines += ["", "}  // namespace hq", ""]
+    for p in sorted(ORDERS):
+        pmono, pquad, poct, kmax, loc, lidx = orders(p)
+        lines += ["constexpr int NLOC_P%d = %d;  // local coefficients, 1 <= |n| <= %d"
+                  % (p, len(loc), kmax),
+                  "// order set p%d: monopole %d, quadrupole %d, octupole %d; coefficient i is"
+                  % (p, pmono, pquad, poct),
+                  "// the multi-index (nx, ny, nz): " + " ".join("%d%d%d" % n for n in loc),
+                  ""]
+        lines += render(gen_m2l(p), "c")
+        lines += render(gen_l2p(p), "c")
+        lines += [""]
+    lines += ["}  // namespace hq", ""]
     with open(OUT, "w") as f:
         f.write("\n".join(lines))
     print(OUT, len(lines), "lines")

@@ -189,6 +189,9 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
     } else if (key == HQ_TUNE_FAR) {
         if (value < 0 || value > 1000) return fail(HQ_ERR_ARGS, "far factor %lld outside 0..1000", (long long)value);
         t->far = (int)value;
+    } else if (key == HQ_TUNE_FAR_INNER) {
+        if (value < 0 || value > 1000) return fail(HQ_ERR_ARGS, "inner far factor %lld outside 0..1000", (long long)value);
+        t->far_inner = (int)value;
     } else if (key == HQ_TUNE_LEAF) {
         if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
         t->leaf = (int)value;

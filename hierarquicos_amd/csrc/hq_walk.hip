@@ -622,12 +622,12 @@ __device__ __forceinline__ float bunch_radius(float tx, float ty, float tz, int 
 
 // Local expansion of this lane's far cell (node >= 0; -1 adds nothing) about
 // the bunch centre bc, added to the lane's coefficients L (scaled by h^|n|).
-template <int MP>
+template <int MP, int P>
 __device__ __forceinline__ void far_m2l(int node, const double* __restrict__ cx,
                                         const double* __restrict__ cy,
                                         const double* __restrict__ cz,
                                         const float* __restrict__ mom, const double bc[3], float h,
-                                        float eps2, float L[NLOC]) {
+                                        float eps2, float* __restrict__ L) {
 #pragma clang fp contract(off)
     float r[REC];
 #pragma unroll
@@ -648,7 +648,8 @@ __device__ __forceinline__ void far_m2l(int node, const double* __restrict__ cx,
             r[4 * q + 3] = v4.w;
         }
     }
-    local_m2l<MP>(Rx, Ry, Rz, eps2, h, r, L);
+    if constexpr (P == 5) local_m2l_p5<MP>(Rx, Ry, Rz, eps2, h, r, L);
+    else local_m2l_p6<MP>(Rx, Ry, Rz, eps2, h, r, L);
 }
 
 constexpr int KBLK = 32;  // interactions per compensated partial sum
@@ -866,7 +867,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    const double* __restrict__ sm, int64_t n,
                                                    const int* __restrict__ bstart, int nbunch,
                                                    double theta2, double G, float eps2, int lbucket,
-                                                   float far2, int* __restrict__ flist,
+                                                   float far2, float far2i, int* __restrict__ flist,
                                                    int* __restrict__ fnum, int fcap,
                                                    const WalkNode* __restrict__ wn,
                                                    const double4* __restrict__ sb,
@@ -905,11 +906,15 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
     const double bc[3] = {0.5 * (bmin[0] + bmax[0]), 0.5 * (bmin[1] + bmax[1]),
                           0.5 * (bmin[2] + bmax[2])};
     const float tx = (float)(sx[i] - bc[0]), ty = (float)(sy[i] - bc[1]), tz = (float)(sz[i] - bc[2]);
-    // far field: accepted cells whose centre of mass is at least sqrt(far2)
-    // bunch radii from the bunch centre go to the bunch's far list for k_far
-    // (at most fcap of them; any more are evaluated here like the others)
+    // far field: accepted cells whose centre of mass is at least sqrt(far2i)
+    // bunch radii from the bunch centre go to the bunch's far list for k_far,
+    // as the node index at sqrt(far2) radii or more (outer tier) and as its
+    // complement below (inner tier); at most fcap of them, any more are
+    // evaluated here like the others
     const float hb = bunch_radius(tx, ty, tz, lane, nb);
-    const double farR2 = far2 > 0.0f ? (double)far2 * ((double)hb * (double)hb) : INFINITY;
+    const double hb2 = (double)hb * (double)hb;
+    const double farR2 = far2 > 0.0f ? (double)far2 * hb2 : INFINITY;
+    const double farR2i = far2i > 0.0f ? (double)far2i * hb2 : farR2;
     int* __restrict__ fl = flist + (int64_t)gi * fcap;
     Kahan3 sum;
     int top = 1, maxtop = 1, chead = 0, ccnt = 0, phead = 0, pcnt = 0, nfar = 0;
@@ -935,7 +940,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         const int k = lane < take ? stk[base + lane] : -1;
         top = base;
         int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell, 4 bucket, 5 far
-        int bfirst = 0, bcount = 0;
+        int bfirst = 0, bcount = 0, fent = 0;
         double cc[3] = {0.0, 0.0, 0.0};
         int kfirst = 0, kcs = 0, knch = 0;
         if (k >= 0) {
@@ -969,7 +974,10 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                         double R2 = 0.0;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) R2 = fma(cc[c] - bc[c], cc[c] - bc[c], R2);
-                        if (R2 >= farR2) action = 5;  // far cell, if its list has room
+                        if (R2 >= farR2i) {  // far cell, if its list has room
+                            action = 5;
+                            fent = R2 >= farR2 ? k : ~k;
+                        }
                     }
                     if (action != 5 && l - f + 1 <= lbucket) {
                         action = 4;  // small cell, not far: its bodies, exactly
@@ -1016,7 +1024,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
             const uint64_t mfar = __ballot(action == 5);
             if (action == 5) {
                 const int slot = nfar + __popcll(mfar & lt);
-                if (slot < fcap) fl[slot] = k;
+                if (slot < fcap) fl[slot] = fent;
                 else action = 3;
             }
             nfar = min(nfar + __popcll(mfar), fcap);
@@ -1202,11 +1210,12 @@ __global__ __launch_bounds__(64) void k_combine(const double* __restrict__ sx,
 }
 
 // Far field of the mixed walk: one wavefront per bunch forms the local
-// expansion of its far list (far_m2l, lane-parallel over the cells), sums it
-// over the wavefront, evaluates it at its bodies and adds G m times the result
-// to the force the walk wrote.  Runs after the walk (fused, or split and
-// combined).
-template <int MP>
+// expansion of one tier of its far list (which 0: the outer cells, stored as
+// node indices, order set P = 5; which 1: the inner ones, stored as their
+// complements, P = 6) lane-parallel over the cells, sums it over the
+// wavefront, evaluates it at its bodies and adds G m times the result to the
+// force the walk wrote.  Runs after the walk (fused, or split and combined).
+template <int MP, int P>
 __global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
                                             const double* __restrict__ sy,
                                             const double* __restrict__ sz,
@@ -1214,7 +1223,7 @@ __global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
                                             const int* __restrict__ bstart, int nbunch,
                                             const int64_t* __restrict__ dnbun,
                                             const int* __restrict__ flist,
-                                            const int* __restrict__ fnum, int fcap,
+                                            const int* __restrict__ fnum, int fcap, int which,
                                             const double* __restrict__ ccx,
                                             const double* __restrict__ ccy,
                                             const double* __restrict__ ccz,
@@ -1227,21 +1236,46 @@ __global__ __launch_bounds__(64) void k_far(const double* __restrict__ sx,
     const int nf = min(fnum[gi], fcap);
     if (nf <= 0) return;
     const int lane = __lane_id();
-    const BunchTargets b = bunch_targets(sx, sy, sz, bstart, gi, lane);
+    const uint64_t lt = lanemask_lt();
     const int* __restrict__ fl = flist + (int64_t)gi * fcap;
-    // lane j takes cells j, j + 64, ... into its own coefficients; the
-    // wavefront sums them once and every lane evaluates the sum at its body
-    float L[NLOC];
+    // the tier's cells, compacted through an LDS ring of 128 (each pass adds
+    // at most 64 and 64 are taken whenever that many are held)
+    __shared__ int sel[128];
+    int head = 0, cnt = 0, tot = 0;
+    BunchTargets b;
+    constexpr int NL = P == 5 ? NLOC_P5 : NLOC_P6;
+    float L[NL];
 #pragma unroll
-    for (int j = 0; j < NLOC; ++j) L[j] = 0.0f;
-    for (int base = 0; base < nf; base += 64) {
-        const int node = base + lane < nf ? fl[base + lane] : -1;
-        far_m2l<MP>(node, ccx, ccy, ccz, mom, b.bc, b.h, eps2, L);
+    for (int j = 0; j < NL; ++j) L[j] = 0.0f;
+    // lane j takes cells j, j + 64, ... of the ring into its own
+    // coefficients; the wavefront sums them once and every lane evaluates the
+    // sum at its body
+    for (int base = 0; base < nf || cnt > 0; base += 64) {
+        if (base < nf) {
+            const int e = base + lane < nf ? fl[base + lane] : 0;
+            const bool mine = base + lane < nf && (which ? e < 0 : e >= 0);
+            const uint64_t m = __ballot(mine);
+            if (mine) sel[(head + cnt + __popcll(m & lt)) & 127] = which ? ~e : e;
+            cnt += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (cnt >= 64 || (base + 64 >= nf && cnt > 0)) {
+            if (tot == 0) b = bunch_targets(sx, sy, sz, bstart, gi, lane);
+            const int take = cnt < 64 ? cnt : 64;
+            const int node = lane < take ? sel[(head + lane) & 127] : -1;
+            __builtin_amdgcn_wave_barrier();
+            head += take;
+            cnt -= take;
+            tot += take;
+            far_m2l<MP, P>(node, ccx, ccy, ccz, mom, b.bc, b.h, eps2, L);
+        }
     }
+    if (tot == 0) return;
 #pragma unroll
-    for (int j = 0; j < NLOC; ++j) L[j] = wave_sum_dpp(L[j]);
+    for (int j = 0; j < NL; ++j) L[j] = wave_sum_dpp(L[j]);
     float a[3];
-    local_l2p(L, b.tx * b.hinv, b.ty * b.hinv, b.tz * b.hinv, a);
+    if constexpr (P == 5) local_l2p_p5(L, b.tx * b.hinv, b.ty * b.hinv, b.tz * b.hinv, a);
+    else local_l2p_p6(L, b.tx * b.hinv, b.ty * b.hinv, b.tz * b.hinv, a);
     if (lane < b.nb) {
         const double gm = G * sm[b.i];
         fx[b.i] = fx[b.i] + gm * (double)(-b.hinv * a[0]);
@@ -1430,13 +1464,28 @@ constexpr double SPLIT_MEM_MAX = 1024.0 * 1024 * 1024;  // bytes of log and part
 
 static int split_bcap(int64_t lcap) { return (int)(lcap / 64 + 3); }
 
-// Square of the far-field factor of the mixed walk (0: no far field).
-static float far2(const hq_tree* t) { return (float)(t->far * t->far) / 100.0f; }
+// Square of the far-field factor of the mixed walk at opening angle theta
+// (theta2 = theta^2; 0: no far field), and of the inner one (0: no inner
+// tier; used only below the outer factor).  Below theta = 0.5 both squares
+// grow as 0.5 / theta: the expansions' truncation error then falls with the
+// multipole error of the cells they take.
+static bool inner_on(const hq_tree* t) {
+    return t->far > 0 && t->far_inner > 0 && t->far_inner < t->far;
+}
+static float far_scale(double theta2) {
+    return theta2 > 0.0 && theta2 < 0.25 ? (float)(0.5 / std::sqrt(theta2)) : 1.0f;
+}
+static float far2(const hq_tree* t, double theta2) {
+    return (float)(t->far * t->far) / 100.0f * far_scale(theta2);
+}
+static float far2i(const hq_tree* t, double theta2) {
+    return inner_on(t) ? (float)(t->far_inner * t->far_inner) / 100.0f * far_scale(theta2) : 0.0f;
+}
 
 // Far cells a bunch's list holds (more are evaluated in the walk as ordinary
-// cells); 0 when the far field is off.
+// cells), twice as many with the inner tier; 0 when the far field is off.
 constexpr int FAR_CAP = 4096;
-static int far_cap(const hq_tree* t) { return t->far > 0 ? FAR_CAP : 0; }
+static int far_cap(const hq_tree* t) { return t->far > 0 ? (inner_on(t) ? 2 : 1) * FAR_CAP : 0; }
 
 static int far_buffers(hq_tree* t, int64_t nbunch) {
     int rc;
@@ -1449,18 +1498,23 @@ static int far_buffers(hq_tree* t, int64_t nbunch) {
 // The far field of a mixed walk over grid bunches (after the walk).
 static void launch_far(hq_tree* t, int grid, const int64_t* dnbun, double G, double eps2) {
     if (t->far <= 0) return;
-    auto launch = [&](auto kern) {
+    // the outer tier (order set p5), then the inner one (p6)
+    auto launch = [&](auto kern, int which) {
         kern<<<grid, 64, 0, t->stream>>>(t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(),
                                          t->sm.as<double>(), t->bstart.as<int>(), grid, dnbun,
-                                         t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
+                                         t->flist.as<int>(), t->fnum.as<int>(), far_cap(t), which,
                                          t->ncomx.as<double>(), t->ncomy.as<double>(),
                                          t->ncomz.as<double>(), t->mom32.as<float>(), G,
                                          (float)eps2, t->fx.as<double>(), t->fy.as<double>(),
                                          t->fz.as<double>());
     };
-    if (t->multipole == 1) launch(k_far<1>);
-    else if (t->multipole == 4) launch(k_far<4>);
-    else launch(k_far<8>);
+    if (t->multipole == 1) launch(k_far<1, 5>, 0);
+    else if (t->multipole == 4) launch(k_far<4, 5>, 0);
+    else launch(k_far<8, 5>, 0);
+    if (!inner_on(t)) return;
+    if (t->multipole == 1) launch(k_far<1, 6>, 1);
+    else if (t->multipole == 4) launch(k_far<4, 6>, 1);
+    else launch(k_far<8, 6>, 1);
 }
 
 static bool use_split(const hq_tree* t, int64_t nbunch) {
@@ -1500,7 +1554,7 @@ static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta
         kern<<<grid, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                  t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                                  t->bstart.as<int>(), grid, theta2, G, (float)eps2, t->leaf,
-                                 far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
+                                 far2(t, theta2), far2i(t, theta2), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                                  t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(),
                                  t->fx.as<double>(), t->fy.as<double>(),
                                  t->fz.as<double>(), t->status.as<int64_t>(), dnbun, so);
@@ -1571,7 +1625,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                     node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                     t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
                     t->bstart.as<int>(), (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf,
-                    far2(t), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
+                    far2(t, theta2), far2i(t, theta2), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                     t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(), t->fx.as<double>(),
                     t->fy.as<double>(),
                     t->fz.as<double>(), t->status.as<int64_t>(), gs + GS_NBUN, SplitOut{});
@@ -1625,7 +1679,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                             t->sy.as<double>(), t->sz.as<double>(),
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
-                                            theta2, G, (float)eps2, t->leaf, far2(t),
+                                            theta2, G, (float)eps2, t->leaf, far2(t, theta2), far2i(t, theta2),
                                             t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                                             t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(),
                                             t->fx.as<double>(), t->fy.as<double>(),

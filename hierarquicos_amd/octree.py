@@ -145,17 +145,22 @@ class OctreeType:
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
     def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None,
-             graph=None, split=None, far=None):
+             graph=None, split=None, far=None, far_inner=None):
         """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
         bucket (cells of <= leaf bodies enter as their bodies) of the bunched
         walk; FMM leaf size; graph: whether integrate() replays one captured
         step (default True); split: 0 never, 1 (default) with few bunches, 2
         always -- the fused walk's batches evaluated one wavefront each (same
         results); far: the mixed walk's far-field factor in bunch radii (default
-        5.0; 0 off): farther accepted cells enter the bunch's local expansion."""
+        5.0; 0 off): farther accepted cells enter the bunch's local expansion;
+        far_inner (default 3.0; 0 off): cells between far_inner and far radii
+        enter an expansion of higher order."""
         lib = L.lib()
         if far is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FAR, int(round(10 * float(far)))))
+        if far_inner is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FAR_INNER,
+                                     int(round(10 * float(far_inner)))))
         if split is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_SPLIT, int(split)))
         if graph is not None:

@@ -125,7 +125,13 @@ int hq_tree_destroy(hq_tree *t);
                               enters the bunch's local expansion (one M2L per cell, orders 5 / 3
                               / 2 for its monopole / quadrupole / octupole terms, evaluated once
                               per body) instead of one M2P per body; 0..1000, default 50; 0 off.
-                              The two-kernel walk (HQ_TUNE_FUSED 0) has no far field */
+                              Below theta = 0.5 this factor and HQ_TUNE_FAR_INNER's grow as
+                              sqrt(0.5 / theta). The two-kernel walk (HQ_TUNE_FUSED 0) has no
+                              far field */
+#define HQ_TUNE_FAR_INNER 9 /* inner tier of the far field: accepted cells between value/10
+                              and HQ_TUNE_FAR/10 bunch radii take a local expansion of
+                              higher orders (6 / 4 / 3); 0..1000, default 30; 0, or a value
+                              not below HQ_TUNE_FAR, switches the tier off */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
 /* Cartesian FMM of order 4 or 6 on the built tree: Dehnen's method carried to

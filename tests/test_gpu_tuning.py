@@ -91,24 +91,54 @@ def test_split_walk_bitidentical_to_fused(method, leaf):
     assert tb.info()["exec_cells"] == ta.info()["exec_cells"]
 
 
+def _far_rows(method, theta2, tiers, seed=25, n=200000):
+    """(far, inner, RMS error, RMS difference from the walk without a far
+    field, RMS error without it, far cells) per tier setting, the errors
+    against the FP64 direct sum on a sample of targets."""
+    m, x, y, z, _ = _tree(n, seed)
+    sample = np.arange(0, n, 101)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    t = OctreeType().pre_init(m, method, precision="mixed").init(x, y, z)
+    t.tune(far=0)
+    Fn = t.forces_bunched(theta2, 1.0, 1e-6, "mixed").T
+    assert t.info()["far_cells"] == 0
+    e_near, _ = O.rel_errors(Fn[sample], Fd)
+    rows = []
+    for far, inner in tiers:
+        t.tune(far=far, far_inner=inner)
+        Ff = t.forces_bunched(theta2, 1.0, 1e-6, "mixed").T
+        e_far, _ = O.rel_errors(Ff[sample], Fd)
+        d, _ = O.rel_errors(Ff[sample], Fn[sample])
+        rows.append((far, inner, e_far, d, e_near, t.info()["far_cells"]))
+    print(method, theta2, rows)
+    return rows
+
+
 @pytest.mark.parametrize("method", [10, 11, 12])
 def test_far_field_error_budget(method):
-    """Far cells through each bunch's local expansion (the default) against the
-    same walk with every cell evaluated per body: the extra error is a small
-    part of the multipole error, and both stay within the budget against the
-    FP64 direct sum."""
-    m, x, y, z, t = _tree(200000, 25)
-    sample = np.arange(0, 200000, 101)
-    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
-    t2 = OctreeType().pre_init(m, method, precision="mixed").init(x, y, z)
-    t2.tune(far=5.0)
-    Ff = t2.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
-    assert t2.info()["far_cells"] > 0
-    t2.tune(far=0)
-    Fn = t2.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
-    assert t2.info()["far_cells"] == 0
-    e_far, _ = O.rel_errors(Ff[sample], Fd)
-    e_near, _ = O.rel_errors(Fn[sample], Fd)
-    d, _ = O.rel_errors(Ff[sample], Fn[sample])
-    assert d < 0.35 * e_near, (d, e_near)
-    assert e_far < 1.05 * e_near + 1e-6, (e_far, e_near)
+    """Far cells through each bunch's local expansion against the same walk
+    with every cell evaluated per body, at theta = 0.5: the outer tier alone
+    (order 5 beyond 5 bunch radii), both tiers (the default: order 6 from 3
+    radii) and the inner tier alone (order 6 from 3 radii out).  The outer
+    tier adds a small part of the multipole error; the inner one, whose
+    truncation at 3 radii is about 4e-5 RMS whatever the multipole order,
+    keeps the octupole walk within the 1e-4 budget against the FP64 direct
+    sum."""
+    rows = _far_rows(method, 0.25, [(5.0, 0.0), (5.0, 3.0), (100.0, 3.0)])
+    (_, _, e5, d5, e0, c5), (_, _, e53, d53, _, c53), (_, _, e3, d3, _, c3) = rows
+    assert d5 < 0.35 * e0 and e5 < 1.05 * e0 + 1e-6, rows[0]
+    for e, d in ((e53, d53), (e3, d3)):
+        assert d < 0.8 * e0 and d < 5e-5 and e < 1.3 * e0, rows
+    if method == 12:
+        assert e53 < 1e-4, rows
+    # the inner tier takes the cells between 3 and 5 radii too; the tiers
+    # only change the orders the same cells are expanded to
+    assert c53 > c5 > 0 and c3 == c53, rows
+
+
+def test_far_field_tiers_at_a_wider_angle():
+    """theta^2 = 0.5: the multipole error grows faster than the expansions'
+    truncation, and both tiers add a small part of it."""
+    rows = _far_rows(12, 0.5, [(5.0, 0.0), (5.0, 3.0)])
+    for _, _, e, d, e0, c in rows:
+        assert c > 0 and d < 0.35 * e0 and e < 1.05 * e0, rows
