@@ -931,7 +931,9 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         else eval_parts(pring, head, cnt, tx, ty, tz, eps2, sum);
     };
     const int sys = nv.sys[gf];
-    const double* __restrict__ gl2 = g + SG * sys + S_L2;
+    // squared side of a cell at depth d: the root's scaled by 4^-d, the same
+    // double as the build's per-depth table (powers of two scale exactly)
+    const double l2root = g[SG * sys + S_L2];
     if (lane == 0) stk[0] = sys;
     __builtin_amdgcn_wave_barrier();
     while (top > 0) {
@@ -942,7 +944,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
         int action = 0;  // 0 skip, 1 open, 2 particle, 3 cell, 4 bucket, 5 far
         int bfirst = 0, bcount = 0, fent = 0;
         double cc[3] = {0.0, 0.0, 0.0};
-        int kfirst = 0, kcs = 0, knch = 0;
+        float kmass = 0.0f;
+        int kcs = 0, knch = 0;
         if (k >= 0) {
             const double2 w0 = reinterpret_cast<const double2*>(wn + k)[0];
             const double2 w2 = reinterpret_cast<const double2*>(wn + k)[1];
@@ -950,9 +953,9 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
             cc[0] = w0.x;
             cc[1] = w0.y;
             cc[2] = w2.x;
-            kfirst = w1.x;
             kcs = w1.z;
             knch = w1.w & 15;
+            kmass = (float)w2.y;
             if (w2.y != 0.0) {
                 const int f = w1.x;
                 const int l = f + w1.y - 1;
@@ -969,7 +972,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                         dist2 = fma(dd, dd, dist2);
                     }
                     const int depth = w1.w >> 12;
-                    action = (gl2[depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                    action = (ldexp(l2root, -2 * depth) <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
                     if (action == 3) {
                         double R2 = 0.0;
 #pragma unroll
@@ -987,36 +990,46 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                 }
             }
         }
+        // a one-body leaf: its record holds the body's position and mass
         const uint64_t mp = __ballot(action == 2);
-        if (action == 2) {
-            const double4 bp = sb[kfirst];
+        if (action == 2)
             pring[(phead + pcnt + __popcll(mp & lt)) & (RING - 1)] =
-                make_float4((float)(bp.x - bc[0]), (float)(bp.y - bc[1]), (float)(bp.z - bc[2]),
-                            (float)bp.w);
-        }
+                make_float4((float)(cc[0] - bc[0]), (float)(cc[1] - bc[1]), (float)(cc[2] - bc[2]),
+                            kmass);
         pcnt += __popcll(mp);
-        {
-            // buckets: one small cell's bodies at a time into the particle ring,
-            // draining it so that it never holds more than RING entries
+        if (__ballot(action == 4)) {
+            // buckets: the small cells' bodies into the particle ring, the
+            // cells in lane order.  Each round drains whole batches, then
+            // every lane whose cell fits entirely in the free space copies its
+            // bodies at its offset; the first cell left always fits (at most
+            // 64 bodies, at least 65 free), so every round makes progress
             __builtin_amdgcn_wave_barrier();
-            uint64_t mb = __ballot(action == 4);
-            while (mb) {
-                const int j = __ffsll((unsigned long long)mb) - 1;
-                mb &= mb - 1;
-                const int f = __shfl(bfirst, j, 64), c = __shfl(bcount, j, 64);
+            const int bn = action == 4 ? bcount : 0;
+            const int bincl = wave_incl_scan_dpp(bn);
+            const int bexcl = bincl - bn;
+            const int btot = __builtin_amdgcn_readlane(bincl, 63);
+            int done = 0;
+            while (done < btot) {
                 while (pcnt >= 64) {
                     batch_parts(phead, 64);
                     phead += 64;
                     pcnt -= 64;
                     ntot_p += 64;
                 }
-                if (lane < c) {
-                    const double4 bp = sb[f + lane];
-                    pring[(phead + pcnt + lane) & (RING - 1)] =
-                        make_float4((float)(bp.x - bc[0]), (float)(bp.y - bc[1]),
-                                    (float)(bp.z - bc[2]), (float)bp.w);
+                const bool mine = bn > 0 && bexcl >= done && bincl <= done + (RING - pcnt);
+                const uint64_t mm = __ballot(mine);
+                const int end = __builtin_amdgcn_readlane(bincl, 63 - __clzll(mm));
+                if (mine) {
+                    const int at = phead + pcnt + bexcl - done;
+                    for (int q = 0; q < bn; ++q) {
+                        const double4 bp = sb[bfirst + q];
+                        pring[(at + q) & (RING - 1)] =
+                            make_float4((float)(bp.x - bc[0]), (float)(bp.y - bc[1]),
+                                        (float)(bp.z - bc[2]), (float)bp.w);
+                    }
                 }
-                pcnt += c;
+                pcnt += end - done;
+                done = end;
                 __builtin_amdgcn_wave_barrier();
             }
         }
@@ -1054,7 +1067,7 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
             ccnt -= 64;
             ntot_c += 64;
         }
-        if (pcnt >= 64) {
+        while (pcnt >= 64) {
             batch_parts(phead, 64);
             phead += 64;
             pcnt -= 64;
