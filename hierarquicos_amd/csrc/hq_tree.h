@@ -61,6 +61,7 @@ struct hq_tree {
     bool build_timed = false;   // the last launched build's events await reading
     int far = 50;          // far-field factor of the fused mixed walk, in tenths (0: off)
     int far_inner = 30;    // inner far-field factor, in tenths (0: no inner tier)
+    bool stack_long = false;  // the fused walk's short stack overflowed: start with the long one
     int64_t far_cells = 0; // cells of the last walk taken through the far-field expansion
     int64_t split_lcap = 0;  // log entries per bunch (from the last walk's lists; 0 unknown)
     int64_t split_walks = 0; // walks evaluated split (diagnostic)

@@ -190,6 +190,7 @@ constexpr int SCAP = 4096;   // LDS walk stack (ints) per wavefront
 constexpr int BUNCH = 64;
 constexpr double LIST_GIB_MAX = 48.0;  // device memory allowed for interaction lists
 constexpr int SCAP_FUSED = 1536;       // LDS walk stack of the fused mixed walk
+constexpr int SCAP_FAST = 1152;        // its first try outside graphs: 8.5 KB of LDS, 18 wavefronts per CU
 
 struct ListOut {
     int* cidx;      // MODE_GROUP / MODE_BUNCH: node indices
@@ -1684,7 +1685,11 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             }
             // the log or the stack overflowed: the fused walk below
         }
-        if (t->fused) {
+        // the fused walk with the short stack, then (if a bunch's stack
+        // overflowed) with the long one; it writes every force and far list
+        // afresh, so a second try starts clean.  After an overflow the handle
+        // starts with the long stack until a walk needs well under the short.
+        for (int attempt = t->stack_long ? 1 : 0; t->fused && attempt < 2; ++attempt) {
             if ((rc = far_buffers(t, ngroups))) return rc;
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
@@ -1699,13 +1704,23 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                                             t->fz.as<double>(), t->status.as<int64_t>(), nullptr,
                                             SplitOut{});
             };
-            if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
-            else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
-            else launch(k_walk_mixed<8, SCAP_FUSED>);
+            if (attempt == 0) {
+                if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FAST>);
+                else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FAST>);
+                else launch(k_walk_mixed<8, SCAP_FAST>);
+            } else {
+                if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
+                else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
+                else launch(k_walk_mixed<8, SCAP_FUSED>);
+            }
             launch_far(t, ngroups, nullptr, G, eps2);
             HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
             if ((rc = read_status(t))) return rc;
             HQ_CHECK_HIP(hipGetLastError());
+            if (attempt == 0 && (t->hstatus[ST_FLAGS] & 16)) t->stack_long = true;
+            if (attempt == 1 && !(t->hstatus[ST_FLAGS] & 16) &&
+                t->hstatus[ST_MAXSTK] <= SCAP_FAST - 128)
+                t->stack_long = false;
             if (!(t->hstatus[ST_FLAGS] & 16)) {
                 note_lists(t);
                 hipEventElapsedTime(&t->ms_eval, t->ev[5], t->ev[8]);
@@ -1721,7 +1736,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 t->max_stack = t->hstatus[ST_MAXSTK];
                 return HQ_OK;
             }
-            // the fused kernel's stack overflowed: redo with the two-kernel walk
+            // a stack overflowed: the long stack, then the two-kernel walk
         }
     }
     if ((rc = t->lstat.ensure((size_t)ngroups * 2 * sizeof(int)))) return rc;

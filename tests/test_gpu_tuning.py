@@ -142,3 +142,29 @@ def test_far_field_tiers_at_a_wider_angle():
     rows = _far_rows(12, 0.5, [(5.0, 0.0), (5.0, 3.0)])
     for _, _, e, d, e0, c in rows:
         assert c > 0 and d < 0.35 * e0 and e < 1.05 * e0, rows
+
+
+def test_fused_walk_stack_retries():
+    """The fused walk first runs with a short LDS stack; when a bunch
+    overflows it, the walk runs again with the long stack (the same bits as
+    the split walk, which always has the long one) and the handle starts
+    there next time.  When the long stack overflows too, the two-kernel walk
+    (no far field) gives the forces."""
+    m, q, _ = ics.plummer(20000, seed=5)
+    x, y, z = (np.ascontiguousarray(q[:, c]) for c in range(3))
+    ta = OctreeType().pre_init(m, 12, precision="mixed").tune(split=0).init(x, y, z)
+    tb = OctreeType().pre_init(m, 12, precision="mixed").tune(split=2).init(x, y, z)
+    Fa1 = ta.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert ta.info()["max_stack"] > 1152 and ta.info()["far_cells"] > 0   # needed the long stack
+    Fa2 = ta.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    tb.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    Fb = tb.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert tb.info()["split_walks"] == 1
+    assert np.array_equal(Fa1, Fb) and np.array_equal(Fa2, Fb)
+    # deeper than either stack: the two-kernel walk, within the budget
+    m, x, y, z, t = _tree(200000, 25)
+    F = t.forces_bunched(0.1, 1.0, 1e-6, "mixed").T
+    assert t.info()["far_cells"] == 0
+    sample = np.arange(0, 200000, 101)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    assert O.rel_errors(F[sample], Fd)[0] < 2e-5
