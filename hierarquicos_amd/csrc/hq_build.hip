@@ -251,6 +251,25 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
     hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Per digit d (one workgroup each): exclusive scan over the tiles of the
+// counts hist[d * nblk + b] in place, the digit's total to dtot[d].  The
+// scatter adds the totals of the smaller digits.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ hist, int nblk,
+                                                        uint32_t* __restrict__ dtot) {
+    __shared__ uint32_t ws[RS_THREADS / 64];
+    uint32_t* __restrict__ h = hist + (int64_t)blockIdx.x * nblk;
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nblk; b0 += RS_THREADS) {
+        const int i = b0 + threadIdx.x;
+        const uint32_t v = i < nblk ? h[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, ws, tot);
+        if (i < nblk) h[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
 // Stable scatter: wave w of the workgroup owns the contiguous slice
 // [w*64*ITEMS, (w+1)*64*ITEMS) of the tile and walks it 64 keys at a time;
 // keys of equal digit inside one 64-key step are ranked by ballot (lanes in
@@ -259,13 +278,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
 template <int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t* __restrict__ kin, const int* __restrict__ vin, uint64_t* __restrict__ kout,
-    int* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ hist, int nblk) {
+    int* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ hist, int nblk,
+    const uint32_t* __restrict__ dtot) {
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     __shared__ uint32_t goff[256];
+    __shared__ uint32_t ws[RS_THREADS / 64];
     const int w = threadIdx.x >> 6;
     const int lane = __lane_id();
     for (int d = threadIdx.x; d < 256 * (RS_THREADS / 64); d += RS_THREADS) (&wcnt[0][0])[d] = 0;
-    goff[threadIdx.x] = hist[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    uint32_t all;
+    const uint32_t below = block_excl_scan(dtot[threadIdx.x], ws, all);  // keys of smaller digits
+    goff[threadIdx.x] = below + hist[(int64_t)threadIdx.x * nblk + blockIdx.x];
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * (RS_THREADS * ITEMS) + (int64_t)w * (ITEMS * 64);
     uint64_t key[ITEMS];
@@ -834,16 +857,17 @@ static int rs_passes(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64
                      DevBuf& hist, DevBuf& part, hipStream_t s) {
     const int nblk = grid_for(n, RS_THREADS * ITEMS);
     int rc;
-    if ((rc = hist.ensure(((size_t)256 * nblk + 1) * sizeof(uint32_t)))) return rc;
+    if ((rc = hist.ensure(((size_t)256 * nblk + 256) * sizeof(uint32_t)))) return rc;
+    uint32_t* dtot = hist.as<uint32_t>() + (size_t)256 * nblk;  // per-digit totals
     uint64_t *ka = keys, *kb = ktmp;
     int *va = vals, *vb = vtmp;
     const int passes = (bits + 7) / 8;
     for (int pass = 0; pass < passes; ++pass) {
         const int shift = 8 * pass;
         k_rs_hist<ITEMS><<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist.as<uint32_t>(), nblk);
-        if ((rc = scan_excl_buf(hist.as<uint32_t>(), (int64_t)256 * nblk, part, s))) return rc;
+        k_rs_scan<<<256, RS_THREADS, 0, s>>>(hist.as<uint32_t>(), nblk, dtot);
         k_rs_scatter<ITEMS><<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift,
-                                                        hist.as<uint32_t>(), nblk);
+                                                        hist.as<uint32_t>(), nblk, dtot);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
