@@ -452,11 +452,11 @@ def main():
     if args.unfused:
         kernel = f"k_eval_mixed<{mp}>"
     elif args.far > 0 and 0 < args.far_inner < args.far:   # the walk and both far-field tiers
-        kernel = f"k_walk_mixed<{mp},1536> + k_far<{mp},5> + k_far<{mp},6>"
+        kernel = f"k_walk_mixed<{mp},1152> + k_far<{mp},5> + k_far<{mp},6>"
     elif args.far > 0:   # the walk and its far-field kernel, timed together
-        kernel = f"k_walk_mixed<{mp},1536> + k_far<{mp},5>"
+        kernel = f"k_walk_mixed<{mp},1152> + k_far<{mp},5>"
     else:
-        kernel = f"k_walk_mixed<{mp},1536>"
+        kernel = f"k_walk_mixed<{mp},1152>"
     order = {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method]
     workload = f"{args.profile}_n{N}_{order}_theta{args.theta}_eps{args.eps}"
     traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
