@@ -212,6 +212,22 @@ def test_morton_keys_and_permutation_bitexact(golden):
     assert np.array_equal(perm, np.argsort(ko, kind="stable"))
 
 
+@pytest.mark.parametrize("n", [70000, 600001])
+@pytest.mark.parametrize("depth", [0, 6])
+def test_radix_sort_large_inputs_stable(n, depth):
+    """The radix sort behind the keys at sizes with 2 and 8 keys per thread
+    and, at 600001 keys, more tiles than one workgroup of the per-digit scan
+    covers at once; depth 6 leaves many equal keys, whose original order the
+    permutation must keep."""
+    rng = np.random.default_rng(n + depth)
+    q = rng.normal(size=(n, 3))
+    m = rng.uniform(0.5, 1.5, n)
+    keys, perm, L = morton_keys(m, q, depth)
+    ko, Lo = O.morton_keys(m, q, depth)
+    assert L == Lo and np.array_equal(keys, ko)
+    assert np.array_equal(perm, np.argsort(ko, kind="stable"))
+
+
 # ---------------------------------------------------------------- mixed precision
 def test_mixed_bunched_error_budget():
     m, q, _ = ics.plummer(50000, seed=12)
