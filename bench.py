@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--method", type=int, default=12)
     ap.add_argument("--profile", choices=("plummer", "hernquist", "cube"), default="plummer")
     ap.add_argument("--bunch", type=int, default=64, help="bunch size (<= 64)")
-    ap.add_argument("--super", type=int, default=256, help="super-group size of the bunching")
+    ap.add_argument("--super", type=int, default=128, help="super-group size of the bunching")
     ap.add_argument("--unfused", action="store_true", help="lists and evaluation as two kernels")
     ap.add_argument("--leaf", type=int, default=4,
                     help="leaf bucket: cells of <= leaf bodies enter the lists as their bodies")
@@ -465,7 +465,8 @@ def main():
         rec = json.load(open(tf))
         if (rec.get("workload") == workload and rec.get("leaf", 0) == args.leaf
                 and rec.get("far", 0.0) == args.far
-                and rec.get("far_inner", 0.0) == args.far_inner):
+                and rec.get("far_inner", 0.0) == args.far_inner
+                and rec.get("super", 256) == args.super):
             traffic = rec["hbm_bytes_per_launch"]
     roof = None
     if flops_alg is not None:
