@@ -21,7 +21,12 @@
 //                    block-compensated (Kahan) accumulation
 //   k_walk_mixed     the mixed walk fused: one wavefront per bunch walks and
 //                    evaluates 64-entry batches from LDS rings as they fill
-//                    (record mode: writes the batches to a log instead)
+//                    (record mode: writes the batches to a log instead);
+//                    far accepted cells go to the bunch's far list.  A
+//                    1152-entry LDS stack first, the 1536-entry one when a
+//                    bunch overflows it
+//   k_far<MP, P>     the far list's tier of order set P (5 outer, 6 inner):
+//                    one local expansion per bunch, evaluated at its bodies
 //   k_eval_batches,  split evaluation for few bunches: one wavefront per
 //   k_combine        logged batch, then the partials summed in batch order
 //                    (the fused kernel's arithmetic, bit for bit)
