@@ -299,7 +299,7 @@ def main():
     ap.add_argument("--far", type=float, default=5.0,
                     help="far-field factor: accepted cells at least this many bunch radii from "
                          "the bunch centre enter its local expansion (0: off)")
-    ap.add_argument("--far-inner", type=float, default=2.7,
+    ap.add_argument("--far-inner", type=float, default=3.0,
                     help="inner far field: accepted cells between this many and --far bunch "
                          "radii take a higher-order local expansion (0: off)")
     ap.add_argument("--autotune", action="store_true",

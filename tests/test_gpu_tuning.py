@@ -168,3 +168,19 @@ def test_fused_walk_stack_retries():
     sample = np.arange(0, 200000, 101)
     Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
     assert O.rel_errors(F[sample], Fd)[0] < 2e-5
+
+
+def test_headline_settings_within_budget():
+    """The settings of the 1M headline (super-groups of 128, outer tier from 5
+    and inner tier from 3 bunch radii, leaf bucket 4) keep the octupole walk
+    within the 1e-4 RMS budget against the FP64 direct sum, here at 2e5."""
+    m, x, y, z, _ = _tree(200000, 25)
+    t = OctreeType().pre_init(m, 12, precision="mixed")
+    t.tune(super_size=128, far=5.0, far_inner=3.0, leaf=4)
+    t.init(x, y, z)
+    F = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
+    assert t.info()["far_cells"] > 0
+    sample = np.arange(0, 200000, 101)
+    Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    rms, _ = O.rel_errors(F[sample], Fd)
+    assert rms < 1e-4, rms
