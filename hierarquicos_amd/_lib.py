@@ -53,7 +53,7 @@ EXPORTS = (
     "hq_tree_energy", "hq_parareal", "hq_errors_ma", "hq_tree_autotune", "hq_tree_fmm",
     "hq_tree_set_masses", "hq_tree_build", "hq_tree_forces", "hq_tree_forces_bunched",
     "hq_tree_destroy", "hq_tree_tune", "hq_tree_info", "hq_tree_export", "hq_tree_order",
-    "hq_direct_forces_device", "hq_morton_keys", "hq_morton_create", "hq_morton_build",
+    "hq_direct_forces_device", "hq_morton_keys", "hq_morton_keys_noalloc", "hq_morton_create", "hq_morton_build",
     "hq_morton_forces", "hq_morton_info", "hq_morton_export", "hq_morton_destroy", "hq_dev_alloc", "hq_dev_free", "hq_dev_copy",
     "hq_dev_sync", "hq_last_error", "hq_version")
 
@@ -123,6 +123,7 @@ def lib():
     L.hq_direct_forces_device.argtypes = [C.c_int64, _vp, _vp, _vp, _vp, C.c_int64, _vp,
                                           C.c_double, C.c_double, C.c_int, _vp]
     L.hq_morton_keys.argtypes = [C.c_int64, _vp, _vp, C.c_int, C.c_double, _vp, _vp, _i, C.c_int]
+    L.hq_morton_keys_noalloc.argtypes = [C.c_int64, _vp, _vp, C.c_int, C.c_double, _vp, _vp, _i]
     L.hq_morton_create.argtypes = [C.POINTER(_vp), C.c_int64, C.c_int, C.c_double, C.c_int]
     L.hq_morton_build.argtypes = [_vp, _vp, _vp, C.c_int]
     L.hq_morton_forces.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _vp, C.c_int,

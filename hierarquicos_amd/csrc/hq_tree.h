@@ -203,6 +203,10 @@ int read_status(hq_tree* t);
 void collect_build_times(hq_tree* t);
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
                      DevBuf& hist, DevBuf& part, hipStream_t s);
+// the same sort with a caller-provided count table of rs_hist_words(n) words
+int radix_sort_pairs_ws(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
+                        uint32_t* hist, hipStream_t s);
+size_t rs_hist_words(int64_t n);
 int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s);
 int direct_forces(int64_t n, const double* m, const double* x, const double* y, const double* z,
                   int64_t nt, const int64_t* targets, double G, double eps2, int precision,

@@ -308,6 +308,20 @@ MODULE hq_iface
             INTEGER(C_INT) :: st
         END FUNCTION
 
+        ! hq_morton_keys on device arrays with no device allocation (no
+        ! reference counterpart; for callers that cannot allocate)
+        FUNCTION hq_morton_keys_noalloc(n, m, q, L, ssa, keys, perm, L_out) &
+                BIND(C, name="hq_morton_keys_noalloc") RESULT(st)
+            IMPORT :: C_INT, C_INT64_T, C_PTR, C_DOUBLE
+            INTEGER(C_INT64_T), VALUE :: n
+            TYPE(C_PTR), VALUE :: m, q
+            INTEGER(C_INT), VALUE :: L
+            REAL(C_DOUBLE), VALUE :: ssa
+            TYPE(C_PTR), VALUE :: keys, perm
+            INTEGER(C_INT), INTENT(OUT) :: L_out
+            INTEGER(C_INT) :: st
+        END FUNCTION
+
         FUNCTION hq_morton_create(h, n, L, ssa, use_quad) BIND(C, name="hq_morton_create") RESULT(st)
             IMPORT :: C_INT, C_INT64_T, C_PTR, C_DOUBLE
             TYPE(C_PTR), INTENT(OUT) :: h

@@ -279,6 +279,16 @@ int hq_direct_forces_device(int64_t n, const double *m, const double *x, const d
  * and the depth used. */
 int hq_morton_keys(int64_t n, const double *m, const double *q /* (3,N) */, int L, double ssa,
                    int64_t *keys, int *perm, int *L_out, int where);
+/* No counterpart in the reference: hq_morton_keys for callers that cannot
+ * allocate device memory (code under HIP stream capture, serving loops).
+ * m, q, keys and perm are device memory of the current device, L_out host;
+ * results equal hq_morton_keys's bit for bit.  Its work arrays are one fixed
+ * device array per GPU (about 12.5 MiB, held until the process ends), so n
+ * is at most 1,048,576 (larger n: HQ_ERR_ARGS); keys and perm are also used
+ * as sort buffers.  Runs on the null stream and returns when done; calls that
+ * overlap on one GPU race on the shared array. */
+int hq_morton_keys_noalloc(int64_t n, const double *m, const double *q /* (3,N) */, int L,
+                           double ssa, int64_t *keys, int *perm, int *L_out);
 
 /* ---- device memory plumbing ------------------------------------------- */
 /* No counterpart in the reference (it has no device): thin wrappers of
