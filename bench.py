@@ -13,6 +13,7 @@ Barnes (1990) list walk with FP32 interactions (hq_tree_forces_bunched).
 Prints one JSON line.  Single GPU only: --gpus above 1 exits with a message.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import subprocess
@@ -33,6 +34,21 @@ FLOPS_P2P = 20
 FLOPS_M2P = {1: 20, 4: 61, 8: 201}
 
 
+def _run_child(cmd, env, timeout=600):
+    """Runs a CPU-baseline child in its own process group and waits for it;
+    on a time-out the whole group is killed, so nothing it started outlives
+    this call."""
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env,
+                         start_new_session=True)
+    try:
+        out, err = p.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, 9)
+        out, err = p.communicate()
+        return subprocess.CompletedProcess(cmd, -9, out, err + "\ntimed out")
+    return subprocess.CompletedProcess(cmd, p.returncode, out, err)
+
+
 def cpu_baseline(args):
     """Reference Fortran (or the C port) on the host cores, bounded sample."""
     env = dict(os.environ)
@@ -44,7 +60,7 @@ def cpu_baseline(args):
            "--seed", str(args.seed), "--theta", str(args.theta), "--eps", str(args.eps),
            "--method", str(args.method), "--profile", args.profile, "--sample",
            str(args.cpu_sample), "--threads", str(threads)]
-    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    r = _run_child(cmd, env)
     if r.returncode != 0:
         return {"value": None, "unit": "particles/s", "cores": threads, "kind": "reference",
                 "sample": "failed: " + r.stderr[-300:]}
@@ -64,7 +80,7 @@ def cpu_verlet(n, method, theta, eps, steps, seed=0):
     cmd = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), "--n", str(n),
            "--seed", str(seed), "--theta", str(theta), "--eps", str(eps), "--method",
            str(method), "--verlet-steps", str(steps), "--threads", str(threads)]
-    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    r = _run_child(cmd, env)
     if r.returncode != 0:
         return None
     return json.loads(r.stdout.strip().splitlines()[-1])
@@ -305,6 +321,13 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="pick bunch / super-group / leaf bucket with the autotuner first")
     ap.add_argument("--sample", type=int, default=4096)
+    ap.add_argument("--seeds", type=int, default=5,
+                    help="accuracy and per-seed timing (median of --seed-reps) on seeds 0..seeds-1")
+    ap.add_argument("--seed-reps", type=int, default=20)
+    ap.add_argument("--thetas", default="0.3,0.7",
+                    help="other opening angles timed beside --theta (mixed and FP64 per-target)")
+    ap.add_argument("--mac", choices=("side", "offset"), default="offset",
+                    help="acceptance test of the bunched walk")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64 paths and counts")
@@ -358,7 +381,7 @@ def main():
     dF = DeviceArray((3 * N,))
     tree = OctreeType().pre_init(dm, args.method, args.bunch, precision="mixed")
     tree.tune(bunch=args.bunch, super_size=args.super, fused=not args.unfused, leaf=args.leaf,
-              far=args.far, far_inner=args.far_inner)
+              far=args.far, far_inner=args.far_inner, mac=args.mac)
     tuned = None
     if args.autotune:
         tree.init(dx, dy, dz)
@@ -408,10 +431,36 @@ def main():
     Fd = direct_forces(dm, dx, dy, dz, 1.0, e2, targets=sample)
     Fm = dF.numpy().reshape(N, 3)[sample]
 
-    def rel(F):
-        r = np.linalg.norm(F - Fd, axis=1) / np.linalg.norm(Fd, axis=1)
+    def rel(F, ref=None):
+        ref = Fd if ref is None else ref
+        r = np.linalg.norm(F - ref, axis=1) / np.linalg.norm(ref, axis=1)
         return float(np.sqrt(np.mean(r * r))), float(r.max())
     rms, mx = rel(Fm)
+    walk_stack = info["walk_stack"]
+
+    def timed_steps(th2_, reps):
+        for _ in range(2):
+            tree.init(dx, dy, dz)
+            tree.forces_bunched(th2_, 1.0, e2, "mixed", out=dF)
+        ts = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            tree.init(dx, dy, dz)
+            tree.forces_bunched(th2_, 1.0, e2, "mixed", out=dF)
+            ts.append((time.perf_counter() - a) * 1e3)
+        return float(np.median(ts))
+
+    # other opening angles on the same input: the mixed walk and, below, the
+    # reference's per-target FP64 walk
+    sweep = []
+    if not args.no_extras:
+        for th in (float(v) for v in args.thetas.split(",") if v):
+            msm = timed_steps(th * th, 10)
+            r_, x_ = rel(dF.numpy().reshape(N, 3)[sample])
+            sweep.append({"theta": th, "mixed_ms_per_eval": msm,
+                          "mixed_particles_per_s": N / (msm * 1e-3), "mixed_rms_rel_err": r_,
+                          "mixed_max_rel_err": x_, "walk_stack": tree.info()["walk_stack"],
+                          "far_cells": tree.info()["far_cells"]})
 
     extras = {}
     flops_alg = None
@@ -437,6 +486,19 @@ def main():
         sync()
         ms_bun64 = (time.perf_counter() - a) * 1e3
         rms_b64, mx_b64 = rel(dF64.numpy().reshape(N, 3)[sample])
+        for rec in sweep:
+            th2_ = rec["theta"] ** 2
+            t64.init(dx, dy, dz)
+            sync()
+            a = time.perf_counter()
+            t64.init(dx, dy, dz)
+            t64.forces_all(th2_, 1.0, e2, out=dF64)
+            sync()
+            ms_ = (time.perf_counter() - a) * 1e3
+            r_, x_ = rel(dF64.numpy().reshape(N, 3)[sample])
+            rec.update({"fp64_reference_bh_ms_per_eval": ms_, "fp64_reference_bh_rms_rel_err": r_,
+                        "fp64_reference_bh_max_rel_err": x_,
+                        "mixed_speedup_vs_fp64_reference_bh": ms_ / rec["mixed_ms_per_eval"]})
         extras = {
             "fp64_bunched": {"ms_per_eval": ms_bun64, "particles_per_s": N / (ms_bun64 * 1e-3),
                              "rms_rel_err": rms_b64, "max_rel_err": mx_b64,
@@ -447,16 +509,37 @@ def main():
                                   "walk_ms": i64["ms_walk"]},
         }
 
+    # every seed of the input: median step time and the error on the sample
+    seeds = []
+    if not args.no_extras:
+        for sd in range(args.seeds):
+            if sd != args.seed:
+                if args.profile == "plummer":
+                    _, qs, _ = ics.plummer(N, a=1.0, rmax=10.0, seed=sd)
+                elif args.profile == "hernquist":
+                    _, qs, _ = ics.hernquist(N, a=1.0, rmax=50.0, seed=sd)
+                else:
+                    _, qs, _ = ics.uniform_cube(N, half=1.0, seed=sd)
+                for arr, k in ((dx, 0), (dy, 1), (dz, 2)):   # equal masses: only positions change
+                    col = np.ascontiguousarray(qs[:, k])   # held until the copy is done
+                    L.check(L.lib().hq_dev_copy(arr.ptr, C.c_void_p(col.ctypes.data), arr.nbytes, 0))
+            ms_sd = timed_steps(th2, args.seed_reps)
+            Fd_sd = direct_forces(dm, dx, dy, dz, 1.0, e2, targets=sample)
+            r_, x_ = rel(dF.numpy().reshape(N, 3)[sample], Fd_sd)
+            seeds.append({"seed": sd, "ms_per_step_median": ms_sd, "rms_rel_err": r_,
+                          "max_rel_err": x_})
+
     eval_ms = float(np.mean(evals))
     mp = {10: 1, 11: 4, 12: 8}[args.method]
-    if args.unfused:
+    walk = f"k_walk_mixed<{mp},{walk_stack}>"
+    if args.unfused or walk_stack == 0:
         kernel = f"k_eval_mixed<{mp}>"
     elif args.far > 0 and 0 < args.far_inner < args.far:   # the walk and both far-field tiers
-        kernel = f"k_walk_mixed<{mp},1152> + k_far<{mp},5> + k_far<{mp},6>"
+        kernel = f"{walk} + k_far<{mp},5> + k_far<{mp},6>"
     elif args.far > 0:   # the walk and its far-field kernel, timed together
-        kernel = f"k_walk_mixed<{mp},1152> + k_far<{mp},5>"
+        kernel = f"{walk} + k_far<{mp},5>"
     else:
-        kernel = f"k_walk_mixed<{mp},1152>"
+        kernel = walk
     order = {10: "monopole", 11: "quadrupole", 12: "octupole"}[args.method]
     workload = f"{args.profile}_n{N}_{order}_theta{args.theta}_eps{args.eps}"
     traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
@@ -471,17 +554,21 @@ def main():
     roof = None
     if flops_alg is not None:
         achieved = flops_alg / (eval_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "compute_unit": "VALU (FP32 FMA/rsqrt; no matrix ops)",
+        executed = (FLOPS_M2P[info["multipole"]] * info["exec_cells"]
+                    + FLOPS_P2P * info["exec_parts"]) / (eval_ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "compute_unit": "VALU (FP32 FMA/rsqrt; no matrix ops)",
                 "kernel": kernel,
                 "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                "frac_note": ("achieved prices the reference's per-target interaction counts "
+                              "(leaf size 1, same theta); executed_frac prices what the walk "
+                              "evaluates one body at a time (far cells excluded)"),
+                "executed_tflops": executed, "executed_frac": executed / FP32_PEAK_TFLOPS,
                 "flops_per_launch": flops_alg, "kernel_ms": eval_ms,
                 "list_entries": {"cells": info["list_cells"], "particles": info["list_parts"]},
                 "executed_interactions": {"cells": info["exec_cells"],
                                           "particles": info["exec_parts"],
                                           "far_cells_m2l": info["far_cells"]},
-                "executed_tflops": (FLOPS_M2P[info["multipole"]] * info["exec_cells"]
-                                    + FLOPS_P2P * info["exec_parts"]) / (eval_ms * 1e-3) / 1e12,
                 "bunches": info["ngroups"],
                 "longest_lists": {"cells": info["max_cells"], "particles": info["max_parts"],
                                   "stack": info["max_stack"]}}
@@ -502,11 +589,14 @@ def main():
                    "N": N, "seed": args.seed, "theta": args.theta, "eps": args.eps,
                    "order": order,
                    "precision": "mixed (FP32 interactions, FP64 positions/keys/moments)",
-                   "walk": (f"bunched Barnes-1990 lists, <= {args.bunch} particles per bunch, "
+                   "walk": (f"bunched Barnes-1990 lists ({args.mac} acceptance test), <= "
+                            f"{args.bunch} particles per bunch, "
                             f"cut from sibling subtrees packed to <= {args.super}, leaf bucket "
                             f"{args.leaf}, " + far_text),
                    "autotuned": tuned},
         "rms_rel_err": rms, "max_rel_err": mx, "err_sample": args.sample,
+        "rms_rel_err_worst_seed": max([s["rms_rel_err"] for s in seeds], default=rms),
+        "seeds": seeds, "theta_sweep": sweep,
         "ms_per_step_median": q50, "ms_per_step_iqr": [q25, q75],
         "host_arrays_ms_per_step": ms_host,
         "host_arrays_particles_per_s": N / (ms_host * 1e-3),

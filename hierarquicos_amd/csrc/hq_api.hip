@@ -192,6 +192,12 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
     } else if (key == HQ_TUNE_FAR_INNER) {
         if (value < 0 || value > 1000) return fail(HQ_ERR_ARGS, "inner far factor %lld outside 0..1000", (long long)value);
         t->far_inner = (int)value;
+    } else if (key == HQ_TUNE_MAC) {
+        if (value < 0 || value > 1) return fail(HQ_ERR_ARGS, "acceptance test %lld outside 0..1", (long long)value);
+        t->mac = (int)value;
+    } else if (key == HQ_TUNE_GRAPH_GRID) {
+        if (value < 0 || value > ((int64_t)1 << 30)) return fail(HQ_ERR_ARGS, "graph grid %lld out of range", (long long)value);
+        t->graph_grid = value;
     } else if (key == HQ_TUNE_LEAF) {
         if (value < 0 || value > 64) return fail(HQ_ERR_ARGS, "leaf bucket %lld outside 0..64", (long long)value);
         t->leaf = (int)value;
@@ -329,6 +335,7 @@ int hq_tree_info(const hq_tree* t, hq_tree_info_t* info) {
     info->max_parts = t->max_parts;
     info->max_stack = t->max_stack;
     info->split_walks = t->split_walks;
+    info->walk_stack = t->walk_stack;
     return HQ_OK;
 }
 

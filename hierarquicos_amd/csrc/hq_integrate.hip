@@ -260,7 +260,7 @@ static int integrate_tree_graph(hq_tree* t, int scheme, double dt, int64_t nstep
         if ((rc = forces())) return rc;
         kick_drift(t, true, dt, 1, dt);
     }
-    t->graph_nbun = std::min<int64_t>(n, 2 * t->ngroups + 64);
+    t->graph_nbun = std::min<int64_t>(n, t->graph_grid > 0 ? t->graph_grid : 2 * t->ngroups + 64);
     if ((rc = graph_prepare(t))) return rc;  // after graph_nbun: split buffers depend on it
     k_fill_i64<<<1, 64, 0, s>>>(t->gstate.as<int64_t>(), 0, GS_SIZE);
     t->graph_mode = true;

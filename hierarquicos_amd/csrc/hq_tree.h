@@ -61,7 +61,8 @@ struct hq_tree {
     bool build_timed = false;   // the last launched build's events await reading
     int far = 50;          // far-field factor of the fused mixed walk, in tenths (0: off)
     int far_inner = 30;    // inner far-field factor, in tenths (0: no inner tier)
-    bool stack_long = false;  // the fused walk's short stack overflowed: start with the long one
+    int stack_level = 0;   // the fused walk's first stack: 0 short, 1 long, 2 deep
+    int walk_stack = 0;    // LDS stack entries of the last mixed walk's kernel (0: two-kernel)
     int64_t far_cells = 0; // cells of the last walk taken through the far-field expansion
     int64_t split_lcap = 0;  // log entries per bunch (from the last walk's lists; 0 unknown)
     int64_t split_walks = 0; // walks evaluated split (diagnostic)
@@ -85,6 +86,8 @@ struct hq_tree {
     // the replays, to rerun them launched if an error was flagged.
     bool graph_mode = false;
     int64_t graph_nbun = 0;  // grid of the walk in graph mode (upper bound of the bunches)
+    int64_t graph_grid = 0;  // HQ_TUNE_GRAPH_GRID: that grid if > 0, else 2 x bunches + 64
+    int mac = 1;             // HQ_TUNE_MAC of the bunched walks: 0 cell side, 1 side + offset
     hq::DevBuf gstate, gsave;
     // status block: device + host mirror
     hq::DevBuf status;
@@ -149,14 +152,29 @@ constexpr int REC = 24;
 struct __align__(16) WalkNode {
     double cx, cy, cz, mass;   // centre of mass, mass
     int first, count, cstart;  // bodies [first, first + count), first child
-    int meta;                  // children | type << 4 | depth << 12
+    int meta;                  // children | type << 4 | depth << 12 | macq << 17
 };
+
+// Offset of a cell's centre of mass from its cube centre, |com - centre| /
+// side, rounded up to a multiple of 1/32767 and returned as that multiple
+// (0..32767; the offset is at most sqrt(3)/2 of the side).  Bunched walks with
+// the offset acceptance test open a cell unless side + theta |com - centre|
+// <= theta dist (Barnes 1994's remedy for cells whose centre of mass sits
+// near a face).
+__device__ __forceinline__ int mac_offset_q(double cx, double cy, double cz, double ox, double oy,
+                                            double oz, double side) {
+#pragma clang fp contract(off)
+    const double dx = cx - ox, dy = cy - oy, dz = cz - oz;
+    const double f = sqrt(dx * dx + dy * dy + dz * dz) / side;
+    const double q = ceil(f * 32767.0);
+    return q < 0.0 ? 0 : (q > 32767.0 ? 32767 : (int)q);
+}
 
 __device__ __forceinline__ void store_walk_records(int64_t k, int multipole, double mass,
                                                    double cx, double cy, double cz, int first,
                                                    int count, int cstart, int nch, int type,
-                                                   int depth, const double* Q, const double* O,
-                                                   float* __restrict__ rec,
+                                                   int depth, int macq, const double* Q,
+                                                   const double* O, float* __restrict__ rec,
                                                    WalkNode* __restrict__ wn) {
     float r[REC];
 #pragma unroll
@@ -184,7 +202,7 @@ __device__ __forceinline__ void store_walk_records(int64_t k, int multipole, dou
     w.first = first;
     w.count = count;
     w.cstart = cstart;
-    w.meta = (nch & 15) | ((type & 0xff) << 4) | (depth << 12);
+    w.meta = (nch & 15) | ((type & 0xff) << 4) | ((depth & 31) << 12) | ((macq & 0x7fff) << 17);
     wn[k] = w;
 }
 

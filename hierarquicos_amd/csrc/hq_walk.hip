@@ -196,6 +196,8 @@ constexpr int BUNCH = 64;
 constexpr double LIST_GIB_MAX = 48.0;  // device memory allowed for interaction lists
 constexpr int SCAP_FUSED = 1536;       // LDS walk stack of the fused mixed walk
 constexpr int SCAP_FAST = 1152;        // its first try outside graphs: 8.5 KB of LDS, 18 wavefronts per CU
+constexpr int SCAP_DEEP = 4096;        // its last try (small angles): 20 KB of LDS, 8 wavefronts per CU
+constexpr int SCAP_LEVEL[3] = {SCAP_FAST, SCAP_FUSED, SCAP_DEEP};
 
 struct ListOut {
     int* cidx;      // MODE_GROUP / MODE_BUNCH: node indices
@@ -213,8 +215,8 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
                                               const double* __restrict__ sz,
                                               const double* __restrict__ sm, int64_t n,
                                               const int* __restrict__ groups, int ngroups,
-                                              double theta2, int lbucket, ListOut out,
-                                              int64_t* __restrict__ status) {
+                                              double theta2, double mac_off, int lbucket,
+                                              ListOut out, int64_t* __restrict__ status) {
 #pragma clang fp contract(off)
     __shared__ int stk[SCAP];
     const int gi = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -290,7 +292,14 @@ __global__ __launch_bounds__(64) void k_lists(NodeView nv, const double* __restr
                         dist2 = dist2 + dd * dd;
                     }
                     const int depth = nv.depth[k];
-                    action = (gl2[depth] <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                    double lim = gl2[depth];
+                    if (MODE != MODE_GROUP && mac_off > 0.0) {  // the offset test
+                        const double side = ldexp(g[SG * sys + S_SIDE], -depth);
+                        const double fe = 1.0 + mac_off * (double)mac_offset_q(
+                            cc[0], cc[1], cc[2], nv.ncx[k], nv.ncy[k], nv.ncz[k], side);
+                        lim = lim * fe * fe;
+                    }
+                    action = (lim <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
                 }
             }
         }
@@ -494,6 +503,7 @@ __global__ __launch_bounds__(64) void k_eval_fp64(NodeView nv, const double* __r
 // The FP32 moment records and the packed traversal records (REC, WalkNode,
 // store_walk_records in hq_tree.h), from the node store, before each mixed walk.
 __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeView nv,
+                                               const double* __restrict__ g,
                                                float* __restrict__ rec,
                                                WalkNode* __restrict__ wn,
                                                const int64_t* __restrict__ dnn) {
@@ -507,9 +517,12 @@ __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeVi
     if (multipole > 4)
         for (int j = 0; j < 14; ++j) O[j] = nv.oct[j * S + k];
     const int cs = nv.cstart[k];
+    const int depth = (int)nv.depth[k];
+    const double side = ldexp(g[SG * nv.sys[nv.first[k]] + S_SIDE], -depth);
+    const int macq = mac_offset_q(nv.cx[k], nv.cy[k], nv.cz[k], nv.ncx[k], nv.ncy[k], nv.ncz[k], side);
     store_walk_records(k, multipole, nv.mass[k], nv.cx[k], nv.cy[k], nv.cz[k], nv.first[k],
-                       nv.count[k], cs, nv.cstart[k + 1] - cs, (int)(uint8_t)nv.type[k],
-                       (int)nv.depth[k], Q, O, rec, wn);
+                       nv.count[k], cs, nv.cstart[k + 1] - cs, (int)(uint8_t)nv.type[k], depth,
+                       macq, Q, O, rec, wn);
 }
 
 struct Kahan3 {
@@ -872,7 +885,8 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                                                    const double* __restrict__ sz,
                                                    const double* __restrict__ sm, int64_t n,
                                                    const int* __restrict__ bstart, int nbunch,
-                                                   double theta2, double G, float eps2, int lbucket,
+                                                   double theta2, double mac_off, double G,
+                                                   float eps2, int lbucket,
                                                    float far2, float far2i, int* __restrict__ flist,
                                                    int* __restrict__ fnum, int fcap,
                                                    const WalkNode* __restrict__ wn,
@@ -977,8 +991,13 @@ __global__ __launch_bounds__(64) void k_walk_mixed(NodeView nv, const double* __
                         const double dd = fmax(fmax(bmin[c] - cc[c], 0.0), cc[c] - bmax[c]);
                         dist2 = fma(dd, dd, dist2);
                     }
-                    const int depth = w1.w >> 12;
-                    action = (ldexp(l2root, -2 * depth) <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
+                    const int depth = (w1.w >> 12) & 31;
+                    double lim = ldexp(l2root, -2 * depth);
+                    if (mac_off > 0.0) {  // the offset test: side + theta |com - centre|
+                        const double fe = 1.0 + mac_off * (double)((unsigned)w1.w >> 17);
+                        lim = lim * fe * fe;
+                    }
+                    action = (lim <= theta2 * dist2 || depth >= MAX_DEPTH) ? 3 : 1;
                     if (action == 3) {
                         double R2 = 0.0;
 #pragma unroll
@@ -1461,12 +1480,15 @@ __global__ void k_set_end(int* __restrict__ a, const uint32_t* __restrict__ pos,
 }
 
 // Graph mode: the bunch count for the walk's grid test, and a flag when the
-// grid (sized from a launched step) is too small for it.
+// grid (sized from a launched step) is too small for it.  The count the
+// kernels read is clamped to the grid, for which every per-bunch buffer is
+// sized; the flag makes the integrator redo the steps launched.
 __global__ void k_graph_bunches(const uint32_t* __restrict__ nb, int64_t grid,
                                 int64_t* __restrict__ gs) {
     if (threadIdx.x != 0) return;
-    gs[GS_NBUN] = *nb;
-    if ((int64_t)*nb > grid) gs[GS_STICKY] |= GF_BUNCHES;
+    const int64_t c = *nb;
+    gs[GS_NBUN] = c < grid ? c : grid;
+    if (c > grid) gs[GS_STICKY] |= GF_BUNCHES;
 }
 
 // Graph mode: the walk's stack (or split log) overflow, kept over replayed steps.
@@ -1491,6 +1513,13 @@ static int split_bcap(int64_t lcap) { return (int)(lcap / 64 + 3); }
 static bool inner_on(const hq_tree* t) {
     return t->far > 0 && t->far_inner > 0 && t->far_inner < t->far;
 }
+// Scale of the offset acceptance test of the bunched walks: theta / 32767
+// (the walk records hold the offset in multiples of 1/32767 of the side);
+// 0 for the plain cell-side test.
+static double mac_off(const hq_tree* t, double theta2) {
+    return t->mac ? std::sqrt(theta2) / 32767.0 : 0.0;
+}
+
 static float far_scale(double theta2) {
     return theta2 > 0.0 && theta2 < 0.25 ? (float)(0.5 / std::sqrt(theta2)) : 1.0f;
 }
@@ -1572,7 +1601,7 @@ static int launch_split(hq_tree* t, int grid, const int64_t* dnbun, double theta
     auto record = [&](auto kern) {
         kern<<<grid, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                  t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
-                                 t->bstart.as<int>(), grid, theta2, G, (float)eps2, t->leaf,
+                                 t->bstart.as<int>(), grid, theta2, mac_off(t, theta2), G, (float)eps2, t->leaf,
                                  far2(t, theta2), far2i(t, theta2), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                                  t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(),
                                  t->fx.as<double>(), t->fy.as<double>(),
@@ -1633,7 +1662,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         if ((rc = t->mom32.ensure((size_t)t->cap * REC * sizeof(float)))) return rc;
         if ((rc = t->wnode.ensure((size_t)t->cap * sizeof(WalkNode)))) return rc;
         k_mom32<<<grid_for(t->cap, 256), 256, 0, s>>>(t->cap, t->multipole, node_view(t),
-                                                      t->mom32.as<float>(),
+                                                      t->geom.as<double>() + G_SYS, t->mom32.as<float>(),
                                                       t->wnode.as<WalkNode>(), gs + GS_NN);
         k_fill_i64<<<1, 64, 0, s>>>(t->status.as<int64_t>(), 0, ST_SIZE);
         if (t->graph_split) {
@@ -1643,7 +1672,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 kern<<<(int)t->graph_nbun, 64, 0, s>>>(
                     node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                     t->sy.as<double>(), t->sz.as<double>(), t->sm.as<double>(), t->n,
-                    t->bstart.as<int>(), (int)t->graph_nbun, theta2, G, (float)eps2, t->leaf,
+                    t->bstart.as<int>(), (int)t->graph_nbun, theta2, mac_off(t, theta2), G, (float)eps2, t->leaf,
                     far2(t, theta2), far2i(t, theta2), t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                     t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(), t->fx.as<double>(),
                     t->fy.as<double>(),
@@ -1655,6 +1684,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             launch_far(t, (int)t->graph_nbun, gs + GS_NBUN, G, eps2);
         }
         k_graph_walk_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), gs);
+        t->walk_stack = SCAP_FUSED;
         HQ_CHECK_HIP(hipGetLastError());
         return HQ_OK;
     }
@@ -1662,7 +1692,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
         if ((rc = t->wnode.ensure((size_t)t->nn * sizeof(WalkNode)))) return rc;
         k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
-                                                     t->mom32.as<float>(),
+                                                     t->geom.as<double>() + G_SYS, t->mom32.as<float>(),
                                                      t->wnode.as<WalkNode>(), nullptr);
         if (t->fused && use_split(t, ngroups)) {
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
@@ -1675,6 +1705,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             note_lists(t);
             if (!(fl & (16 | 32))) {
                 ++t->split_walks;
+                t->walk_stack = SCAP_FUSED;
                 hipEventElapsedTime(&t->ms_eval, t->ev[5], t->ev[8]);
                 t->ms_lists = 0.0f;
                 t->ms_walk = t->ms_eval;
@@ -1691,10 +1722,13 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             // the log or the stack overflowed: the fused walk below
         }
         // the fused walk with the short stack, then (if a bunch's stack
-        // overflowed) with the long one; it writes every force and far list
-        // afresh, so a second try starts clean.  After an overflow the handle
-        // starts with the long stack until a walk needs well under the short.
-        for (int attempt = t->stack_long ? 1 : 0; t->fused && attempt < 2; ++attempt) {
+        // overflowed) with the long one, then the deep one; it writes every
+        // force and far list afresh, so a retry starts clean.  The handle
+        // starts at the stack that last sufficed and steps back down once a
+        // walk needs well under the next smaller one; when even the deep
+        // stack overflowed it goes straight to the two-kernel walk until that
+        // walk needs less.
+        for (int attempt = t->stack_level; t->fused && attempt < 3; ++attempt) {
             if ((rc = far_buffers(t, ngroups))) return rc;
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
@@ -1702,7 +1736,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 kern<<<ngroups, 64, 0, s>>>(node_view(t), t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                             t->sy.as<double>(), t->sz.as<double>(),
                                             t->sm.as<double>(), t->n, t->bstart.as<int>(), ngroups,
-                                            theta2, G, (float)eps2, t->leaf, far2(t, theta2), far2i(t, theta2),
+                                            theta2, mac_off(t, theta2), G, (float)eps2, t->leaf, far2(t, theta2), far2i(t, theta2),
                                             t->flist.as<int>(), t->fnum.as<int>(), far_cap(t),
                                             t->wnode.as<WalkNode>(), t->sbody.as<double4>(), t->mom32.as<float>(),
                                             t->fx.as<double>(), t->fy.as<double>(),
@@ -1713,20 +1747,25 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FAST>);
                 else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FAST>);
                 else launch(k_walk_mixed<8, SCAP_FAST>);
-            } else {
+            } else if (attempt == 1) {
                 if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_FUSED>);
                 else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_FUSED>);
                 else launch(k_walk_mixed<8, SCAP_FUSED>);
+            } else {
+                if (t->multipole == 1) launch(k_walk_mixed<1, SCAP_DEEP>);
+                else if (t->multipole == 4) launch(k_walk_mixed<4, SCAP_DEEP>);
+                else launch(k_walk_mixed<8, SCAP_DEEP>);
             }
             launch_far(t, ngroups, nullptr, G, eps2);
             HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
             if ((rc = read_status(t))) return rc;
             HQ_CHECK_HIP(hipGetLastError());
-            if (attempt == 0 && (t->hstatus[ST_FLAGS] & 16)) t->stack_long = true;
-            if (attempt == 1 && !(t->hstatus[ST_FLAGS] & 16) &&
-                t->hstatus[ST_MAXSTK] <= SCAP_FAST - 128)
-                t->stack_long = false;
-            if (!(t->hstatus[ST_FLAGS] & 16)) {
+            const bool over = t->hstatus[ST_FLAGS] & 16;
+            if (over) t->stack_level = std::min(attempt + 1, 2);
+            else if (attempt > 0 && t->hstatus[ST_MAXSTK] <= SCAP_LEVEL[attempt - 1] - 128)
+                t->stack_level = attempt - 1;
+            if (!over) {
+                t->walk_stack = SCAP_LEVEL[attempt];
                 note_lists(t);
                 hipEventElapsedTime(&t->ms_eval, t->ev[5], t->ev[8]);
                 t->ms_lists = 0.0f;
@@ -1741,7 +1780,8 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                 t->max_stack = t->hstatus[ST_MAXSTK];
                 return HQ_OK;
             }
-            // a stack overflowed: the long stack, then the two-kernel walk
+            // a stack overflowed: the next larger one; past the deep stack the
+            // two-kernel walk, whose stack is as deep, reports the overflow
         }
     }
     if ((rc = t->lstat.ensure((size_t)ngroups * 2 * sizeof(int)))) return rc;
@@ -1767,19 +1807,20 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
             k_lists<MODE_GROUP><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
                                                        t->sm.as<double>(), t->n,
-                                                       t->groups.as<int>(), ngroups, theta2, 0, lo,
+                                                       t->groups.as<int>(), ngroups, theta2, 0.0, 0, lo,
                                                        t->status.as<int64_t>());
         else if (!mixed)
             k_lists<MODE_BUNCH><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS, t->sx.as<double>(),
                                                        t->sy.as<double>(), t->sz.as<double>(),
                                                        t->sm.as<double>(), t->n, t->bstart.as<int>(),
-                                                       ngroups, theta2, t->leaf, lo, t->status.as<int64_t>());
+                                                       ngroups, theta2, mac_off(t, theta2), t->leaf, lo, t->status.as<int64_t>());
         else
             k_lists<MODE_BUNCH32><<<ngroups, 64, 0, s>>>(nv, t->geom.as<double>() + G_SYS,
                                                          t->sx.as<double>(), t->sy.as<double>(),
                                                          t->sz.as<double>(), t->sm.as<double>(),
                                                          t->n, t->bstart.as<int>(), ngroups, theta2,
-                                                         t->leaf, lo, t->status.as<int64_t>());
+                                                         mac_off(t, theta2), t->leaf, lo,
+                                                         t->status.as<int64_t>());
         HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
         if ((rc = read_status(t))) return rc;
         const int64_t flags = t->hstatus[ST_FLAGS];
@@ -1794,6 +1835,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
                             "cells, %lld particles)", gib, (long long)needc, (long long)needp);
             continue;  // lists were truncated: rebuild with room for the largest
         }
+        t->walk_stack = 0;
         t->list_cells = t->hstatus[ST_LISTC];
         t->list_parts = t->hstatus[ST_LISTP];
         t->far_cells = t->hstatus[ST_LISTF];

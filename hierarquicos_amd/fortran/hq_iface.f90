@@ -21,7 +21,8 @@ MODULE hq_iface
     ! tuning keys
     INTEGER(C_INT), PARAMETER :: HQ_TUNE_BUNCH = 1, HQ_TUNE_SUPER = 2, HQ_TUNE_FUSED = 3
     INTEGER(C_INT), PARAMETER :: HQ_TUNE_LEAF = 4, HQ_TUNE_FMM_LEAF = 5, HQ_TUNE_GRAPH = 6, &
-                                   HQ_TUNE_SPLIT = 7, HQ_TUNE_FAR = 8, HQ_TUNE_FAR_INNER = 9
+                                   HQ_TUNE_SPLIT = 7, HQ_TUNE_FAR = 8, HQ_TUNE_FAR_INNER = 9, &
+                                   HQ_TUNE_GRAPH_GRID = 10, HQ_TUNE_MAC = 11
     ! FMM modes
     INTEGER(C_INT), PARAMETER :: HQ_FMM_MFMA = 0, HQ_FMM_FP64 = 1
     ! integration
@@ -55,7 +56,7 @@ MODULE hq_iface
         REAL(C_DOUBLE) :: ms_build, ms_keys, ms_sort, ms_nodes, ms_upward
         REAL(C_DOUBLE) :: ms_walk, ms_lists, ms_eval
         INTEGER(C_INT64_T) :: exec_cells, exec_parts, max_cells, max_parts, max_stack
-        INTEGER(C_INT64_T) :: split_walks, far_cells
+        INTEGER(C_INT64_T) :: split_walks, far_cells, walk_stack
     END TYPE
 
     INTERFACE

@@ -58,32 +58,50 @@ def _finish(m, q, v):
     return m, q, v
 
 
+def plummer_inverse_cdf(x, a=1.0):
+    """r with M(r) / M_0 = x for the Plummer sphere (plummer.f90:53-62)."""
+    return a * x ** (1.0 / 3.0) / np.sqrt(1.0 - x ** (2.0 / 3.0))
+
+
+def hernquist_inverse_cdf(x, a=1.0):
+    """r with M(r) / M_0 = x for the Hernquist profile (hernquist.f90:53-66)."""
+    s = np.sqrt(x)
+    return a * s / (1.0 - s)
+
+
+def _plummer_speed_ratios(rng, n):
+    """q = v / v_esc by von Neumann rejection on q^2 (1 - q^2)^3.5 under the
+    reference's bound (initial_values.f90:237-257)."""
+    qv = np.empty(n)
+    todo = np.arange(n)
+    while len(todo):
+        x = rng.random(len(todo))
+        y = _PLUMMER_PDF_MAX * rng.random(len(todo))
+        ok = y < x * x * (1.0 - x * x) ** 3.5
+        qv[todo[ok]] = x[ok]
+        todo = todo[~ok]
+    return qv
+
+
 def plummer(n, a=1.0, rmax=10.0, seed=0, G=1.0, velocities=False):
     """Plummer sphere of scale radius ``a``, truncated at ``rmax``."""
     rng = np.random.default_rng(seed)
     m = np.full(n, 1.0 / n)
-
-    def inv(x):
-        return a * x ** (1.0 / 3.0) / np.sqrt(1.0 - x ** (2.0 / 3.0))
-
-    r = _radii(rng, n, inv, rmax)
+    r = _radii(rng, n, lambda x: plummer_inverse_cdf(x, a), rmax)
     q = r[:, None] * _directions(rng, n)
     v = None
     if velocities:
         vesc = np.sqrt(2.0 * G / np.sqrt(r * r + a * a))
-        qv = np.empty(n)
-        todo = np.arange(n)
-        while len(todo):
-            x = rng.random(len(todo))
-            y = _PLUMMER_PDF_MAX * rng.random(len(todo))
-            ok = y < x * x * (1.0 - x * x) ** 3.5
-            qv[todo[ok]] = x[ok]
-            todo = todo[~ok]
+        qv = _plummer_speed_ratios(rng, n)
         v = (qv * vesc)[:, None] * _directions(rng, n)
     return _finish(m, q, v)
 
 
 def _hernquist_edf(r, x, a):
+    """hernquist_edf_norm of hernquist.f90:95-107 as written there (G = M = 1):
+    its E is sqrt(-Phi (1 - x^2)), and the second term of the bracket is
+    scaled by the first; both differ from Hernquist's (1990) closed form, and
+    the sampler follows the reference."""
     e = np.sqrt((1.0 / (a + r)) * (1.0 - x * x))
     f = 1.0 / (8.0 * np.sqrt(2.0) * a * np.pi ** 3)
     f = f / np.sqrt(1.0 - e) ** 5
@@ -95,12 +113,7 @@ def hernquist(n, a=1.0, rmax=50.0, seed=0, G=1.0, velocities=False):
     """Hernquist profile (cusp rho ~ 1/r), truncated at ``rmax``."""
     rng = np.random.default_rng(seed)
     m = np.full(n, 1.0 / n)
-
-    def inv(x):
-        s = np.sqrt(x)
-        return a * s / (1.0 - s)
-
-    r = _radii(rng, n, inv, rmax)
+    r = _radii(rng, n, lambda x: hernquist_inverse_cdf(x, a), rmax)
     q = r[:, None] * _directions(rng, n)
     v = None
     if velocities:

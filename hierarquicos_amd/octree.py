@@ -125,7 +125,8 @@ class OctreeType:
 
     def forces_bunched(self, theta2, G=1.0, eps2=0.0, precision="mixed", out=None):
         """Barnes (1990) shared-list walk over 64-particle bunches in tree order
-        (FP64 or mixed precision); (3, N)."""
+        (FP64 or mixed precision), by default with the offset acceptance test
+        (tune(mac=...)); (3, N)."""
         prec = {"fp64": L.HQ_FP64, "mixed": L.HQ_MIXED}[precision]
         lib = L.lib()
 
@@ -145,7 +146,7 @@ class OctreeType:
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
     def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None,
-             graph=None, split=None, far=None, far_inner=None):
+             graph=None, split=None, far=None, far_inner=None, graph_grid=None, mac=None):
         """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
         bucket (cells of <= leaf bodies enter as their bodies) of the bunched
         walk; FMM leaf size; graph: whether integrate() replays one captured
@@ -154,8 +155,15 @@ class OctreeType:
         results); far: the mixed walk's far-field factor in bunch radii (default
         5.0; 0 off): farther accepted cells enter the bunch's local expansion;
         far_inner (default 3.0; 0 off): cells between far_inner and far radii
-        enter an expansion of higher order."""
+        enter an expansion of higher order; graph_grid: the replayed walk's grid
+        in bunches (0, default: twice the first step's bunch count plus 64);
+        mac: the bunched walks' acceptance test, "side" (the grouped test of
+        octree_default.f90:1927-1937) or "offset" (side + theta |com - centre|)."""
         lib = L.lib()
+        if mac is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_MAC, {"side": 0, "offset": 1}[mac]))
+        if graph_grid is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_GRAPH_GRID, int(graph_grid)))
         if far is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FAR, int(round(10 * float(far)))))
         if far_inner is not None:

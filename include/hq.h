@@ -96,7 +96,8 @@ int hq_tree_forces(hq_tree *t, double theta2, double G, double eps2, double *F, 
                    int where);
 /* Mixed-precision bunch walk of the same tree: contiguous 64-particle
  * bunches in tree order with their tight bounding boxes under the grouped
- * (Barnes 1990) acceptance test of octree_default.f90:1927-1937. */
+ * (Barnes 1990) acceptance test of octree_default.f90:1927-1937, by default
+ * with the cell's centre-of-mass offset added to its size (HQ_TUNE_MAC). */
 int hq_tree_forces_bunched(hq_tree *t, double theta2, double G, double eps2, double *F,
                            int layout, int where, int precision);
 int hq_tree_destroy(hq_tree *t);
@@ -132,6 +133,18 @@ int hq_tree_destroy(hq_tree *t);
                               and HQ_TUNE_FAR/10 bunch radii take a local expansion of
                               higher orders (6 / 4 / 3); 0..1000, default 30; 0, or a value
                               not below HQ_TUNE_FAR, switches the tier off */
+#define HQ_TUNE_GRAPH_GRID 10 /* graph replay of bunched steps: the captured walk's grid in
+                              bunches; 0 (default): twice the launched first step's bunch count
+                              plus 64.  A replayed step with more bunches than the grid is
+                              flagged and all steps are redone launched (same results) */
+#define HQ_TUNE_MAC 11     /* acceptance test of the bunched walks (fused, split, two-kernel,
+                              FP64 or mixed): 0 the grouped test of octree_default.f90:1927-1937,
+                              side^2 <= theta^2 dist^2 with dist from the centre of mass to the
+                              bunch's box; 1 (default) the offset test
+                              (side + theta |com - centre|)^2 <= theta^2 dist^2, which also
+                              opens cells whose centre of mass sits near a face (at 1M Plummer
+                              bodies, octupole, theta 0.5: 18% lower RMS error for 4% more
+                              time).  Methods 30-32 keep the reference's test */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
 /* Cartesian FMM of order 4 or 6 on the built tree: Dehnen's method carried to
@@ -250,6 +263,8 @@ typedef struct {
     int64_t max_stack;     /* deepest walk stack */
     int64_t split_walks;   /* mixed walks of this handle evaluated split (HQ_TUNE_SPLIT) */
     int64_t far_cells;     /* cells of the last mixed walk taken through the far field (HQ_TUNE_FAR) */
+    int64_t walk_stack;    /* LDS stack entries per wavefront of the last mixed bunched walk's
+                              kernel (1152, 1536 or 4096; 0 for the two-kernel walk) */
 } hq_tree_info_t;
 int hq_tree_info(const hq_tree *t, hq_tree_info_t *info);
 

@@ -89,6 +89,25 @@ def test_bunched_graph_replay_matches_launched_steps(scheme):
         q, p = qg, pg
 
 
+@pytest.mark.parametrize("split", [0, 1])
+def test_bunched_graph_grid_too_small_reruns_launched(split):
+    """A replayed step with more bunches than the captured walk's grid (forced
+    here: a grid of 8 bunches for about 80) flags the replays; the kernels
+    walk only the bunches the grid holds, and the steps are then redone
+    launched, which gives the launched path's state bit for bit."""
+    n = 5000
+    m, q, v = ics.plummer(n, seed=21, velocities=True)
+    p = v * m[:, None]
+    tg = OctreeType().pre_init(m, 12, precision="mixed").tune(graph_grid=8, split=split)
+    te = OctreeType().pre_init(m, 12, precision="mixed").tune(graph=False, split=split)
+    for scheme in ("verlet", "euler"):
+        tg.set_state(q, p).integrate(1e-3, 5, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
+        te.set_state(q, p).integrate(1e-3, 5, 0.25, 1.0, 1e-4, scheme=scheme, force="bunched")
+        (qg, pg), (qe, pe) = tg.get_state(), te.get_state()
+        assert np.array_equal(qg, qe) and np.array_equal(pg, pe), scheme
+        assert tg.info()["ngroups"] > 8
+
+
 def test_bunched_graph_replay_tiny_systems():
     """Graph replay and split walk with systems of 1, 2, 3 and 65 bodies in one
     batch (single-leaf roots, bunches smaller than a wavefront)."""

@@ -148,8 +148,9 @@ def test_fused_walk_stack_retries():
     """The fused walk first runs with a short LDS stack; when a bunch
     overflows it, the walk runs again with the long stack (the same bits as
     the split walk, which always has the long one) and the handle starts
-    there next time.  When the long stack overflows too, the two-kernel walk
-    (no far field) gives the forces."""
+    there next time.  When the long stack overflows too, the deep stack
+    (4096 entries) takes the walk, far field included, and the handle starts
+    there until a walk needs well under the long one."""
     m, q, _ = ics.plummer(20000, seed=5)
     x, y, z = (np.ascontiguousarray(q[:, c]) for c in range(3))
     ta = OctreeType().pre_init(m, 12, precision="mixed").tune(split=0).init(x, y, z)
@@ -161,13 +162,27 @@ def test_fused_walk_stack_retries():
     Fb = tb.forces_bunched(0.25, 1.0, 1e-6, "mixed")
     assert tb.info()["split_walks"] == 1
     assert np.array_equal(Fa1, Fb) and np.array_equal(Fa2, Fb)
-    # deeper than either stack: the two-kernel walk, within the budget
+    assert ta.info()["walk_stack"] == 1536 and tb.info()["walk_stack"] == 1536
+    # deeper than either: the deep stack, within the budget
     m, x, y, z, t = _tree(200000, 25)
     F = t.forces_bunched(0.1, 1.0, 1e-6, "mixed").T
-    assert t.info()["far_cells"] == 0
+    inf = t.info()
+    assert inf["far_cells"] > 0 and inf["walk_stack"] == 4096 and inf["max_stack"] > 1536
     sample = np.arange(0, 200000, 101)
     Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
     assert O.rel_errors(F[sample], Fd)[0] < 2e-5
+    # the handle starts with the deep stack next time (the same forces), and
+    # steps back down at a wider angle
+    assert np.array_equal(t.forces_bunched(0.1, 1.0, 1e-6, "mixed").T, F)
+    assert t.info()["walk_stack"] == 4096
+    t.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    t.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert t.info()["walk_stack"] < 4096
+    # the two-kernel walk (no far field) stays within the budget too
+    t.tune(fused=False)
+    F2 = t.forces_bunched(0.1, 1.0, 1e-6, "mixed").T
+    assert t.info()["far_cells"] == 0 and t.info()["walk_stack"] == 0
+    assert O.rel_errors(F2[sample], Fd)[0] < 2e-5
 
 
 def test_headline_settings_within_budget():
