@@ -340,7 +340,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 }
 
 // Runs of equal high words (particles sharing levels 1..21) are ordered by
-// the low word (levels 22..25), stably.  Such runs are rare and short.
+// the low word (levels 22..25), stably, one run per thread.  Such runs are
+// rare and short; a longer one flags the build, which then sorts by the full
+// key instead (full_key_sort).
+constexpr int64_t TIE_RUN_MAX = 64;
 __global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __restrict__ klo,
                           int* __restrict__ idx, const int* __restrict__ sysid, int64_t n,
                           int64_t* __restrict__ status) {
@@ -353,7 +356,7 @@ __global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __re
     if (i > 0 && same(i - 1, i)) return;
     int64_t j = i + 1;
     while (j + 1 < n && same(j + 1, i)) ++j;
-    if (j - i + 1 > 4096) {
+    if (j - i + 1 > TIE_RUN_MAX) {
         atomicOr((unsigned long long*)&status[ST_FLAGS], 4ull);
         return;
     }
@@ -910,6 +913,100 @@ static int radix_sort(hq_tree* t) {
                             t->idx2.as<int>(), t->n, 63, t->hist, t->scanpart, t->stream);
 }
 
+// Gather the sorted bodies, then the nodes from the common prefixes of
+// neighbouring keys (levels in order, tree order within a level).
+template <class Mark>
+static int gather_and_emit(hq_tree* t, int64_t cap, Mark&& mark) {
+    hipStream_t s = t->stream;
+    const int64_t n = t->n;
+    const double* sg = t->geom.as<double>() + G_SYS;
+    int rc;
+    k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->prec.as<double4>(),
+                                              t->paux.as<uint2>(), n, t->sx.as<double>(),
+                                              t->sy.as<double>(), t->sz.as<double>(),
+                                              t->sm.as<double>(), t->sklo.as<uint32_t>(),
+                                              t->ssys.as<int>(), t->sbody.as<double4>());
+    HQ_CHECK_HIP(mark(2));
+    // 4. nodes
+    k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),
+                                           t->ssys.as<int>(), n, t->lcp.as<int8_t>());
+    const int nblk = grid_for(n, NB_THREADS);
+    if ((rc = t->hist.ensure(((size_t)NDEP * nblk + 1) * sizeof(uint32_t)))) return rc;
+    k_node_count<<<nblk, NB_THREADS, 0, s>>>(t->lcp.as<int8_t>(), n, t->hist.as<uint32_t>(), nblk,
+                                             t->status.as<int64_t>());
+    if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), (int64_t)NDEP * nblk, t->scanpart, s))) return rc;
+    k_node_emit<<<nblk, NB_THREADS, 0, s>>>(
+        t->lcp.as<int8_t>(), t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), sg,
+        t->ssys.as<int>(), n, t->hist.as<uint32_t>(), nblk, cap, t->ndepth.as<uint8_t>(), t->ntype.as<int8_t>(),
+        t->nfirst.as<int>(), t->ncstart.as<int>(), t->ndigit.as<uint8_t>(), t->ncx.as<double>(),
+        t->ncy.as<double>(), t->ncz.as<double>(), t->nkhi.as<uint64_t>(), t->nklo.as<uint32_t>(),
+        t->status.as<int64_t>());
+    k_levels<<<1, 64, 0, s>>>(t->hist.as<uint32_t>(), nblk, t->status.as<int64_t>());
+    HQ_CHECK_HIP(mark(3));
+    HQ_CHECK_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+// A batch: stably by system after the key sort (the keys ride along from khio).
+static int sort_by_system(hq_tree* t) {
+    hipStream_t s = t->stream;
+    const int64_t n = t->n;
+    int bits = 1;
+    while ((1ll << bits) < t->nsys) ++bits;
+    k_syskey<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->sysid.as<int>(), n,
+                                              t->khi.as<uint64_t>());
+    int rc;
+    if ((rc = radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
+                               t->idx2.as<int>(), n, bits, t->hist, t->scanpart, s)))
+        return rc;
+    k_restore_keys<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->khio.as<uint64_t>(), n,
+                                                    t->khi.as<uint64_t>());
+    HQ_CHECK_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
+// the low words (levels 22..25) as sort keys, with the identity permutation
+__global__ __launch_bounds__(256) void k_lo_keys(const uint32_t* __restrict__ klo, int64_t n,
+                                                 uint64_t* __restrict__ key,
+                                                 int* __restrict__ idx) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = klo[i];
+    idx[i] = (int)i;
+}
+
+// The full 25-level order, for inputs whose runs of equal high words are too
+// long for k_tie_fix: a stable sort by the low word, then (keys carried from
+// khio, the original-order high words) a stable sort by the high word, then
+// by system for a batch.  The same order k_tie_fix gives to shorter runs.
+static int full_key_sort(hq_tree* t) {
+    hipStream_t s = t->stream;
+    const int64_t n = t->n;
+    const int* sysid = t->nsys > 1 ? t->sysid.as<int>() : nullptr;
+    const double* sg = t->geom.as<double>() + G_SYS;
+    int rc;
+    if ((rc = t->khio.ensure(n * 8))) return rc;
+    k_keys<<<grid_for(n, 256), 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(),
+                                            t->z.as<double>(), n, sg, sysid,
+                                            t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
+                                            t->idx.as<int>(), t->khio.as<uint64_t>(),
+                                            t->m.as<double>(), t->prec.as<double4>(),
+                                            t->paux.as<uint2>());
+    k_lo_keys<<<grid_for(n, 256), 256, 0, s>>>(t->klo.as<uint32_t>(), n, t->khi.as<uint64_t>(),
+                                               t->idx.as<int>());
+    if ((rc = radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
+                               t->idx2.as<int>(), n, 12, t->hist, t->scanpart, s)))
+        return rc;
+    k_restore_keys<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->khio.as<uint64_t>(), n,
+                                                    t->khi.as<uint64_t>());
+    if ((rc = radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
+                               t->idx2.as<int>(), n, 63, t->hist, t->scanpart, s)))
+        return rc;
+    if (t->nsys > 1 && (rc = sort_by_system(t))) return rc;
+    HQ_CHECK_HIP(hipGetLastError());
+    return HQ_OK;
+}
+
 int tree_build(hq_tree* t) {
     // the previous build's phase times, before this build records its events
     if (!capturing()) collect_build_times(t);
@@ -985,48 +1082,25 @@ int tree_build(hq_tree* t) {
     HQ_CHECK_HIP(mark(1));
     // 3. sort: by octant key; a batch then stably by system
     if ((rc = radix_sort(t))) return rc;
-    if (batch) {
-        int bits = 1;
-        while ((1ll << bits) < nsys) ++bits;
-        k_syskey<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), sysid, n,
-                                                  t->khi.as<uint64_t>());
-        if ((rc = radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(),
-                                   t->khi2.as<uint64_t>(), t->idx2.as<int>(), n, bits, t->hist,
-                                   t->scanpart, s)))
-            return rc;
-        k_restore_keys<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->khio.as<uint64_t>(),
-                                                        n, t->khi.as<uint64_t>());
-    }
+    if (batch && (rc = sort_by_system(t))) return rc;
     k_tie_fix<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
                                                t->idx.as<int>(), sysid, n,
                                                t->status.as<int64_t>());
-    k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->prec.as<double4>(),
-                                              t->paux.as<uint2>(), n, t->sx.as<double>(),
-                                              t->sy.as<double>(), t->sz.as<double>(),
-                                              t->sm.as<double>(), t->sklo.as<uint32_t>(),
-                                              t->ssys.as<int>(), t->sbody.as<double4>());
-    HQ_CHECK_HIP(mark(2));
-    // 4. nodes
-    k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),
-                                           t->ssys.as<int>(), n, t->lcp.as<int8_t>());
-    const int nblk = grid_for(n, NB_THREADS);
-    if ((rc = t->hist.ensure(((size_t)NDEP * nblk + 1) * sizeof(uint32_t)))) return rc;
-    k_node_count<<<nblk, NB_THREADS, 0, s>>>(t->lcp.as<int8_t>(), n, t->hist.as<uint32_t>(), nblk,
-                                             t->status.as<int64_t>());
-    if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), (int64_t)NDEP * nblk, t->scanpart, s))) return rc;
-    k_node_emit<<<nblk, NB_THREADS, 0, s>>>(
-        t->lcp.as<int8_t>(), t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(), sg,
-        t->ssys.as<int>(), n, t->hist.as<uint32_t>(), nblk, cap, t->ndepth.as<uint8_t>(), t->ntype.as<int8_t>(),
-        t->nfirst.as<int>(), t->ncstart.as<int>(), t->ndigit.as<uint8_t>(), t->ncx.as<double>(),
-        t->ncy.as<double>(), t->ncz.as<double>(), t->nkhi.as<uint64_t>(), t->nklo.as<uint32_t>(),
-        t->status.as<int64_t>());
-    k_levels<<<1, 64, 0, s>>>(t->hist.as<uint32_t>(), nblk, t->status.as<int64_t>());
+    if ((rc = gather_and_emit(t, cap, mark))) return rc;
     if (gm) return build_graph_tail(t, cap, nsys);
-    HQ_CHECK_HIP(hipEventRecord(t->ev[3], s));
     if ((rc = read_status(t))) return rc;
     int64_t flags = t->hstatus[ST_FLAGS];
     if (flags & 1) return fail(HQ_ERR_NONFINITE, "non-finite input position");
-    if (flags & 4) return fail(HQ_ERR_CAPACITY, "more than 4096 particles share 21 levels");
+    if (flags & 4) {
+        // a run of more than TIE_RUN_MAX bodies shares levels 1..21 (a cell
+        // 2^-21 of the root wide): sort by the full 25-level key instead --
+        // levels 22..25 first, then 1..21 -- and build the nodes again
+        if ((rc = full_key_sort(t))) return rc;
+        HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
+        if ((rc = gather_and_emit(t, cap, mark))) return rc;
+        if ((rc = read_status(t))) return rc;
+        flags = t->hstatus[ST_FLAGS];
+    }
     t->nn = t->hstatus[ST_NNODES];
     t->lost = t->hstatus[ST_LOST];
     if (flags & 2)

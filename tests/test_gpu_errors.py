@@ -1,0 +1,88 @@
+"""GPU: the reference's STOPs as status codes, and inputs at the edges of
+the build (SURVEY's failure handling: capacity, depth, ties, lost bodies).
+
+* Node capacity (3N for N >= 1000, octree_default.f90:219-223): close pairs
+  whose chains of nodes exceed it give HQ_ERR_CAPACITY, where the reference
+  stops with "INSUFFICIENT NUMBER OF NODES" (:457-460); the oracle refuses
+  the same input.
+* Many bodies sharing all 21 levels of the high key word: the build sorts by
+  the full 25-level key and matches the oracle's insertion tree (the
+  reference drops bodies that still share a cell at depth 25, :570-579).
+* Morton depth outside 1..21 or keys not distinct at 21 levels: HQ_ERR_DEPTH.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from hierarquicos_amd import HQError, OctreeType, ics
+from hierarquicos_amd.device import morton_keys
+
+pytestmark = pytest.mark.gpu
+
+
+def _close_pairs(npairs, sep, seed=0):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, (npairs, 3))
+    q = np.concatenate([c, c + sep * rng.normal(size=(npairs, 3))])
+    return np.full(2 * npairs, 1.0 / (2 * npairs)), q
+
+
+def test_node_capacity_exceeded_is_a_status():
+    m, q = _close_pairs(500, 1e-6)          # about 17 extra levels per pair
+    x, y, z = (np.ascontiguousarray(q[:, k]) for k in range(3))
+    with pytest.raises(HQError) as e:
+        OctreeType().pre_init(m, 11).init(x, y, z)
+    assert e.value.code == 2 and "INSUFFICIENT NUMBER OF NODES" in str(e.value)
+    with pytest.raises(RuntimeError):
+        O.Tree(m, 11).init(x, y, z)
+    # the same pairs further apart fit, and match the oracle
+    m, q = _close_pairs(500, 1e-2)
+    x, y, z = (np.ascontiguousarray(q[:, k]) for k in range(3))
+    t = OctreeType().pre_init(m, 11).init(x, y, z)
+    F = t.forces_all(0.25, 1.0, 1e-6).T
+    Fo = O.Tree(m, 11).init(x, y, z).bh_forces(0.25, 1.0, 1e-6)
+    assert np.max(np.linalg.norm(F - Fo, axis=1) / np.linalg.norm(Fo, axis=1)) < 1e-10
+
+
+@pytest.mark.parametrize("ncluster,side", [(300, 5e-6), (3000, 2e-6)])
+def test_long_runs_of_shared_high_keys(ncluster, side):
+    """A cluster far smaller than a depth-21 cell: runs of equal high words
+    longer than the per-run fix-up handles; the build takes the full-key sort
+    and its topology, lost bodies and FP64 forces equal the oracle's."""
+    rng = np.random.default_rng(ncluster)
+    m0, q0, _ = ics.plummer(2000, seed=7)
+    qc = 0.3 + side * rng.random((ncluster, 3))
+    q = np.concatenate([q0, qc])
+    m = np.full(len(q), 1.0 / len(q))
+    x, y, z = (np.ascontiguousarray(q[:, k]) for k in range(3))
+    t = OctreeType().pre_init(m, 11).init(x, y, z)
+    e = t.export()
+    r = O.Tree(m, 11).init(x, y, z).export()
+    cg = np.lexsort((e["key_lo"], e["key_hi"], e["depth"]))
+    cr = np.lexsort((r["klo"], r["khi"], r["depth"]))
+    assert len(cg) == len(cr)
+    for a, b in (("depth", "depth"), ("key_hi", "khi"), ("key_lo", "klo"), ("type", "type"),
+                 ("particle", "particle"), ("mass", "mass")):
+        assert np.array_equal(np.asarray(e[a])[cg], np.asarray(r[b])[cr]), a
+    # twigs at depth 25 keep the reference's 0/0 centre of mass (NaN), here too
+    com = np.stack([r["qx"], r["qy"], r["qz"]], 1)[cr]
+    assert np.array_equal(e["com"][cg], com, equal_nan=True)
+    assert np.array_equal(e["quad"][cg], r["quad"][cr], equal_nan=True)
+    assert t.info()["lost"] > 0
+    F = t.forces_all(0.25, 1.0, 1e-6).T
+    Fo = O.Tree(m, 11).init(x, y, z).bh_forces(0.25, 1.0, 1e-6)
+    assert np.array_equal(np.isnan(F), np.isnan(Fo))
+    ok = np.all(np.isfinite(Fo), axis=1) & (np.linalg.norm(np.nan_to_num(Fo), axis=1) > 0)
+    rel = np.linalg.norm(F - Fo, axis=1)[ok] / np.linalg.norm(Fo, axis=1)[ok]
+    assert ok.sum() > 2000 and float(rel.max()) < 1e-10
+
+
+def test_morton_depth_errors():
+    m, q, _ = ics.plummer(1000, seed=2)
+    with pytest.raises(HQError) as e:
+        morton_keys(m, q, 22)
+    assert e.value.code == 6
+    q2 = np.concatenate([q, q[:10]])          # duplicates: no depth separates them
+    with pytest.raises(HQError) as e:
+        morton_keys(np.full(1010, 1e-3), q2, 0)
+    assert e.value.code == 6
