@@ -51,7 +51,8 @@ class PararealParams(C.Structure):
 EXPORTS = (
     "hq_compute_forces_direct", "hq_compute_forces", "hq_generate_tree", "hq_tree_create",
     "hq_tree_create_batch", "hq_tree_set_state", "hq_tree_get_state", "hq_tree_integrate",
-    "hq_tree_energy", "hq_parareal", "hq_errors_ma", "hq_tree_autotune", "hq_tree_fmm",
+    "hq_tree_energy", "hq_parareal", "hq_errors_ma", "hq_tree_autotune", "hq_tree_get_tune", "hq_tune_table_save",
+    "hq_tune_table_load", "hq_tune_table_clear", "hq_tree_fmm",
     "hq_tree_set_masses", "hq_tree_build", "hq_tree_forces", "hq_tree_forces_bunched",
     "hq_tree_destroy", "hq_tree_tune", "hq_tree_info", "hq_tree_export", "hq_tree_order",
     "hq_direct_forces_device", "hq_morton_keys", "hq_morton_keys_noalloc", "hq_morton_create", "hq_morton_build",
@@ -105,6 +106,10 @@ def lib():
     L.hq_tree_integrate.argtypes = [_vp, C.c_int, C.c_double, C.c_int64, C.c_double, C.c_double,
                                     C.c_double, C.c_int]
     L.hq_tree_energy.argtypes = [_vp, C.c_double, C.c_double, _d]
+    L.hq_tree_get_tune.argtypes = [_vp, C.c_int, _i64]
+    L.hq_tune_table_save.argtypes = [C.c_char_p]
+    L.hq_tune_table_load.argtypes = [C.c_char_p]
+    L.hq_tune_table_clear.argtypes = []
     L.hq_tree_autotune.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                    _i, _i, _i, _d]
     L.hq_tree_fmm.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp,

@@ -6,7 +6,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <chrono>
 #include <memory>
 
 #include "hq_tree.h"
@@ -204,6 +203,7 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
     } else {
         return fail(HQ_ERR_ARGS, "unknown tuning key %d", key);
     }
+    t->tune_user |= 1u << key;  // set by the caller: the tuning table leaves it alone
     return HQ_OK;
 }
 
@@ -256,39 +256,48 @@ int hq_tree_autotune(hq_tree* t, double theta2, double G, double eps2, int preci
     if (!t) return fail(HQ_ERR_ARGS, "null handle");
     if (!t->built) return fail(HQ_ERR_ARGS, "tree not built (call hq_tree_build)");
     if (precision != HQ_FP64 && precision != HQ_MIXED) return fail(HQ_ERR_ARGS, "bad precision");
-    if (reps < 1) reps = 1;
-    static const int kB[] = {32, 48, 64}, kS[] = {128, 256, 512}, kL[] = {0, 2, 4, 8};
-    const int b0 = t->group_size, s0 = t->super_size, l0 = t->leaf;
-    int bb = b0, bs = s0, bl = l0;
-    double best = 1e300;
-    int rc;
-    for (int B : kB)
-        for (int S : kS)
-            for (int Lf : kL) {
-                if (S < B) continue;
-                t->group_size = B;
-                t->super_size = S;
-                t->leaf = Lf;
-                double tmin = 1e300;
-                for (int r = 0; r <= reps; ++r) {  // r == 0 warms up
-                    const auto a = std::chrono::steady_clock::now();
-                    if ((rc = tree_forces_groups(t, theta2, G, eps2, precision, true))) {
-                        t->group_size = b0, t->super_size = s0, t->leaf = l0;
-                        return rc;
-                    }
-                    const double el =
-                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-                    if (r > 0) tmin = std::min(tmin, el);
-                }
-                if (tmin < best) best = tmin, bb = B, bs = S, bl = Lf;
-            }
-    t->group_size = bb;
-    t->super_size = bs;
-    t->leaf = bl;
-    if (bunch) *bunch = bb;
-    if (super_size) *super_size = bs;
-    if (leaf) *leaf = bl;
-    if (ms) *ms = best;
+    TuneSetting best;
+    double best_ms = 0.0;
+    const int rc = autotune(t, theta2, G, eps2, precision, reps < 1 ? 1 : reps, &best, &best_ms);
+    if (rc) return rc;
+    if (bunch) *bunch = best.bunch;
+    if (super_size) *super_size = best.super_size;
+    if (leaf) *leaf = best.leaf;
+    if (ms) *ms = best_ms;
+    return HQ_OK;
+}
+
+int hq_tree_get_tune(const hq_tree* t, int key, int64_t* value) {
+    if (!t || !value) return fail(HQ_ERR_ARGS, "null argument");
+    switch (key) {
+        case HQ_TUNE_BUNCH: *value = t->group_size; break;
+        case HQ_TUNE_SUPER: *value = t->super_size; break;
+        case HQ_TUNE_FUSED: *value = t->fused ? 1 : 0; break;
+        case HQ_TUNE_LEAF: *value = t->leaf; break;
+        case HQ_TUNE_FMM_LEAF: *value = t->fmm_leaf; break;
+        case HQ_TUNE_GRAPH: *value = t->graph ? 1 : 0; break;
+        case HQ_TUNE_SPLIT: *value = t->split; break;
+        case HQ_TUNE_FAR: *value = t->far; break;
+        case HQ_TUNE_FAR_INNER: *value = t->far_inner; break;
+        case HQ_TUNE_GRAPH_GRID: *value = t->graph_grid; break;
+        case HQ_TUNE_MAC: *value = t->mac; break;
+        default: return fail(HQ_ERR_ARGS, "unknown tuning key %d", key);
+    }
+    return HQ_OK;
+}
+
+int hq_tune_table_save(const char* path) {
+    if (!path) return fail(HQ_ERR_ARGS, "null path");
+    return tune_table_save(path);
+}
+
+int hq_tune_table_load(const char* path) {
+    if (!path) return fail(HQ_ERR_ARGS, "null path");
+    return tune_table_load(path);
+}
+
+int hq_tune_table_clear(void) {
+    tune_table_clear();
     return HQ_OK;
 }
 

@@ -88,6 +88,9 @@ struct hq_tree {
     int64_t graph_nbun = 0;  // grid of the walk in graph mode (upper bound of the bunches)
     int64_t graph_grid = 0;  // HQ_TUNE_GRAPH_GRID: that grid if > 0, else 2 x bunches + 64
     int mac = 1;             // HQ_TUNE_MAC of the bunched walks: 0 cell side, 1 side + offset
+    unsigned tune_user = 0;  // HQ_TUNE_* keys the caller set (bit per key): the table keeps off them
+    int table_tag = -1;      // angle and precision of the last setting taken from the tuning table
+    hq::DevBuf tune_buf, tune_tmp;  // autotuner: sample slots, direct forces, error; direct scratch
     hq::DevBuf gstate, gsave;
     // status block: device + host mirror
     hq::DevBuf status;
@@ -221,6 +224,18 @@ int read_status(hq_tree* t);
 void collect_build_times(hq_tree* t);
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
                      DevBuf& hist, DevBuf& part, hipStream_t s);
+// autotuner of the bunched walk and its table (hq_tune.hip)
+struct TuneSetting {
+    int bunch, super_size, leaf, split;
+};
+int autotune(hq_tree* t, double theta2, double G, double eps2, int precision, int reps,
+             TuneSetting* best, double* best_ms);
+// takes the tuning table's setting for the handle's workload class (knobs the
+// caller has not set); true when it changed the handle
+bool tune_from_table(hq_tree* t, double theta2, int precision);
+int tune_table_save(const char* path);
+int tune_table_load(const char* path);
+void tune_table_clear();
 // the same sort with a caller-provided count table of rs_hist_words(n) words
 int radix_sort_pairs_ws(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
                         uint32_t* hist, hipStream_t s);
@@ -228,7 +243,7 @@ size_t rs_hist_words(int64_t n);
 int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s);
 int direct_forces(int64_t n, const double* m, const double* x, const double* y, const double* z,
                   int64_t nt, const int64_t* targets, double G, double eps2, int precision,
-                  double* F, hipStream_t s);
+                  double* F, hipStream_t s, DevBuf* scratch = nullptr);
 // FP64 direct sum inside each system of the handle, input order, into t->fx/fy/fz
 int direct_forces_systems(hq_tree* t, double G, double eps2);
 // forces of the current positions t->x/y/z into t->fx/fy/fz; *sorted tells

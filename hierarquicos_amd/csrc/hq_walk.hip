@@ -1651,6 +1651,7 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
     if ((rc = ensure_forces(t))) return rc;
     hipStream_t s = t->stream;
     int ngroups;
+    if (bunched) tune_from_table(t, theta2, precision);
     if ((rc = tree_make_groups(t, bunched))) return rc;
     ngroups = (int)t->ngroups;
     const bool mixed = bunched && precision == HQ_MIXED;

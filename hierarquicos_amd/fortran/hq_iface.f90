@@ -190,6 +190,32 @@ MODULE hq_iface
             INTEGER(C_INT) :: st
         END FUNCTION
 
+        FUNCTION hq_tree_get_tune(h, key, value) BIND(C, name="hq_tree_get_tune") RESULT(st)
+            IMPORT :: C_INT, C_INT64_T, C_PTR
+            TYPE(C_PTR), VALUE :: h
+            INTEGER(C_INT), VALUE :: key
+            INTEGER(C_INT64_T), INTENT(OUT) :: value
+            INTEGER(C_INT) :: st
+        END FUNCTION
+
+        ! the tuning table; path: a NUL-terminated file name (TRIM(p)//C_NULL_CHAR)
+        FUNCTION hq_tune_table_save(path) BIND(C, name="hq_tune_table_save") RESULT(st)
+            IMPORT :: C_INT, C_CHAR
+            CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+            INTEGER(C_INT) :: st
+        END FUNCTION
+
+        FUNCTION hq_tune_table_load(path) BIND(C, name="hq_tune_table_load") RESULT(st)
+            IMPORT :: C_INT, C_CHAR
+            CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+            INTEGER(C_INT) :: st
+        END FUNCTION
+
+        FUNCTION hq_tune_table_clear() BIND(C, name="hq_tune_table_clear") RESULT(st)
+            IMPORT :: C_INT
+            INTEGER(C_INT) :: st
+        END FUNCTION
+
         ! ---- time integration around the force path ---------------------------
         FUNCTION hq_tree_set_state(h, x, y, z, px, py, pz, where) &
                 BIND(C, name="hq_tree_set_state") RESULT(st)

@@ -233,14 +233,27 @@ class OctreeType:
         return out
 
     def autotune(self, theta2, G=1.0, eps2=0.0, precision="mixed", reps=2):
-        """Times the bunched walk of the built tree over bunch / super-group /
-        leaf-bucket settings and keeps the fastest; returns it with its time."""
+        """Searches the bunched walk's bunch size, super-group size, leaf
+        bucket and evaluation tiling on the built tree (HIP-event timing, an
+        accuracy guard on a sample), keeps the fastest setting and stores it
+        in the process's tuning table; returns it with its time in ms."""
         prec = {"fp64": L.HQ_FP64, "mixed": L.HQ_MIXED}[precision]
         b, s_, lf = C.c_int(), C.c_int(), C.c_int()
         ms = C.c_double()
         L.check(L.lib().hq_tree_autotune(self._h, theta2, G, eps2, prec, reps, C.byref(b),
                                          C.byref(s_), C.byref(lf), C.byref(ms)))
-        return {"bunch": b.value, "super_size": s_.value, "leaf": lf.value, "ms": ms.value}
+        return {"bunch": b.value, "super_size": s_.value, "leaf": lf.value,
+                "split": self.get_tune("split"), "ms": ms.value}
+
+    _TUNE_KEYS = {"bunch": 1, "super_size": 2, "fused": 3, "leaf": 4, "fmm_leaf": 5, "graph": 6,
+                  "split": 7, "far": 8, "far_inner": 9, "graph_grid": 10, "mac": 11}
+
+    def get_tune(self, key):
+        """Current value of a tuning knob (the names of tune(); far and
+        far_inner in tenths of a bunch radius, as the library stores them)."""
+        v = C.c_int64()
+        L.check(L.lib().hq_tree_get_tune(self._h, self._TUNE_KEYS[key], C.byref(v)))
+        return v.value
 
     # -- introspection ----------------------------------------------------------
     def info(self):
@@ -283,3 +296,18 @@ class OctreeType:
             self.close()
         except Exception:
             pass
+
+
+def tune_table_save(path):
+    """Writes the process's tuning table (autotuned bunched-walk settings per
+    workload class) to a text file."""
+    L.check(L.lib().hq_tune_table_save(str(path).encode()))
+
+
+def tune_table_load(path):
+    """Merges a saved tuning table into the process's table."""
+    L.check(L.lib().hq_tune_table_load(str(path).encode()))
+
+
+def tune_table_clear():
+    L.check(L.lib().hq_tune_table_clear())

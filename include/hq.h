@@ -160,15 +160,31 @@ int hq_tree_tune(hq_tree *t, int key, int64_t value);
 #define HQ_FMM_FP64 1
 int hq_tree_fmm(hq_tree *t, int order, double theta2, double G, double eps2, int mode, double *F,
                 int layout, int where);
-/* Autotuner (no reference counterpart): on the built tree, times the bunched
- * walk at `precision` for bunch sizes {32, 48, 64}, super-group sizes
- * {128, 256, 512} and leaf buckets {0, 2, 4, 8} (best of `reps` runs each,
- * after one warm-up), keeps the fastest setting on the handle and returns it
- * with its time in milliseconds.  Results of later bunched walks depend on
- * the setting (within the walk's error budget), so a caller that needs
- * bit-reproducible forces fixes the knobs with hq_tree_tune instead. */
+/* Autotuner (no reference counterpart): on the built tree, searches the
+ * bunched walk's settings at `precision` by coordinate descent from the
+ * handle's current one -- bunch size {32, 48, 64}, super-group size {128, 256,
+ * 512}, leaf bucket {0, 2, 4, 8, 16, 32, 64} and evaluation tiling (fused or
+ * split, HQ_TUNE_SPLIT 0 / 2) -- timing each with HIP events on the handle's
+ * stream (best of `reps` runs after a warm-up).  A setting whose RMS relative
+ * force error on 512 sample bodies (against the FP64 direct sum) exceeds the
+ * starting setting's by more than 2% is not taken (single-system handles).
+ * Keeps the fastest setting on the handle, returns it with its time in
+ * milliseconds, and stores it in the process's tuning table under the
+ * workload class (log2 N, systems, multipole order, theta, precision, tree
+ * depth).  Handles of the same class then start from the table's setting at
+ * their first bunched walk, for every knob their caller has not set with
+ * hq_tree_tune.  Results of bunched walks depend on the setting (within the
+ * walk's error budget), so a caller that needs bit-reproducible forces fixes
+ * the knobs with hq_tree_tune. */
 int hq_tree_autotune(hq_tree *t, double theta2, double G, double eps2, int precision, int reps,
                      int *bunch, int *super_size, int *leaf, double *ms);
+/* The handle's current value of a tuning knob (HQ_TUNE_*). */
+int hq_tree_get_tune(const hq_tree *t, int key, int64_t *value);
+/* The tuning table as text, one setting per workload class; load merges a
+ * file into the table (its rows win), clear empties it. */
+int hq_tune_table_save(const char *path);
+int hq_tune_table_load(const char *path);
+int hq_tune_table_clear(void);
 
 /* ---- Time integration around the force path ----------------------------
  * The handle holds a phase-space state in HBM, input order: positions

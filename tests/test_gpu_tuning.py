@@ -62,15 +62,60 @@ def test_fused_and_two_kernel_agree_with_buckets():
     assert float(rel.max()) < 1e-5
 
 
-def test_autotune_picks_a_valid_setting():
+def test_autotune_picks_a_valid_setting(tmp_path):
+    """The autotuner's setting lies on its axes, keeps the error within 2% of
+    the starting setting's on its sample (and within budget here), goes into
+    the tuning table, and a second handle of the same workload takes it from
+    the table at its first bunched walk without tuning; a knob its caller set
+    stays as set.  The table survives a save / clear / load round trip."""
+    from hierarquicos_amd import tune_table_clear, tune_table_load, tune_table_save
+    tune_table_clear()
     m, x, y, z, t = _tree(100000, 24)
-    r = t.autotune(0.25, 1.0, 1e-6, "mixed", reps=1)
-    assert r["bunch"] in (32, 48, 64) and r["super_size"] in (128, 256, 512)
-    assert r["leaf"] in (0, 2, 4, 8) and r["ms"] > 0
     sample = np.arange(0, 100000, 97)
     Fd = O.direct_targets(m, x, y, z, sample, 1.0, 1e-6)
+    e_start = O.rel_errors(t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T[sample], Fd)[0]
+    r = t.autotune(0.25, 1.0, 1e-6, "mixed", reps=1)
+    assert r["bunch"] in (32, 48, 64) and r["super_size"] in (128, 256, 512)
+    assert r["leaf"] in (0, 2, 4, 8, 16, 32, 64) and r["split"] in (0, 1, 2) and r["ms"] > 0
+    for k in ("bunch", "super_size", "leaf", "split"):
+        assert t.get_tune(k) == r[k]
     F = t.forces_bunched(0.25, 1.0, 1e-6, "mixed").T
-    assert O.rel_errors(F[sample], Fd)[0] < 1e-4
+    e_tuned = O.rel_errors(F[sample], Fd)[0]
+    assert e_tuned < 1e-4 and e_tuned < 1.1 * e_start, (e_tuned, e_start)
+    # a second handle of the same workload: the table's setting, no tuning
+    t2 = OctreeType().pre_init(m, 12).init(x, y, z)
+    t2.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert [t2.get_tune(k) for k in ("bunch", "super_size", "leaf", "split")] == \
+        [r["bunch"], r["super_size"], r["leaf"], r["split"]]
+    # a knob the caller set is kept
+    t3 = OctreeType().pre_init(m, 12).tune(leaf=5).init(x, y, z)
+    t3.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert t3.get_tune("leaf") == 5 and t3.get_tune("super_size") == r["super_size"]
+    # another angle is another class: defaults
+    t4 = OctreeType().pre_init(m, 12).init(x, y, z)
+    t4.forces_bunched(0.16, 1.0, 1e-6, "mixed")
+    assert t4.get_tune("leaf") == 0 and t4.get_tune("super_size") == 256
+    # save, clear, load
+    path = tmp_path / "tune.txt"
+    tune_table_save(path)
+    tune_table_clear()
+    t5 = OctreeType().pre_init(m, 12).init(x, y, z)
+    t5.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert t5.get_tune("leaf") == 0 and t5.get_tune("super_size") == 256
+    tune_table_load(path)
+    t6 = OctreeType().pre_init(m, 12).init(x, y, z)
+    t6.forces_bunched(0.25, 1.0, 1e-6, "mixed")
+    assert [t6.get_tune(k) for k in ("bunch", "super_size", "leaf", "split")] == \
+        [r["bunch"], r["super_size"], r["leaf"], r["split"]]
+    tune_table_clear()
+
+
+def test_tune_table_rejects_bad_rows(tmp_path):
+    from hierarquicos_amd import HQError, tune_table_load
+    path = tmp_path / "bad.txt"
+    path.write_text("17 0 8 50 1 18 64 128 4\n")
+    with pytest.raises(HQError):
+        tune_table_load(path)
 
 
 @pytest.mark.parametrize("method,leaf", [(10, 0), (11, 4), (12, 2)])
