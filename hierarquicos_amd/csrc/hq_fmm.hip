@@ -39,6 +39,9 @@ __host__ __device__ constexpr int group_of(int) { return 256; }
 
 __host__ __device__ constexpr int ncoef(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int kpad(int p) { return (ncoef(p) + 15) / 16 * 16; }
+// the M2L contraction length: the moments padded to the MFMA's k-step of 4
+// (rows of the operator and the locals are padded to 16 instead)
+__host__ __device__ constexpr int kq(int p) { return (ncoef(p) + 3) / 4 * 4; }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -103,7 +106,7 @@ void fmm_state_free(FmmState* s) { delete s; }
 static int build_tables(FmmTables& tb, int p, double theta) {
     if (tb.p == p && tb.theta == theta) return HQ_OK;
     const std::vector<MI> k1 = graded(PMAX), k2 = graded(2 * PMAX);
-    const int K = ncoef(p), Kp = kpad(p), K2 = ncoef(2 * p);
+    const int K = ncoef(p), Kp = kpad(p), Kq = kq(p), K2 = ncoef(2 * p);
     std::vector<int> pos(13 * 13 * 13, -1);
     for (size_t i = 0; i < k2.size(); ++i) pos[(k2[i].x * 13 + k2[i].y) * 13 + k2[i].z] = (int)i;
     auto at = [&](int x, int y, int z) { return (x < 0 || y < 0 || z < 0) ? -1 : pos[(x * 13 + y) * 13 + z]; };
@@ -156,19 +159,19 @@ static int build_tables(FmmTables& tb, int p, double theta) {
     const int nd1 = 2 * R + 2, noff1 = nd1 * nd1 * nd1;
     const int noff = noff0 + 2 * noff1;
     std::vector<MI> ks2(k2.begin(), k2.begin() + K2);
-    std::vector<float> T((size_t)noff * Kp * Kp, 0.0f);
+    std::vector<float> T((size_t)noff * Kp * Kq, 0.0f);
     std::vector<double> a;
     auto fill = [&](int o, const double u[3], int cls) {
         // u is the source's offset from the target; the expansion needs r = -u
         const double Ro[3] = {-u[0], -u[1], -u[2]};
         taylor(Ro, ks2, rec, a);
-        float* t = T.data() + (size_t)o * Kp * Kp;
+        float* t = T.data() + (size_t)o * Kp * Kq;
         for (int n = 0; n < K; ++n)
             for (int m = 0; m < K; ++m) {
                 double c = mcoef[n * K + m] * a[midx[n * K + m]];
                 if (cls == 1) c = ldexp(c, k1[n].x + k1[n].y + k1[n].z + 1);
                 if (cls == 2) c = ldexp(c, k1[m].x + k1[m].y + k1[m].z);
-                t[n * Kp + m] = (float)c;
+                t[n * Kq + m] = (float)c;
             }
     };
     for (int o = 0; o < noff0; ++o) {
@@ -334,11 +337,11 @@ __global__ __launch_bounds__(64) void k_fmm_m2m(int f0, int f1, FView v, const d
 template <int P>
 __global__ __launch_bounds__(256) void k_fmm_qf(int64_t nf, const double* __restrict__ Qd,
                                                 float* __restrict__ Qf) {
-    constexpr int K = ncoef(P), Kp = kpad(P);
+    constexpr int K = ncoef(P), Kq = kq(P);
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= nf * Kp) return;
-    const int64_t f = i / Kp;
-    const int n = (int)(i % Kp);
+    if (i >= nf * Kq) return;
+    const int64_t f = i / Kq;
+    const int n = (int)(i % Kq);
     Qf[i] = n < K ? (float)Qd[f * K + n] : 0.0f;
 }
 
@@ -556,7 +559,7 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
                                                       const float* __restrict__ T,
                                                       const float* __restrict__ Qf,
                                                       float* __restrict__ Lf) {
-    constexpr int K = ncoef(P), Kp = kpad(P), RT = Kp / 16, KT = Kp / 4, NW = 8;
+    constexpr int K = ncoef(P), Kp = kpad(P), Kq = kq(P), RT = Kp / 16, KT = Kq / 4, NW = 8;
     constexpr int GT = group_of(P);
     __shared__ float acc[GT * K];
     // block (g, sp): target group g, offsets [o_lo, o_hi); with several
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
         if (b0 == b1) continue;
         if (nbin++ % NW != wv) continue;
         const int nt = (int)((b1 - b0 + 15) / 16);
-        const float* to = T + (int64_t)o * Kp * Kp;
+        const float* to = T + (int64_t)o * Kp * Kq;
         // the operator and the first tile's sources are fetched together; each
         // later tile's sources are fetched while the previous tile computes
         float bn[KT];
@@ -588,12 +591,12 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
             const int s = okn ? src[e] : 0;
             cn = okn ? col[e] : 0;
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+            for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kq + 4 * kt + lk] : 0.0f;
         }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kp + 4 * kt + lk];
+            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + 4 * kt + lk];
         for (int t = 0; t < nt; ++t) {
             float bv[KT];
 #pragma unroll
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
                 const int s = okn ? src[e] : 0;
                 cn = okn ? col[e] : 0;
 #pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kp + 4 * kt + lk] : 0.0f;
+                for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kq + 4 * kt + lk] : 0.0f;
             }
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
@@ -971,7 +974,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     v.cz = t->ncz.as<double>();
     v.sg = t->geom.as<double>() + G_SYS;
     // 2. upward
-    if ((rc = st.Qd.ensure(nf * K * 8)) || (rc = st.Qf.ensure(nf * Kp * 4)) ||
+    if ((rc = st.Qd.ensure(nf * K * 8)) || (rc = st.Qf.ensure(nf * kq(P) * 4)) ||
         (rc = st.Ld.ensure(nf * K * 8)) || (rc = st.Lf.ensure(nf * Kp * 4)) ||
         (rc = st.rmax.ensure(nf * 8)))
         return rc;
@@ -984,7 +987,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
             k_fmm_m2m<P><<<f1 - f0, 64, 0, s>>>(f0, f1, v, tb.m2m.as<double>(), st.Qd.as<double>(),
                                                 st.rmax.as<double>());
     }
-    k_fmm_qf<P><<<grid_for(nf * Kp, 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
+    k_fmm_qf<P><<<grid_for(nf * kq(P), 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
     HQ_CHECK_HIP(hipMemsetAsync(st.Ld.p, 0, nf * K * 8, s));
     // 3. dual walk: frontier ping-pong in fa / fb; lists appended
     WalkPar w{theta, tb.R, tb.nd, tb.nd1, tb.noff0, tb.noff1};

@@ -179,6 +179,9 @@ __device__ __forceinline__ void store_walk_records(int64_t k, int multipole, dou
                                                    int depth, int macq, const double* Q,
                                                    const double* O, float* __restrict__ rec,
                                                    WalkNode* __restrict__ wn) {
+    // a leaf enters every walk as its body (P2P): its moment record is never
+    // read, so only twigs write one
+    if (type != 1) {
     float r[REC];
 #pragma unroll
     for (int j = 0; j < REC; ++j) r[j] = 0.0f;
@@ -197,6 +200,7 @@ __device__ __forceinline__ void store_walk_records(int64_t k, int multipole, dou
     float4* o = reinterpret_cast<float4*>(rec + k * REC);
 #pragma unroll
     for (int j = 0; j < REC / 4; ++j) o[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+    }
     WalkNode w;
     w.cx = cx;
     w.cy = cy;

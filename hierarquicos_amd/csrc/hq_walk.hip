@@ -511,17 +511,18 @@ __global__ __launch_bounds__(256) void k_mom32(int64_t nn, int multipole, NodeVi
     if (dnn) nn = *dnn;  // graph mode: the count is on the device
     if (k >= nn) return;
     const int64_t S = nv.stride;
+    const int type = (int)(uint8_t)nv.type[k];
     double Q[6] = {0, 0, 0, 0, 0, 0}, O[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (multipole >= 4)
+    if (multipole >= 4 && type != 1)
         for (int j = 0; j < 6; ++j) Q[j] = nv.quad[j * S + k];
-    if (multipole > 4)
+    if (multipole > 4 && type != 1)
         for (int j = 0; j < 14; ++j) O[j] = nv.oct[j * S + k];
     const int cs = nv.cstart[k];
     const int depth = (int)nv.depth[k];
     const double side = ldexp(g[SG * nv.sys[nv.first[k]] + S_SIDE], -depth);
     const int macq = mac_offset_q(nv.cx[k], nv.cy[k], nv.cz[k], nv.ncx[k], nv.ncy[k], nv.ncz[k], side);
     store_walk_records(k, multipole, nv.mass[k], nv.cx[k], nv.cy[k], nv.cz[k], nv.first[k],
-                       nv.count[k], cs, nv.cstart[k + 1] - cs, (int)(uint8_t)nv.type[k], depth,
+                       nv.count[k], cs, nv.cstart[k + 1] - cs, type, depth,
                        macq, Q, O, rec, wn);
 }
 
