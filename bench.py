@@ -486,6 +486,9 @@ def main():
         sync()
         ms_bun64 = (time.perf_counter() - a) * 1e3
         rms_b64, mx_b64 = rel(dF64.numpy().reshape(N, 3)[sample])
+        ib64 = t64.info()
+        flops_b64 = (FLOPS_M2P[ib64["multipole"]] * ib64["exec_cells"]
+                     + FLOPS_P2P * ib64["exec_parts"])
         for rec in sweep:
             th2_ = rec["theta"] ** 2
             t64.init(dx, dy, dz)
@@ -502,11 +505,19 @@ def main():
         extras = {
             "fp64_bunched": {"ms_per_eval": ms_bun64, "particles_per_s": N / (ms_bun64 * 1e-3),
                              "rms_rel_err": rms_b64, "max_rel_err": mx_b64,
-                             "mixed_speedup": ms_bun64 / ms_step},
+                             "mixed_speedup": ms_bun64 / ms_step,
+                             "walk_ms": ib64["ms_walk"],
+                             "executed_tflops": flops_b64 / (ib64["ms_walk"] * 1e-3) / 1e12,
+                             "executed_frac_fp64_peak": flops_b64 / (ib64["ms_walk"] * 1e-3)
+                             / 1e12 / FP64_PEAK_TFLOPS},
             "fp64_reference_bh": {"ms_per_eval": ms_bh64, "particles_per_s": N / (ms_bh64 * 1e-3),
                                   "rms_rel_err": rms_bh64, "max_rel_err": mx_bh64,
                                   "n_p2p": i64["n_p2p"], "n_m2p": i64["n_m2p"],
-                                  "walk_ms": i64["ms_walk"]},
+                                  "walk_ms": i64["ms_walk"],
+                                  "tflops": flops_alg / (i64["ms_walk"] * 1e-3) / 1e12,
+                                  "frac_fp64_peak": flops_alg / (i64["ms_walk"] * 1e-3) / 1e12
+                                  / FP64_PEAK_TFLOPS,
+                                  "mixed_speedup": ms_bh64 / ms_step},
         }
 
     # every seed of the input: median step time and the error on the sample

@@ -72,6 +72,17 @@ static NodeView node_view(const hq_tree* t) {
 }
 
 // ------------------------------------------------------------------ BH walk
+// bh_forces_over_p (octree_default.f90:895-1078) for every target, one target
+// per lane, the 64 targets of a wavefront (neighbours in tree order) walking
+// together: the wavefront follows the union of its lanes' depth-first walks.
+// At each node every lane still on its own walk decides as the reference does
+// (skip an empty node or its own leaf; accept a leaf, or L^2 <= theta^2 d^2,
+// or depth 25; else open).  If any lane opens, the wavefront descends; a lane
+// that skipped or accepted the node waits until the wavefront reaches the
+// node's skip pointer, where its own walk resumes.  Each lane therefore meets
+// exactly the nodes of its own walk, in its order, and adds the same terms in
+// the same order as a walk of its own; the node records are read once per
+// wavefront (scalar loads) instead of once per lane.
 __global__ __launch_bounds__(128) void k_bh_walk(NodeView nv, const double* __restrict__ g,
                                                  const double* __restrict__ sx,
                                                  const double* __restrict__ sy,
@@ -82,107 +93,128 @@ __global__ __launch_bounds__(128) void k_bh_walk(NodeView nv, const double* __re
                                                  double* __restrict__ fy, double* __restrict__ fz,
                                                  int64_t* __restrict__ status) {
 #pragma clang fp contract(off)
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    constexpr int ACTIVE = -2, NEVER = -3;
+    const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = i0 < n;
+    const int64_t i = valid ? i0 : n - 1;
     const double pm = sm[i], px = sx[i], py = sy[i], pz = sz[i];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     long long np2p = 0, nm2p = 0;
     const int64_t S = nv.stride;
     const int sys = nv.sys[i];
-    const double* __restrict__ gl2 = g + SG * sys + S_L2;
-    int k = sys;
-    while (k != END) {
-        const double mass = nv.mass[k];
-        const bool leaf = nv.type[k] == 1;
-        if (mass == 0.0 || (leaf && nv.first[k] == (int)i)) {
-            k = nv.next[k];
-            continue;
-        }
-        const double dx = nv.cx[k] - px;
-        const double dy = nv.cy[k] - py;
-        const double dz = nv.cz[k] - pz;
-        const double dist2 = dx * dx + dy * dy + dz * dz;
-        const int depth = nv.depth[k];
-        if (leaf || gl2[depth] <= theta2 * dist2 || depth == MAX_DEPTH) {
-            if (leaf || multipole == 1) {
-                double rinv = 1.0 / sqrt(dist2 + eps2);
-                rinv = rinv * rinv * rinv;
-                const double f = G * pm * mass * rinv;
-                f0 = f0 + f * dx;
-                f1 = f1 + f * dy;
-                f2 = f2 + f * dz;
-                ++np2p;
-            } else {
-                const double invR = 1.0 / sqrt(dist2 + eps2);
-                const double invR2 = 1.0 / (dist2 + eps2);
-                const double invR3 = invR * invR2;
-                const double invR5 = invR3 * invR * invR;
-                const double invR7 = invR5 * invR * invR;
-                const double invR9 = invR7 * invR * invR;
-                const double Mx = dx * invR3 * mass;
-                const double My = dy * invR3 * mass;
-                const double Mz = dz * invR3 * mass;
-                const double mxi2 = nv.quad[0 * S + k], myi2 = nv.quad[1 * S + k],
-                             mzi2 = nv.quad[2 * S + k], mxyi = nv.quad[3 * S + k],
-                             mxzi = nv.quad[4 * S + k], myzi = nv.quad[5 * S + k];
-                const double dx2 = 15.0 * dx * dx * invR2 - 3.0;
-                const double dy2 = 15.0 * dy * dy * invR2 - 3.0;
-                const double dz2 = 15.0 * dz * dz * invR2 - 3.0;
-                const double dxyz = 15.0 * dx * dy * dz * invR2;
-                double Qx = (mxi2 * (dx2 - 6.0) + myi2 * dy2 + mzi2 * dz2) * 0.5 * dx;
-                double Qy = (myi2 * (dy2 - 6.0) + mzi2 * dz2 + mxi2 * dx2) * 0.5 * dy;
-                double Qz = (mzi2 * (dz2 - 6.0) + mxi2 * dx2 + myi2 * dy2) * 0.5 * dz;
-                Qx = Qx + mxyi * dy * dx2 + mxzi * dz * dx2 + myzi * dxyz;
-                Qy = Qy + myzi * dz * dy2 + mxyi * dx * dy2 + mxzi * dxyz;
-                Qz = Qz + mxzi * dx * dz2 + myzi * dy * dz2 + mxyi * dxyz;
-                if (multipole == 4) {
-                    f0 = f0 + G * pm * (Mx + invR5 * Qx);
-                    f1 = f1 + G * pm * (My + invR5 * Qy);
-                    f2 = f2 + G * pm * (Mz + invR5 * Qz);
+    // the systems of the wavefront's targets, one tree walk each (a batch's
+    // systems are contiguous in tree order, so a wavefront meets few)
+    uint64_t todo = __ballot(valid);
+    while (todo) {
+        const int s = __builtin_amdgcn_readlane(sys, __ffsll((long long)todo) - 1);
+        const bool mine = valid && sys == s;
+        todo &= ~__ballot(mine);
+        const double* __restrict__ gl2 = g + SG * s + S_L2;
+        int wait = mine ? ACTIVE : NEVER;
+        int k = s;
+        while (k != END) {
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (wait == k) wait = ACTIVE;
+            const bool act = wait == ACTIVE;
+            bool open = false;
+            const double mass = nv.mass[k];
+            const bool leaf = nv.type[k] == 1;
+            if (act && !(mass == 0.0 || (leaf && nv.first[k] == (int)i))) {
+                const double dx = nv.cx[k] - px;
+                const double dy = nv.cy[k] - py;
+                const double dz = nv.cz[k] - pz;
+                const double dist2 = dx * dx + dy * dy + dz * dz;
+                const int depth = nv.depth[k];
+                if (leaf || gl2[depth] <= theta2 * dist2 || depth == MAX_DEPTH) {
+                    if (leaf || multipole == 1) {
+                        double rinv = 1.0 / sqrt(dist2 + eps2);
+                        rinv = rinv * rinv * rinv;
+                        const double f = G * pm * mass * rinv;
+                        f0 = f0 + f * dx;
+                        f1 = f1 + f * dy;
+                        f2 = f2 + f * dz;
+                        ++np2p;
+                    } else {
+                        const double invR = 1.0 / sqrt(dist2 + eps2);
+                        const double invR2 = 1.0 / (dist2 + eps2);
+                        const double invR3 = invR * invR2;
+                        const double invR5 = invR3 * invR * invR;
+                        const double invR7 = invR5 * invR * invR;
+                        const double invR9 = invR7 * invR * invR;
+                        const double Mx = dx * invR3 * mass;
+                        const double My = dy * invR3 * mass;
+                        const double Mz = dz * invR3 * mass;
+                        const double mxi2 = nv.quad[0 * S + k], myi2 = nv.quad[1 * S + k],
+                                     mzi2 = nv.quad[2 * S + k], mxyi = nv.quad[3 * S + k],
+                                     mxzi = nv.quad[4 * S + k], myzi = nv.quad[5 * S + k];
+                        const double dx2 = 15.0 * dx * dx * invR2 - 3.0;
+                        const double dy2 = 15.0 * dy * dy * invR2 - 3.0;
+                        const double dz2 = 15.0 * dz * dz * invR2 - 3.0;
+                        const double dxyz = 15.0 * dx * dy * dz * invR2;
+                        double Qx = (mxi2 * (dx2 - 6.0) + myi2 * dy2 + mzi2 * dz2) * 0.5 * dx;
+                        double Qy = (myi2 * (dy2 - 6.0) + mzi2 * dz2 + mxi2 * dx2) * 0.5 * dy;
+                        double Qz = (mzi2 * (dz2 - 6.0) + mxi2 * dx2 + myi2 * dy2) * 0.5 * dz;
+                        Qx = Qx + mxyi * dy * dx2 + mxzi * dz * dx2 + myzi * dxyz;
+                        Qy = Qy + myzi * dz * dy2 + mxyi * dx * dy2 + mxzi * dxyz;
+                        Qz = Qz + mxzi * dx * dz2 + myzi * dy * dz2 + mxyi * dxyz;
+                        if (multipole == 4) {
+                            f0 = f0 + G * pm * (Mx + invR5 * Qx);
+                            f1 = f1 + G * pm * (My + invR5 * Qy);
+                            f2 = f2 + G * pm * (Mz + invR5 * Qz);
+                        } else {
+                            double O[14];
+        #pragma unroll
+                            for (int j = 0; j < 14; ++j) O[j] = nv.oct[j * S + k];
+                            double Ox = -(1.5 * invR5 * O[11]);
+                            double Oy = -(1.5 * invR5 * O[12]);
+                            double Oz = -(1.5 * invR5 * O[13]);
+                            double Oc = 7.5 * invR7 * (dx * O[11] + dy * O[12] + dz * O[13]);
+                            Ox = Ox + dx * Oc;
+                            Oy = Oy + dy * Oc;
+                            Oz = Oz + dz * Oc;
+                            const double ex2 = dx * dx, ey2 = dy * dy, ez2 = dz * dz;
+                            Ox = Ox + 7.5 * invR7 *
+                                          (ex2 * O[0] + ey2 * O[1] + ez2 * O[2] + 2.0 * dx * dy * O[3] +
+                                           2.0 * dx * dz * O[6] + 2.0 * dy * dz * O[9]);
+                            Oy = Oy + 7.5 * invR7 *
+                                          (ex2 * O[3] + ey2 * O[4] + ez2 * O[5] + 2.0 * dx * dy * O[1] +
+                                           2.0 * dx * dz * O[9] + 2.0 * dy * dz * O[7]);
+                            Oz = Oz + 7.5 * invR7 *
+                                          (ex2 * O[6] + ey2 * O[7] + ez2 * O[8] + 2.0 * dx * dy * O[9] +
+                                           2.0 * dx * dz * O[2] + 2.0 * dy * dz * O[5]);
+                            Oc = -(17.5 * invR9 *
+                                   (ex2 * dx * O[0] + ey2 * dy * O[4] + ez2 * dz * O[8] +
+                                    3.0 * ex2 * dy * O[3] + 3.0 * ex2 * dz * O[6] + 3.0 * ey2 * dx * O[1] +
+                                    3.0 * ey2 * dz * O[7] + 3.0 * ez2 * dx * O[2] + 3.0 * ez2 * dy * O[5] +
+                                    6.0 * dx * dy * dz * O[9]));
+                            Ox = Ox + dx * Oc;
+                            Oy = Oy + dy * Oc;
+                            Oz = Oz + dz * Oc;
+                            f0 = f0 + G * pm * (Mx + invR5 * Qx + Ox);
+                            f1 = f1 + G * pm * (My + invR5 * Qy + Oy);
+                            f2 = f2 + G * pm * (Mz + invR5 * Qz + Oz);
+                        }
+                        ++nm2p;
+                    }
                 } else {
-                    double O[14];
-#pragma unroll
-                    for (int j = 0; j < 14; ++j) O[j] = nv.oct[j * S + k];
-                    double Ox = -(1.5 * invR5 * O[11]);
-                    double Oy = -(1.5 * invR5 * O[12]);
-                    double Oz = -(1.5 * invR5 * O[13]);
-                    double Oc = 7.5 * invR7 * (dx * O[11] + dy * O[12] + dz * O[13]);
-                    Ox = Ox + dx * Oc;
-                    Oy = Oy + dy * Oc;
-                    Oz = Oz + dz * Oc;
-                    const double ex2 = dx * dx, ey2 = dy * dy, ez2 = dz * dz;
-                    Ox = Ox + 7.5 * invR7 *
-                                  (ex2 * O[0] + ey2 * O[1] + ez2 * O[2] + 2.0 * dx * dy * O[3] +
-                                   2.0 * dx * dz * O[6] + 2.0 * dy * dz * O[9]);
-                    Oy = Oy + 7.5 * invR7 *
-                                  (ex2 * O[3] + ey2 * O[4] + ez2 * O[5] + 2.0 * dx * dy * O[1] +
-                                   2.0 * dx * dz * O[9] + 2.0 * dy * dz * O[7]);
-                    Oz = Oz + 7.5 * invR7 *
-                                  (ex2 * O[6] + ey2 * O[7] + ez2 * O[8] + 2.0 * dx * dy * O[9] +
-                                   2.0 * dx * dz * O[2] + 2.0 * dy * dz * O[5]);
-                    Oc = -(17.5 * invR9 *
-                           (ex2 * dx * O[0] + ey2 * dy * O[4] + ez2 * dz * O[8] +
-                            3.0 * ex2 * dy * O[3] + 3.0 * ex2 * dz * O[6] + 3.0 * ey2 * dx * O[1] +
-                            3.0 * ey2 * dz * O[7] + 3.0 * ez2 * dx * O[2] + 3.0 * ez2 * dy * O[5] +
-                            6.0 * dx * dy * dz * O[9]));
-                    Ox = Ox + dx * Oc;
-                    Oy = Oy + dy * Oc;
-                    Oz = Oz + dz * Oc;
-                    f0 = f0 + G * pm * (Mx + invR5 * Qx + Ox);
-                    f1 = f1 + G * pm * (My + invR5 * Qy + Oy);
-                    f2 = f2 + G * pm * (Mz + invR5 * Qz + Oz);
+                    open = true;
                 }
-                ++nm2p;
             }
-            k = nv.next[k];
-        } else {
-            k = nv.cstart[k];
+            const int nxt = nv.next[k];
+            if (__ballot(open)) {
+                if (act && !open) wait = nxt;  // its own walk goes on after this subtree
+                k = nv.cstart[k];
+            } else {
+                k = nxt;
+            }
         }
     }
-    fx[i] = f0;
-    fy[i] = f1;
-    fz[i] = f2;
-    long long a = wave_sum(np2p), b = wave_sum(nm2p);
+    if (valid) {
+        fx[i] = f0;
+        fy[i] = f1;
+        fz[i] = f2;
+    }
+    long long a = wave_sum(valid ? np2p : 0ll), b = wave_sum(valid ? nm2p : 0ll);
     if (__lane_id() == 0) {
         atomicAdd((unsigned long long*)&status[ST_P2P], (unsigned long long)a);
         atomicAdd((unsigned long long*)&status[ST_M2P], (unsigned long long)b);
