@@ -194,8 +194,7 @@ __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
                                               int* __restrict__ idx,
                                               uint64_t* __restrict__ khio,
                                               const double* __restrict__ m,
-                                              double4* __restrict__ rec,
-                                              uint2* __restrict__ aux) {
+                                              double4* __restrict__ rec) {
 #pragma clang fp contract(off)
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -226,7 +225,6 @@ __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x,
     // one packed record per particle, so that the gather after the sort reads
     // one line per particle instead of one per array
     rec[p] = make_double4(px, py, pz, m[p]);
-    aux[p] = make_uint2(lo, (uint32_t)sp);
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -372,26 +370,70 @@ __global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __re
     }
 }
 
+// Sorted bodies from the packed records.  The system of sorted slot i is the
+// one whose input range holds i (the sort leaves systems in order, each with
+// as many slots as bodies), and the low key word (levels 22..25) is the
+// comparisons of k_keys below the level-21 cube, whose centre follows from
+// the high word's digits by the same c +- h/2 steps: both are recomputed here
+// rather than gathered.
 __global__ __launch_bounds__(256) void k_gather(const int* __restrict__ idx,
                                                 const double4* __restrict__ rec,
-                                                const uint2* __restrict__ aux, int64_t n,
-                                                double* __restrict__ sx, double* __restrict__ sy,
-                                                double* __restrict__ sz, double* __restrict__ sm,
+                                                const uint64_t* __restrict__ khi,
+                                                const double* __restrict__ sg,
+                                                const int64_t* __restrict__ sysoff, int nsys,
+                                                int64_t n, double* __restrict__ sx,
+                                                double* __restrict__ sy, double* __restrict__ sz,
+                                                double* __restrict__ sm,
                                                 uint32_t* __restrict__ sklo,
                                                 int* __restrict__ ssys,
                                                 double4* __restrict__ sbody) {
+#pragma clang fp contract(off)
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = idx[i];
     const double4 r = rec[p];
-    const uint2 a = aux[p];
+    int sys = 0;
+    if (nsys > 1) {  // last system with sysoff[sys] <= i
+        int lo = 0, hi = nsys - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sysoff[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        sys = lo;
+    }
+    const double* g = sg + SG * sys;
+    double cx = g[S_CX], cy = g[S_CY], cz = g[S_CZ], h = g[S_HALF];
+    const uint64_t hk = khi[i];
+#pragma unroll
+    for (int d = 1; d <= HI_LEVELS; ++d) {
+        const uint32_t dig = (uint32_t)(hk >> (3 * (HI_LEVELS - d))) & 7u;
+        const double hh = h / 2.0;
+        cx = cx + ((dig & 4u) ? hh : -hh);
+        cy = cy + ((dig & 2u) ? hh : -hh);
+        cz = cz + ((dig & 1u) ? hh : -hh);
+        h = hh;
+    }
+    uint32_t lo = 0;
+#pragma unroll
+    for (int d = HI_LEVELS + 1; d <= MAX_DEPTH; ++d) {
+        const int xb = r.x >= cx;
+        const int yb = !(r.y < cy);
+        const int zb = !(r.z < cz);
+        const double hh = h / 2.0;
+        cx = cx + (xb ? hh : -hh);
+        cy = cy + (yb ? hh : -hh);
+        cz = cz + (zb ? hh : -hh);
+        h = hh;
+        lo = (lo << 3) | (uint32_t)(4 * xb + 2 * yb + zb);
+    }
     sbody[i] = r;
     sx[i] = r.x;
     sy[i] = r.y;
     sz[i] = r.z;
     sm[i] = r.w;
-    sklo[i] = a.x;
-    ssys[i] = (int)a.y;
+    sklo[i] = lo;
+    ssys[i] = sys;
 }
 
 // A batch is sorted by octant key, then stably by system: the second sort's
@@ -922,10 +964,11 @@ static int gather_and_emit(hq_tree* t, int64_t cap, Mark&& mark) {
     const double* sg = t->geom.as<double>() + G_SYS;
     int rc;
     k_gather<<<grid_for(n, 256), 256, 0, s>>>(t->idx.as<int>(), t->prec.as<double4>(),
-                                              t->paux.as<uint2>(), n, t->sx.as<double>(),
-                                              t->sy.as<double>(), t->sz.as<double>(),
-                                              t->sm.as<double>(), t->sklo.as<uint32_t>(),
-                                              t->ssys.as<int>(), t->sbody.as<double4>());
+                                              t->khi.as<uint64_t>(), sg, t->sysoff.as<int64_t>(),
+                                              t->nsys, n, t->sx.as<double>(), t->sy.as<double>(),
+                                              t->sz.as<double>(), t->sm.as<double>(),
+                                              t->sklo.as<uint32_t>(), t->ssys.as<int>(),
+                                              t->sbody.as<double4>());
     HQ_CHECK_HIP(mark(2));
     // 4. nodes
     k_lcp<<<grid_for(n, 256), 256, 0, s>>>(t->khi.as<uint64_t>(), t->sklo.as<uint32_t>(),
@@ -990,8 +1033,7 @@ static int full_key_sort(hq_tree* t) {
                                             t->z.as<double>(), n, sg, sysid,
                                             t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
                                             t->idx.as<int>(), t->khio.as<uint64_t>(),
-                                            t->m.as<double>(), t->prec.as<double4>(),
-                                            t->paux.as<uint2>());
+                                            t->m.as<double>(), t->prec.as<double4>());
     k_lo_keys<<<grid_for(n, 256), 256, 0, s>>>(t->klo.as<uint32_t>(), n, t->khi.as<uint64_t>(),
                                                t->idx.as<int>());
     if ((rc = radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
@@ -1017,7 +1059,7 @@ int tree_build(hq_tree* t) {
     size_t n8 = n * sizeof(double);
     if ((rc = t->geom.ensure((G_SYS + (size_t)SG * nsys) * sizeof(double)))) return rc;
     if ((rc = t->khi.ensure(n * 8)) || (rc = t->klo.ensure(n * 4)) || (rc = t->idx.ensure(n * 4)) ||
-        (rc = t->prec.ensure(n * 32)) || (rc = t->paux.ensure(n * 8)) ||
+        (rc = t->prec.ensure(n * 32)) ||
         (rc = t->sbody.ensure(n * 32)))
         return rc;
     if ((rc = t->sx.ensure(n8)) || (rc = t->sy.ensure(n8)) || (rc = t->sz.ensure(n8)) ||
@@ -1077,8 +1119,7 @@ int tree_build(hq_tree* t) {
                                             t->khi.as<uint64_t>(), t->klo.as<uint32_t>(),
                                             t->idx.as<int>(),
                                             batch ? t->khio.as<uint64_t>() : nullptr,
-                                            t->m.as<double>(), t->prec.as<double4>(),
-                                            t->paux.as<uint2>());
+                                            t->m.as<double>(), t->prec.as<double4>());
     HQ_CHECK_HIP(mark(1));
     // 3. sort: by octant key; a batch then stably by system
     if ((rc = radix_sort(t))) return rc;

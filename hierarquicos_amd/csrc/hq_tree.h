@@ -37,7 +37,7 @@ struct hq_tree {
     hq::DevBuf m, x, y, z;
     // keys and sort
     hq::DevBuf khi, klo, idx, khi2, idx2, hist, geom, l2tab, scanpart;
-    hq::DevBuf prec, paux;  // packed (x, y, z, m) and (key_lo, system) per input particle
+    hq::DevBuf prec;  // packed (x, y, z, m) per input particle
     // sorted particles (tree order)
     hq::DevBuf sx, sy, sz, sm, sklo, lcp;
     hq::DevBuf sbody;  // sorted bodies as packed (x, y, z, m) records
