@@ -560,7 +560,7 @@ def main():
         if (rec.get("workload") == workload and rec.get("leaf", 0) == args.leaf
                 and rec.get("far", 0.0) == args.far
                 and rec.get("far_inner", 0.0) == args.far_inner
-                and rec.get("super", 256) == args.super):
+                and rec.get("super", 256) == args.super and rec.get("mac", "side") == args.mac):
             traffic = rec["hbm_bytes_per_launch"]
     roof = None
     if flops_alg is not None:
