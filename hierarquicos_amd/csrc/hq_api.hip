@@ -119,7 +119,8 @@ int hq_tree_create(hq_tree** out, int64_t n, int method, int group_size, int pre
     for (auto& e : t->ev) HQ_CHECK_HIP(hipEventCreate(&e));
     int rc;
     if ((rc = t->status.ensure(ST_SIZE * sizeof(int64_t)))) return rc;
-    HQ_CHECK_HIP(hipHostMalloc((void**)&t->hstatus, ST_SIZE * sizeof(int64_t), hipHostMallocDefault));
+    // the status block, then host slots of other read-backs (ST_HOST_BUNCHES)
+    HQ_CHECK_HIP(hipHostMalloc((void**)&t->hstatus, ST_HOST_SIZE * sizeof(int64_t), hipHostMallocDefault));
     // one system [0, n): a batch handle overwrites these
     t->hsysoff = {0, n};
     if ((rc = t->sysoff.ensure(2 * sizeof(int64_t)))) return rc;

@@ -272,23 +272,31 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* __restrict__ h
 // [w*64*ITEMS, (w+1)*64*ITEMS) of the tile and walks it 64 keys at a time;
 // keys of equal digit inside one 64-key step are ranked by ballot (lanes in
 // index order), across steps by per-wave running counters, across waves by a
-// prefix over the per-wave totals.
+// prefix over the per-wave totals.  The tile is then placed in LDS in sorted
+// order and written out in that order, so that consecutive lanes write
+// consecutive addresses of one digit's run (1.2x faster than each key written
+// from its own lane).
 template <int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t* __restrict__ kin, const int* __restrict__ vin, uint64_t* __restrict__ kout,
     int* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ hist, int nblk,
     const uint32_t* __restrict__ dtot) {
+    constexpr int TILE = RS_THREADS * ITEMS;
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     __shared__ uint32_t goff[256];
+    __shared__ uint32_t lstart[256];
     __shared__ uint32_t ws[RS_THREADS / 64];
+    __shared__ uint64_t skey[TILE];
+    __shared__ int sval[TILE];
     const int w = threadIdx.x >> 6;
     const int lane = __lane_id();
     for (int d = threadIdx.x; d < 256 * (RS_THREADS / 64); d += RS_THREADS) (&wcnt[0][0])[d] = 0;
     uint32_t all;
     const uint32_t below = block_excl_scan(dtot[threadIdx.x], ws, all);  // keys of smaller digits
-    goff[threadIdx.x] = below + hist[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    const uint32_t gd = below + hist[(int64_t)threadIdx.x * nblk + blockIdx.x];
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * (RS_THREADS * ITEMS) + (int64_t)w * (ITEMS * 64);
+    const int64_t t0 = (int64_t)blockIdx.x * TILE;
+    const int64_t base = t0 + (int64_t)w * (ITEMS * 64);
     uint64_t key[ITEMS];
     int val[ITEMS];
     uint32_t rank[ITEMS];
@@ -315,7 +323,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     }
     __syncthreads();
     {
-        int d = threadIdx.x;  // one digit per thread
+        const int d = threadIdx.x;  // one digit per thread
         uint32_t run = 0;
 #pragma unroll
         for (int k = 0; k < RS_THREADS / 64; ++k) {
@@ -323,6 +331,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
             wcnt[k][d] = run;
             run += c;
         }
+        uint32_t tt;
+        const uint32_t ls = block_excl_scan(run, ws, tt);  // the digit's start in the tile
+        lstart[d] = ls;
+        goff[d] = gd - ls;  // output position of tile position ls is gd
     }
     __syncthreads();
 #pragma unroll
@@ -330,9 +342,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         int64_t i = base + it * 64 + lane;
         if (i < n) {
             uint32_t dg = (uint32_t)((key[it] >> shift) & 255u);
-            int64_t pos = (int64_t)goff[dg] + wcnt[w][dg] + rank[it];
-            kout[pos] = key[it];
-            vout[pos] = val[it];
+            uint32_t lp = lstart[dg] + wcnt[w][dg] + rank[it];
+            skey[lp] = key[it];
+            sval[lp] = val[it];
+        }
+    }
+    __syncthreads();
+    const int cnt = (int)min<int64_t>(TILE, n - t0);
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const int i = it * RS_THREADS + threadIdx.x;
+        if (i < cnt) {
+            const uint64_t k = skey[i];
+            // goff = gd - ls wraps modulo 2^32 when gd < ls; the sum is the
+            // position (< n < 2^32) again
+            const int64_t pos = (int64_t)(uint32_t)(goff[(k >> shift) & 255u] + (uint32_t)i);
+            kout[pos] = k;
+            vout[pos] = sval[i];
         }
     }
 }
@@ -612,7 +638,7 @@ struct UpArgs {
     const int8_t* ntype;
     const int* nfirst;
     const int* ncstart;
-    const uint8_t* ndigit;
+    const uint8_t* nmask;
     const double *sx, *sy, *sz, *sm;
     double *nmass, *ncomx, *ncomy, *ncomz, *nquad, *noct;
     int* ncount;
@@ -626,7 +652,7 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
     const int8_t* __restrict__ ntype = u.ntype;
     const int* __restrict__ nfirst = u.nfirst;
     const int* __restrict__ ncstart = u.ncstart;
-    const uint8_t* __restrict__ ndigit = u.ndigit;
+    const uint8_t* __restrict__ nmask = u.nmask;
     const double* __restrict__ sx = u.sx;
     const double* __restrict__ sy = u.sy;
     const double* __restrict__ sz = u.sz;
@@ -660,17 +686,18 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
         }
     } else {
         // children in the reference's order r = 0..7 (octant index
-        // 1 + x + 2(1-y) + 4(1-z), :527-531, of digit 4x + 2y + z): the present digits as a
-        // mask; the child of digit d is the (number of present digits below
-        // d)-th of the contiguous range.  Every child's record is loaded up
-        // front (absent slots read the first child, unused), so the loads are
-        // in flight together; the sums below keep the reference's order.
-        const int cs = ncstart[k], ce = ncstart[k + 1];
-        unsigned present = 0u;
-        for (int c = cs; c < ce; ++c) present |= 1u << ndigit[c];
+        // 1 + x + 2(1-y) + 4(1-z), :527-531, of digit 4x + 2y + z): the present
+        // digits as a mask (written with the parent pointers); the child of
+        // digit d is the (number of present digits below d)-th of the
+        // contiguous range.  Every child's record is loaded up front (absent
+        // slots read the first child, unused), so the loads are in flight
+        // together; the sums below keep the reference's order.
+        const int cs = ncstart[k];
+        const unsigned present = nmask[k];
         int cidx[8];
         double mcv[8], cxv[8], cyv[8], czv[8];
         int cnv[8];
+        bool twv[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const unsigned d = 4u * (unsigned)(r & 1) + 2u * (1u - ((r >> 1) & 1u)) +
@@ -682,6 +709,7 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
             cyv[r] = ncomy[c];
             czv[r] = ncomz[c];
             cnv[r] = ncount[c];
+            twv[r] = ntype[c] == 2;
         }
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -706,13 +734,15 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
                 double dxi = cxv[r] - qx;
                 double dyi = cyv[r] - qy;
                 double dzi = czv[r] - qz;
-                bool twig = ntype[c] == 2;
-                double Qc[6];
-                for (int j = 0; j < 6; ++j) Qc[j] = nquad[(int64_t)j * nn + c];
+                // a leaf child's moments are zero: only twigs' are read.  O[10..13]
+                // are recomputed from O[0..9] below, so children's are not summed
+                const bool twig = twv[r];
+                double Qc[6] = {0, 0, 0, 0, 0, 0};
                 if (twig) {
+                    for (int j = 0; j < 6; ++j) Qc[j] = nquad[(int64_t)j * nn + c];
                     for (int j = 0; j < 6; ++j) Q[j] = Q[j] + Qc[j];
                     if (oct)
-                        for (int j = 0; j < 14; ++j) O[j] = O[j] + noct[(int64_t)j * nn + c];
+                        for (int j = 0; j < 10; ++j) O[j] = O[j] + noct[(int64_t)j * nn + c];
                 }
                 Q[0] = Q[0] + pm * (dxi * dxi);
                 Q[1] = Q[1] + pm * (dyi * dyi);
@@ -801,16 +831,34 @@ __global__ void k_graph_build_check(const int64_t* __restrict__ st, int64_t cap,
     for (int d = 0; d <= NDEP; ++d) gs[GS_LEVEL0 + d] = ok ? min(st[ST_LEVEL0 + d], cap) : 0;
 }
 
-// Parent pointers of every node; roots (the first nroots nodes) get -1.
+// Launched build, after node emission: the node count of the pointer kernels
+// queued while the host waits for the status (0 when the tree is unusable:
+// non-finite input, capacity, a long tie run, a missing root).
+__global__ void k_safe_nn(int64_t* __restrict__ st, int64_t cap, int nsys) {
+    if (threadIdx.x != 0) return;
+    const int64_t fl = st[ST_FLAGS], nn = st[ST_NNODES];
+    const bool ok = (fl & (1 | 2 | 4)) == 0 && nn <= cap && st[ST_LEVEL0 + 1] == nsys;
+    st[ST_SAFENN] = ok ? nn : 0;
+}
+
+// Parent pointers of every node (roots, the first nroots nodes, get -1) and
+// the mask of the octant digits of each node's children.
 __global__ __launch_bounds__(256) void k_parents(int64_t nn, int64_t nroots,
                                                  const int* __restrict__ ncstart,
+                                                 const uint8_t* __restrict__ ndigit,
                                                  int* __restrict__ nparent,
+                                                 uint8_t* __restrict__ nmask,
                                                  const int64_t* __restrict__ dnn) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (dnn) nn = *dnn;  // graph mode: the count is on the device
+    if (dnn) nn = *dnn;  // the count is on the device (0 when the tree is unusable)
     if (k >= nn) return;
     if (k < nroots) nparent[k] = -1;
-    for (int c = ncstart[k]; c < ncstart[k + 1]; ++c) nparent[c] = (int)k;
+    unsigned m = 0u;
+    for (int c = ncstart[k]; c < ncstart[k + 1]; ++c) {
+        nparent[c] = (int)k;
+        m |= 1u << ndigit[c];
+    }
+    nmask[k] = (uint8_t)m;
 }
 
 // Skip pointer of every node: its next sibling, else that of the nearest
@@ -839,11 +887,22 @@ __global__ __launch_bounds__(256) void k_next(int64_t nn, const int* __restrict_
 // ------------------------------------------------------------------ driver
 static UpArgs up_args(const hq_tree* t) {
     return UpArgs{t->multipole, t->ntype.as<int8_t>(), t->nfirst.as<int>(), t->ncstart.as<int>(),
-                  t->ndigit.as<uint8_t>(), t->sx.as<double>(), t->sy.as<double>(),
+                  t->nmask.as<uint8_t>(), t->sx.as<double>(), t->sy.as<double>(),
                   t->sz.as<double>(), t->sm.as<double>(), t->nmass.as<double>(),
                   t->ncomx.as<double>(), t->ncomy.as<double>(), t->ncomz.as<double>(),
                   t->multipole >= 4 ? t->nquad.as<double>() : nullptr,
                   t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), t->cap};
+}
+
+// Parent, child-mask and skip pointers of the nodes; with dnn the count is
+// read on the device (grids over the capacity).
+static void launch_pointers(hq_tree* t, int64_t nn, int nsys, const int64_t* dnn) {
+    hipStream_t s = t->stream;
+    k_parents<<<grid_for(nn, 256), 256, 0, s>>>(nn, nsys, t->ncstart.as<int>(),
+                                                t->ndigit.as<uint8_t>(), t->nparent.as<int>(),
+                                                t->nmask.as<uint8_t>(), dnn);
+    k_next<<<grid_for(nn, 256), 256, 0, s>>>(nn, t->ncstart.as<int>(), t->nparent.as<int>(),
+                                             t->nnext.as<int>(), dnn);
 }
 
 // Graph mode, after node emission: the checks on the device, then pointers and
@@ -853,10 +912,7 @@ static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
     hipStream_t s = t->stream;
     int64_t* gs = t->gstate.as<int64_t>();
     k_graph_build_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), cap, nsys, gs);
-    k_parents<<<grid_for(cap, 256), 256, 0, s>>>(cap, nsys, t->ncstart.as<int>(),
-                                                 t->nparent.as<int>(), gs + GS_NN);
-    k_next<<<grid_for(cap, 256), 256, 0, s>>>(cap, t->ncstart.as<int>(), t->nparent.as<int>(),
-                                              t->nnext.as<int>(), gs + GS_NN);
+    launch_pointers(t, cap, nsys, gs + GS_NN);
     // grids from the level sizes of the last launched build, with a margin
     // (the kernel strides over whatever the level holds at replay)
     const UpArgs ua = up_args(t);
@@ -1082,6 +1138,7 @@ int tree_build(hq_tree* t) {
         (rc = t->nfirst.ensure(cap * 4)) || (rc = t->ncount.ensure(cap * 4)) ||
         (rc = t->ncstart.ensure((cap + 1) * 4)) || (rc = t->nnext.ensure(cap * 4)) ||
         (rc = t->nparent.ensure(cap * 4)) || (rc = t->ndigit.ensure(cap)) ||
+        (rc = t->nmask.ensure(cap)) ||
         (rc = t->nmass.ensure(cap * 8)) || (rc = t->ncomx.ensure(cap * 8)) ||
         (rc = t->ncomy.ensure(cap * 8)) || (rc = t->ncomz.ensure(cap * 8)) ||
         (rc = t->ncx.ensure(cap * 8)) || (rc = t->ncy.ensure(cap * 8)) ||
@@ -1129,7 +1186,17 @@ int tree_build(hq_tree* t) {
                                                t->status.as<int64_t>());
     if ((rc = gather_and_emit(t, cap, mark))) return rc;
     if (gm) return build_graph_tail(t, cap, nsys);
-    if ((rc = read_status(t))) return rc;
+    // the status goes to the host; the pointer kernels (over the device-held
+    // node count) run while the host waits for it
+    k_safe_nn<<<1, 64, 0, s>>>(t->status.as<int64_t>(), cap, nsys);
+    HQ_CHECK_HIP(hipMemcpyAsync(t->hstatus, t->status.p, ST_SIZE * sizeof(int64_t),
+                                hipMemcpyDeviceToHost, s));
+    HQ_CHECK_HIP(hipEventRecord(t->ev[11], s));
+    launch_pointers(t, cap, nsys, t->status.as<int64_t>() + ST_SAFENN);
+    HQ_CHECK_HIP(hipEventSynchronize(t->ev[11]));
+    HQ_CHECK_HIP(hipGetLastError());
+    collect_build_times(t);
+    bool pointers = t->hstatus[ST_SAFENN] > 0;
     int64_t flags = t->hstatus[ST_FLAGS];
     if (flags & 1) return fail(HQ_ERR_NONFINITE, "non-finite input position");
     if (flags & 4) {
@@ -1141,6 +1208,7 @@ int tree_build(hq_tree* t) {
         if ((rc = gather_and_emit(t, cap, mark))) return rc;
         if ((rc = read_status(t))) return rc;
         flags = t->hstatus[ST_FLAGS];
+        pointers = false;
     }
     t->nn = t->hstatus[ST_NNODES];
     t->lost = t->hstatus[ST_LOST];
@@ -1154,12 +1222,9 @@ int tree_build(hq_tree* t) {
                     t->level_start[1], nsys);
     for (int d = 0; d < NDEP; ++d)
         if (t->level_start[d + 1] > t->level_start[d]) t->most_depth = d;
-    // 5. parent and skip pointers (the walks' threading; the roots end every walk)
-    const int64_t nn = t->nn;
-    k_parents<<<grid_for(nn, 256), 256, 0, s>>>(nn, nsys, t->ncstart.as<int>(),
-                                                t->nparent.as<int>(), nullptr);
-    k_next<<<grid_for(nn, 256), 256, 0, s>>>(nn, t->ncstart.as<int>(), t->nparent.as<int>(),
-                                             t->nnext.as<int>(), nullptr);
+    // 5. parent and skip pointers (the walks' threading; the roots end every
+    // walk), unless queued above
+    if (!pointers) launch_pointers(t, t->nn, nsys, nullptr);
     // 6. moments, deepest level first
     const UpArgs ua = up_args(t);
     for (int d = t->most_depth; d >= 0; --d) {

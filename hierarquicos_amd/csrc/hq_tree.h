@@ -30,7 +30,7 @@ struct hq_tree {
     std::vector<int64_t> hsysoff;
     hq::DevBuf sysoff, sysid, ssys, khio;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[12] = {};
+    hipEvent_t ev[14] = {};
     bool have_masses = false, built = false;
 
     // inputs, original order
@@ -43,7 +43,7 @@ struct hq_tree {
     hq::DevBuf sbody;  // sorted bodies as packed (x, y, z, m) records
     // node store (level order)
     int64_t nn = 0, cap = 0;
-    hq::DevBuf ndepth, ntype, nfirst, ncount, ncstart, nnext, nparent, ndigit;
+    hq::DevBuf ndepth, ntype, nfirst, ncount, ncstart, nnext, nparent, ndigit, nmask;
     hq::DevBuf nmass, ncomx, ncomy, ncomz, nquad, noct, ncx, ncy, ncz, nkhi, nklo;
     hq::DevBuf mom32;  // float moment records for the mixed-precision walk
     hq::DevBuf wnode;  // packed per-node records for the mixed walk's traversal
@@ -94,6 +94,7 @@ struct hq_tree {
     hq::DevBuf gstate, gsave;
     // status block: device + host mirror
     hq::DevBuf status;
+    bool groups_pending = false;  // a bunch count read-back is in flight (finish_groups)
     int64_t* hstatus = nullptr;
     // last timings / diagnostics
     float ms_keys = 0, ms_sort = 0, ms_nodes = 0, ms_upward = 0, ms_build = 0;
@@ -127,7 +128,12 @@ enum {
     ST_EXECP = 40,     // executed target-particle interactions
     ST_MAXSTK = 41,    // deepest walk stack of the last grouped walk
     ST_LISTF = 42,     // cells of the mixed walk taken through the far-field expansion
-    ST_SIZE = 48
+    ST_SAFENN = 43,    // node count for kernels queued before the host checks the build
+                       // (0 when the build is unusable)
+    ST_SIZE = 48,
+    // host-only slots after the status block (the status copy leaves them alone)
+    ST_HOST_BUNCHES = 48,  // bunch count, read back while k_mom32 runs
+    ST_HOST_SIZE = 56
 };
 
 // graph-mode slots of hq_tree::gstate (int64)

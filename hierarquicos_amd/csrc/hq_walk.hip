@@ -1530,6 +1530,7 @@ __global__ void k_graph_walk_check(const int64_t* __restrict__ st, int64_t* __re
 }
 
 int tree_make_groups(hq_tree* t, bool bunched);
+int finish_groups(hq_tree* t);
 
 // Split evaluation (record, per-batch partials, combine) is used for few
 // bunches, once a walk has measured the list lengths (split_lcap).
@@ -1686,8 +1687,17 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
     int ngroups;
     if (bunched) tune_from_table(t, theta2, precision);
     if ((rc = tree_make_groups(t, bunched))) return rc;
-    ngroups = (int)t->ngroups;
     const bool mixed = bunched && precision == HQ_MIXED;
+    if (mixed && !t->graph_mode) {
+        // the walk records need no bunch count: queued before the wait for it
+        if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
+        if ((rc = t->wnode.ensure((size_t)t->nn * sizeof(WalkNode)))) return rc;
+        k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
+                                                     t->geom.as<double>() + G_SYS, t->mom32.as<float>(),
+                                                     t->wnode.as<WalkNode>(), nullptr);
+    }
+    if ((rc = finish_groups(t))) return rc;
+    ngroups = (int)t->ngroups;
     if (t->graph_mode) {
         // captured step: node and bunch counts stay on the device, grids are
         // upper bounds (capacity, bunches of the launched step with margin)
@@ -1723,11 +1733,6 @@ int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int pre
         return HQ_OK;
     }
     if (mixed) {
-        if ((rc = t->mom32.ensure((size_t)t->nn * REC * sizeof(float)))) return rc;
-        if ((rc = t->wnode.ensure((size_t)t->nn * sizeof(WalkNode)))) return rc;
-        k_mom32<<<grid_for(t->nn, 256), 256, 0, s>>>(t->nn, t->multipole, node_view(t),
-                                                     t->geom.as<double>() + G_SYS, t->mom32.as<float>(),
-                                                     t->wnode.as<WalkNode>(), nullptr);
         if (t->fused && use_split(t, ngroups)) {
             HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
             HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
@@ -1962,11 +1967,12 @@ int tree_make_groups(hq_tree* t, bool bunched) {
             HQ_CHECK_HIP(hipGetLastError());
             return HQ_OK;
         }
-        uint32_t nbun = 0;
-        HQ_CHECK_HIP(hipMemcpyAsync(&nbun, cnt + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HQ_CHECK_HIP(hipStreamSynchronize(s));
-        HQ_CHECK_HIP(hipGetLastError());
-        t->ngroups = nbun;
+        // the count comes back through a pinned slot; the caller may queue
+        // work that does not need it (k_mom32) before finish_groups waits
+        uint32_t* slot = reinterpret_cast<uint32_t*>(t->hstatus + ST_HOST_BUNCHES);
+        HQ_CHECK_HIP(hipMemcpyAsync(slot, cnt + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HQ_CHECK_HIP(hipEventRecord(t->ev[12], s));
+        t->groups_pending = true;
         return HQ_OK;
     }
     uint32_t total = 0;
@@ -1974,6 +1980,16 @@ int tree_make_groups(hq_tree* t, bool bunched) {
                                 hipMemcpyDeviceToHost, s));
     HQ_CHECK_HIP(hipStreamSynchronize(s));
     t->ngroups = total;
+    return HQ_OK;
+}
+
+// Waits for the bunch count that tree_make_groups left in flight.
+int finish_groups(hq_tree* t) {
+    if (!t->groups_pending) return HQ_OK;
+    t->groups_pending = false;
+    HQ_CHECK_HIP(hipEventSynchronize(t->ev[12]));
+    HQ_CHECK_HIP(hipGetLastError());
+    t->ngroups = *reinterpret_cast<const uint32_t*>(t->hstatus + ST_HOST_BUNCHES);
     return HQ_OK;
 }
 
