@@ -199,6 +199,51 @@ def test_multipole_order_convergence():
     assert errs[0] > 5 * errs[1] and errs[1] > 5 * errs[2] and errs[2] > 0
 
 
+def _node_errors(m, q, d, eps2):
+    """Relative error of one particle-node interaction (mono, quad, oct) for a
+    unit-mass target at distance d along the diagonal of a cluster at the
+    origin: the target shares a root with the cluster, which fills one of its
+    octants and is accepted whole (one interaction)."""
+    mm = np.concatenate([[1.0], m])
+    t = np.full(3, d / np.sqrt(3.0))
+    x, y, z = np.vstack([t, q]).T.copy()
+    Fd = O.direct_targets(mm, x, y, z, [0], 1.0, eps2)[0]
+    out = []
+    for method in (10, 11, 12):
+        F, c = O.Tree(mm, method).init(x, y, z).bh_forces(0.81, 1.0, eps2, targets=[0],
+                                                           counts=True)
+        assert c.sum() == 1
+        out.append(np.linalg.norm(F[0] - Fd) / np.linalg.norm(Fd))
+    return out
+
+
+@pytest.mark.parametrize("tag,eps", [("eps0", 0.0), ("eps015", 0.15)])
+def test_multipole_error_matches_published(golden, tag, eps):
+    # benchmarks/multipoles_error/out/multipoles_error.txt: 20 random 3000-body
+    # Plummer nodes (a = 0.58, truncated at 6, multipoles_error.f90:41-49).  At
+    # each distance, the median over 20 seeded nodes of our error must lie in
+    # the published 10-90% band and within a factor 2 of the published median,
+    # for every order; and the fall with distance (about d^-2, d^-3, d^-4) must
+    # match: a dropped or wrong term moves an order by decades.
+    pub = golden("published_errors.npz")["multipole_" + tag]  # (test, 4, row)
+    dgrid = np.array([400.0, 200.0, 100.0, 50.0])
+
+    def at(d, e):  # log-log interpolation of a descending sequence
+        ok = np.isfinite(d)
+        return np.exp(np.interp(np.log(dgrid), np.log(d[ok][::-1]), np.log(e[ok][::-1])))
+
+    P = np.array([[at(p[0], p[k]) for k in (1, 2, 3)] for p in pub])  # (20, 3, nd)
+    ours = np.array([[_node_errors(*ics.plummer(3000, a=0.58, rmax=6.0, seed=s)[:2], d, eps * eps)
+                      for d in dgrid] for s in range(20)]).transpose(0, 2, 1)  # (20, 3, nd)
+    med_p, med_o = np.median(P, axis=0), np.median(ours, axis=0)
+    lo, hi = np.percentile(P, 10, axis=0), np.percentile(P, 90, axis=0)
+    assert np.all((med_o >= lo) & (med_o <= hi)), (med_o, lo, hi)
+    assert np.all(np.abs(np.log(med_o / med_p)) < np.log(2.0)), med_o / med_p
+    slope = lambda a: np.log(a[:, 0] / a[:, -1]) / np.log(dgrid[-1] / dgrid[0])
+    assert np.allclose(slope(med_o), slope(med_p), atol=0.15), (slope(med_o), slope(med_p))
+    assert np.allclose(slope(med_p), [2.0, 3.0, 4.0], atol=0.3)
+
+
 def test_morton_key_interleave_brute_force():
     # cell centres ((i+1/2)/2^L) quantise to i; the key interleaves bits MSB
     # first with x the high bit of each octal digit (morton_octree.f90:205-212)
