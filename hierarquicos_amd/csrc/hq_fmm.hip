@@ -34,14 +34,19 @@ namespace hq {
 constexpr int PMAX = 6;
 constexpr int KMAX = 84;     // (p+1)(p+2)(p+3)/6 at p = 6
 constexpr int K2MAX = 455;   // same at 2p = 12
-// target cubes per M2L workgroup (their FP32 locals live in LDS)
-__host__ __device__ constexpr int group_of(int) { return 256; }
-
 __host__ __device__ constexpr int ncoef(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 __host__ __device__ constexpr int kpad(int p) { return (ncoef(p) + 15) / 16 * 16; }
 // the M2L contraction length: the moments padded to the MFMA's k-step of 4
 // (rows of the operator and the locals are padded to 16 instead)
 __host__ __device__ constexpr int kq(int p) { return (ncoef(p) + 3) / 4 * 4; }
+// M2L workgroup of M2L_WAVES wavefronts and group_of(p) target cubes whose FP32
+// locals live in LDS.  At order 4 each wavefront keeps its own copy of them
+// (plain adds, no atomics; 8 x 128 targets x 36 floats = 144 KiB).  At order 6
+// the operators are 4.6x larger, so groups stay at 256 targets for their reuse,
+// and the wavefronts share one copy through LDS atomic adds.
+constexpr int M2L_WAVES = 8;
+__host__ __device__ constexpr bool m2l_private(int p) { return p <= 4; }
+__host__ __device__ constexpr int group_of(int p) { return m2l_private(p) ? 128 : 256; }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -545,23 +550,27 @@ __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict_
 // Lattice M2L on the matrix cores.  One workgroup per group of GT target
 // cubes keeps their FP32 locals in LDS.  The group's pairs are binned by
 // operator; wavefront w takes the non-empty bins w, w+8, ..., loads the bin's
-// operator T (Kp x Kp) from L2 into registers once, multiplies it with the
-// bin's source moments 16 columns at a time by v_mfma_f32_16x16x4f32 and adds
-// the result into the targets' locals with LDS atomics: no barriers inside
-// the loop.
+// operator T (Kp x Kq) into registers once, multiplies it with the bin's
+// source moments 16 columns at a time by v_mfma_f32_16x16x4f32 and adds each
+// column into its target's locals.  With private copies (order 4) that add is
+// a plain 16-byte read-add-write in the wavefront's own copy: a target appears
+// once per bin and no other wavefront writes that copy; the copies are summed
+// in wavefront order at the end.  LDS float atomics, which the shared copy of
+// order 6 uses, cost about 135 cycles per instruction.
 // Operand layout (wave64): A[i][k] from lane i + 16k, B[k][j] from lane
 // j + 16k, D[4q + r][j] in lane j + 16q, element r.
 template <int P>
-__global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int nsplit,
-                                                      const uint32_t* __restrict__ bstart,
-                                                      const int* __restrict__ src,
-                                                      const int16_t* __restrict__ col,
-                                                      const float* __restrict__ T,
-                                                      const float* __restrict__ Qf,
-                                                      float* __restrict__ Lf) {
-    constexpr int K = ncoef(P), Kp = kpad(P), Kq = kq(P), RT = Kp / 16, KT = Kq / 4, NW = 8;
-    constexpr int GT = group_of(P);
-    __shared__ float acc[GT * K];
+__global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
+    int64_t nf, int noff, int nsplit, const uint32_t* __restrict__ bstart,
+    const int* __restrict__ src, const int16_t* __restrict__ col, const float* __restrict__ T,
+    const float* __restrict__ Qf, float* __restrict__ Lf) {
+    constexpr int K = ncoef(P), Kp = kpad(P), Kq = kq(P), RT = Kp / 16, KT = Kq / 4;
+    constexpr int NW = M2L_WAVES, NT = 64 * NW, GT = group_of(P);
+    constexpr bool PRIV = m2l_private(P);
+    constexpr int NC = PRIV ? NW : 1;      // copies of the locals
+    constexpr int LS = PRIV ? Kq : K;      // floats per target in a copy
+    __shared__ float4 acc4[(NC * GT * LS + 3) / 4];
+    float* acc = reinterpret_cast<float*>(acc4);
     // block (g, sp): target group g, offsets [o_lo, o_hi); with several
     // splits per group the partial sums meet in Lf by atomic adds
     const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
@@ -569,8 +578,9 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
     const int o_lo = sp * per, o_hi = min(noff, o_lo + per);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    for (int i = tid; i < GT * K; i += 512) acc[i] = 0.0f;
+    for (int i = tid; i < (NC * GT * LS + 3) / 4; i += NT) acc4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
+    float* __restrict__ mine = acc + (PRIV ? wv * GT * LS : 0);
     const uint32_t* bsg = bstart + (int64_t)g * noff;
     float av[RT][KT];
     int nbin = 0;  // running count of non-empty bins: bin k belongs to wavefront k % NW
@@ -617,23 +627,39 @@ __global__ __launch_bounds__(512) void k_fmm_m2l_mfma(int64_t nf, int noff, int 
 #pragma unroll
                 for (int kt = 0; kt < KT; ++kt)
                     d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d, 0, 0, 0);
-                if (ok)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * rt + 4 * lk + r;
-                        if (row < K) atomicAdd(&acc[c * K + row], d[r]);
+                const int row = 16 * rt + 4 * lk;  // rows row .. row + 3 of column li
+                if (!ok) continue;
+                if (PRIV) {
+                    // rows at or past Kq are padding (zero operator rows)
+                    if (row < Kq) {
+                        float4* a = reinterpret_cast<float4*>(mine + c * LS + row);
+                        float4 v = *a;
+                        v.x += d[0];
+                        v.y += d[1];
+                        v.z += d[2];
+                        v.w += d[3];
+                        *a = v;
                     }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (row + r < K) atomicAdd(&mine[c * LS + row + r], d[r]);
+                }
             }
         }
     }
     __syncthreads();
     const int64_t f0 = (int64_t)g * GT;
-    for (int i = tid; i < GT * K; i += 512) {
-        const int64_t f = f0 + i / K;
+    for (int i = tid; i < GT * K; i += NT) {
+        const int tc = i / K, k = i % K;
+        const int64_t f = f0 + tc;
         if (f >= nf) break;
-        float* dst = &Lf[f * Kp + i % K];
-        if (nsplit == 1) *dst = acc[i];
-        else if (acc[i] != 0.0f) atomicAdd(dst, acc[i]);
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NC; ++w) v += acc[(w * GT + tc) * LS + k];
+        float* dst = &Lf[f * Kp + k];
+        if (nsplit == 1) *dst = v;
+        else if (v != 0.0f) atomicAdd(dst, v);
     }
 }
 
@@ -1067,7 +1093,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
         if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
         HQ_CHECK_HIP(hipEventRecord(t->ev[6], s));
-        k_fmm_m2l_mfma<P><<<ngroups * nsplit, 512, 0, s>>>(
+        k_fmm_m2l_mfma<P><<<ngroups * nsplit, 64 * M2L_WAVES, 0, s>>>(
             nf, tb.noff, nsplit, st.bins.as<uint32_t>(), st.lsrc.as<int>(), st.lcol.as<int16_t>(),
             tb.T.as<float>(), st.Qf.as<float>(), st.Lf.as<float>());
         HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
