@@ -558,7 +558,9 @@ __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict_
 // in wavefront order at the end.  LDS float atomics, which the shared copy of
 // order 6 uses, cost about 135 cycles per instruction.
 // Operand layout (wave64): A[i][k] from lane i + 16k, B[k][j] from lane
-// j + 16k, D[4q + r][j] in lane j + 16q, element r.
+// j + 16k, D[4q + r][j] in lane j + 16q, element r.  The contraction index of
+// step kt in lane group lk is lk * KT + kt, so that each lane reads KT
+// consecutive moments of its source (wide loads).
 template <int P>
 __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
     int64_t nf, int noff, int nsplit, const uint32_t* __restrict__ bstart,
@@ -600,13 +602,18 @@ __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
             okn = e < b1;
             const int s = okn ? src[e] : 0;
             cn = okn ? col[e] : 0;
+            // this lane's KT consecutive moments (wide loads); s = 0 for an
+            // empty column reads a valid record
+            const float* q = Qf + (int64_t)s * Kq + lk * KT;
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kq + 4 * kt + lk] : 0.0f;
+            for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
         }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + 4 * kt + lk];
+            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + lk * KT + kt];
         for (int t = 0; t < nt; ++t) {
             float bv[KT];
 #pragma unroll
@@ -618,8 +625,11 @@ __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
                 okn = e < b1;
                 const int s = okn ? src[e] : 0;
                 cn = okn ? col[e] : 0;
+                const float* q = Qf + (int64_t)s * Kq + lk * KT;
 #pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? Qf[(int64_t)s * Kq + 4 * kt + lk] : 0.0f;
+                for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
             }
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
