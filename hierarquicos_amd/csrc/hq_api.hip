@@ -159,6 +159,8 @@ int hq_tree_create_batch(hq_tree** out, int nsys, const int64_t* counts, int met
 int hq_tree_destroy(hq_tree* t) {
     if (!t) return HQ_OK;
     if (t->stream) hipStreamSynchronize(t->stream);
+    for (auto& g : t->gcache)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     for (auto& e : t->ev)
         if (e) hipEventDestroy(e);
     if (t->hstatus) hipHostFree(t->hstatus);

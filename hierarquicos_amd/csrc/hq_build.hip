@@ -913,16 +913,24 @@ static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
     int64_t* gs = t->gstate.as<int64_t>();
     k_graph_build_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), cap, nsys, gs);
     launch_pointers(t, cap, nsys, gs + GS_NN);
-    // grids from the level sizes of the last launched build, with a margin
-    // (the kernel strides over whatever the level holds at replay)
+    // grids set by graph_grids before the capture (the kernel strides over
+    // whatever the level holds at replay)
     const UpArgs ua = up_args(t);
-    for (int d = NDEP - 1; d >= 0; --d) {
-        const int64_t sz = t->level_start[d + 1] - t->level_start[d];
-        const int g = (int)std::min<int64_t>(grid_for(sz + sz / 4 + 128, 128), grid_for(cap, 128));
-        k_upward_dev<<<g, 128, 0, s>>>(gs + GS_LEVEL0, d, ua);
-    }
+    for (int d = NDEP - 1; d >= 0; --d)
+        k_upward_dev<<<std::max(1, t->graph_lgrid[d]), 128, 0, s>>>(gs + GS_LEVEL0, d, ua);
     HQ_CHECK_HIP(hipGetLastError());
     return HQ_OK;
+}
+
+// Grids of a captured build's upward pass: each level's size in the last
+// launched build with a margin, never smaller than before, so that calls on
+// similar systems capture the same graph.
+void graph_grids(hq_tree* t) {
+    for (int d = 0; d < NDEP; ++d) {
+        const int64_t sz = t->level_start[d + 1] - t->level_start[d];
+        const int g = (int)std::min<int64_t>(grid_for(sz + sz / 4 + 128, 128), grid_for(t->cap, 128));
+        t->graph_lgrid[d] = std::max(t->graph_lgrid[d], g);
+    }
 }
 
 void collect_build_times(hq_tree* t) {

@@ -18,6 +18,8 @@
 #include <memory>
 #include <vector>
 
+#include <string.h>
+
 #include "hq_tree.h"
 
 namespace hq {
@@ -169,17 +171,55 @@ static void kick_drift(hq_tree* t, bool sorted, double h, int nkick, double dt) 
         t->z.as<double>(), t->px.as<double>(), t->py.as<double>(), t->pz.as<double>());
 }
 
-// Captures the launches body() makes on t->stream into a HIP graph and
-// replays it reps times.  body must only launch kernels on buffers that
-// already have their size (no allocation, copy from host or synchronisation).
+// Key of a captured step graph: everything its kernels take as arguments
+// (FNV-1a over the values).
+struct GraphKey {
+    uint64_t h = 1469598103934665603ull;
+    GraphKey& add(uint64_t v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v >> (8 * b)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+        return *this;
+    }
+    GraphKey& add(double v) {
+        uint64_t u;
+        memcpy(&u, &v, 8);
+        return add(u);
+    }
+    GraphKey& add(int64_t v) { return add((uint64_t)v); }
+    GraphKey& add(int v) { return add((uint64_t)(int64_t)v); }
+};
+
+// Steps unrolled into one captured graph.  Measured on the 64-window
+// Parareal run (10^4 bodies, 47 iterates): 1 step per graph 10.96 s, 8 steps
+// 11.63 s, 64 steps 12.26 s, and 11.75 s re-capturing every call.  The fine
+// propagator's batch shrinks every iterate, so its graph is captured anew
+// each time, and a longer graph takes longer to instantiate than its
+// launches save.
+constexpr int64_t GRAPH_UNROLL = 1;
+
+// The instantiated graph of `steps` unrolled calls of body() under `key`,
+// from the handle's cache, or captured now (evicting the least recently used
+// entry).  body must only launch kernels on buffers that already have their
+// size (no allocation, copy from host or synchronisation).
 template <class F>
-static int replay(hq_tree* t, F body, int64_t reps) {
+static int cached_graph(hq_tree* t, uint64_t key, F& body, int64_t steps, hipGraphExec_t* out) {
+    for (auto& e : t->gcache)
+        if (e.exec && e.key == key) {
+            e.used = ++t->gcache_clock;
+            *out = e.exec;
+            return HQ_OK;
+        }
+    hq_tree::GraphEntry* slot = &t->gcache[0];  // an empty entry, else the least recently used
+    for (auto& e : t->gcache)
+        if (!e.exec ? slot->exec != nullptr : (slot->exec && e.used < slot->used)) slot = &e;
     hipStream_t s = t->stream;
     hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
     HQ_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     g_capturing = true;
-    const int rc = body();
+    int rc = HQ_OK;
+    for (int64_t k = 0; k < steps && rc == HQ_OK; ++k) rc = body();
     g_capturing = false;
     const hipError_t ec = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -188,11 +228,37 @@ static int replay(hq_tree* t, F body, int64_t reps) {
     }
     if (ec != hipSuccess)
         return fail(HQ_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec));
-    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    for (int64_t r = 0; e == hipSuccess && r < reps; ++r) e = hipGraphLaunch(ge, s);
-    if (ge) (void)hipGraphExecDestroy(ge);
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(HQ_ERR_HIP, "graph replay failed: %s", hipGetErrorString(e));
+    if (ei != hipSuccess)
+        return fail(HQ_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    slot->exec = ge;
+    slot->key = key;
+    slot->used = ++t->gcache_clock;
+    *out = ge;
+    return HQ_OK;
+}
+
+// reps steps of body() as graph launches: groups of GRAPH_UNROLL steps
+// captured once, then the remainder; each graph is kept on the handle for the
+// next call with the same key (the captured kernels' arguments, reached
+// through base_key, and the buffer addresses, through the allocation epoch).
+template <class F>
+static int replay(hq_tree* t, GraphKey base_key, F body, int64_t reps) {
+    if (reps <= 0) return HQ_OK;
+    const int64_t u = std::min(reps, GRAPH_UNROLL), full = reps / u, rem = reps % u;
+    base_key.add(g_devbuf_epoch.load(std::memory_order_relaxed));
+    hipGraphExec_t ge = nullptr;
+    int rc;
+    if ((rc = cached_graph(t, GraphKey(base_key).add(u).h, body, u, &ge))) return rc;
+    for (int64_t r = 0; r < full; ++r)
+        HQ_CHECK_HIP(hipGraphLaunch(ge, t->stream));
+    if (rem) {
+        if ((rc = cached_graph(t, GraphKey(base_key).add(rem).h, body, rem, &ge))) return rc;
+        HQ_CHECK_HIP(hipGraphLaunch(ge, t->stream));
+    }
     return HQ_OK;
 }
 
@@ -210,16 +276,19 @@ static int integrate_direct_graph(hq_tree* t, int scheme, double dt, int64_t nst
             t->fy.as<double>(), t->fz.as<double>());
         return HQ_OK;
     };
+    GraphKey key;
+    key.add(1).add(scheme).add(dt).add(G).add(eps2).add(t->n).add(t->nsys).add(nblk);
     if (scheme == HQ_SCHEME_VERLET) {
         const double h = 0.5 * dt;
         forces();
         kick_drift(t, false, h, 1, dt);
         forces();
-        if ((rc = replay(t, [&]() { kick_drift(t, false, h, 2, dt); return forces(); }, nsteps - 1)))
+        if ((rc = replay(t, key, [&]() { kick_drift(t, false, h, 2, dt); return forces(); },
+                         nsteps - 1)))
             return rc;
         kick_drift(t, false, h, 1, 0.0);
     } else {
-        if ((rc = replay(t, [&]() { forces(); kick_drift(t, false, dt, 1, dt); return HQ_OK; },
+        if ((rc = replay(t, key, [&]() { forces(); kick_drift(t, false, dt, 1, dt); return HQ_OK; },
                          nsteps)))
             return rc;
     }
@@ -260,14 +329,25 @@ static int integrate_tree_graph(hq_tree* t, int scheme, double dt, int64_t nstep
         if ((rc = forces())) return rc;
         kick_drift(t, true, dt, 1, dt);
     }
-    t->graph_nbun = std::min<int64_t>(n, t->graph_grid > 0 ? t->graph_grid : 2 * t->ngroups + 64);
+    // the walk's grid: HQ_TUNE_GRAPH_GRID, else twice the bunches of this
+    // step plus 64, and never smaller than before on this handle (so that
+    // calls on similar systems capture the same graph)
+    t->graph_nbun = std::min<int64_t>(
+        n, t->graph_grid > 0 ? t->graph_grid : std::max<int64_t>(t->graph_nbun, 2 * t->ngroups + 64));
     if ((rc = graph_prepare(t))) return rc;  // after graph_nbun: split buffers depend on it
+    graph_grids(t);
     k_fill_i64<<<1, 64, 0, s>>>(t->gstate.as<int64_t>(), 0, GS_SIZE);
+    GraphKey key;
+    key.add(2).add(scheme).add(dt).add(theta2).add(G).add(eps2).add(n).add(t->nsys).add(t->cap);
+    key.add(t->graph_nbun).add((int)t->graph_split).add(t->split_lcap).add(t->multipole);
+    key.add(t->group_size).add(t->super_size).add(t->leaf).add(t->mac).add(t->far);
+    key.add(t->far_inner).add(t->split).add((int)t->fused).add(t->precision);
+    for (int d = 0; d <= MAX_DEPTH + 1; ++d) key.add(t->graph_lgrid[d]);
     t->graph_mode = true;
     if (scheme == HQ_SCHEME_VERLET)
-        rc = replay(t, [&]() { kick_drift(t, true, h, 2, dt); return forces(); }, nsteps - 1);
+        rc = replay(t, key, [&]() { kick_drift(t, true, h, 2, dt); return forces(); }, nsteps - 1);
     else
-        rc = replay(t, [&]() {
+        rc = replay(t, key, [&]() {
             int r = forces();
             kick_drift(t, true, dt, 1, dt);
             return r;

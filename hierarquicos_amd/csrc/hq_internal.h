@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <atomic>
 #include <string>
 
 #include "../../include/hq.h"
@@ -30,6 +31,10 @@ int fail(int code, const char* fmt, ...);
 // True while this thread records a stream capture (set by the integrator).
 bool capturing();
 
+// Bumped whenever any DevBuf gets or frees device memory: a captured HIP graph
+// holds buffer addresses, so one captured at another epoch may not be replayed.
+inline std::atomic<uint64_t> g_devbuf_epoch{0};
+
 // Simple owning device buffer that only grows (no free on shrink).
 struct DevBuf {
     void* p = nullptr;
@@ -45,6 +50,7 @@ struct DevBuf {
         if (p) hipFree(p);
         p = nullptr;
         bytes = 0;
+        g_devbuf_epoch.fetch_add(1, std::memory_order_relaxed);
         size_t b = need < 256 ? 256 : need;
         hipError_t e = hipMalloc(&p, b);
         if (e != hipSuccess) return fail(HQ_ERR_NOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
@@ -52,7 +58,10 @@ struct DevBuf {
         return HQ_OK;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) {
+            hipFree(p);
+            g_devbuf_epoch.fetch_add(1, std::memory_order_relaxed);
+        }
         p = nullptr;
         bytes = 0;
     }

@@ -85,7 +85,16 @@ struct hq_tree {
     // the errors of all replayed steps (GS_* slots); gsave the state before
     // the replays, to rerun them launched if an error was flagged.
     bool graph_mode = false;
-    int64_t graph_nbun = 0;  // grid of the walk in graph mode (upper bound of the bunches)
+    int64_t graph_nbun = 0;  // grid of the walk in graph mode (upper bound of the bunches;
+                             // only grows, so that repeated calls capture the same graph)
+    int graph_lgrid[hq::MAX_DEPTH + 2] = {};  // grids of the captured upward pass (only grow)
+    // instantiated step graphs of integrate, kept for calls with the same key
+    struct GraphEntry {
+        uint64_t key = 0, used = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    GraphEntry gcache[4];
+    uint64_t gcache_clock = 0;
     int64_t graph_grid = 0;  // HQ_TUNE_GRAPH_GRID: that grid if > 0, else 2 x bunches + 64
     int mac = 1;             // HQ_TUNE_MAC of the bunched walks: 0 cell side, 1 side + offset
     unsigned tune_user = 0;  // HQ_TUNE_* keys the caller set (bit per key): the table keeps off them
@@ -230,6 +239,8 @@ int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2);
 int tree_forces_fmm(hq_tree* t, int order, double theta, double G, double eps2, bool mfma, int S);
 int tree_scatter_forces(hq_tree* t, double* F, int layout, int where);
 int read_status(hq_tree* t);
+// grids of a captured build's upward pass from the last launched build (graph mode)
+void graph_grids(hq_tree* t);
 // the last launched build's phase times from its events (the stream has passed them)
 void collect_build_times(hq_tree* t);
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,

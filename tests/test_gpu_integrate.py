@@ -89,6 +89,30 @@ def test_bunched_graph_replay_matches_launched_steps(scheme):
         q, p = qg, pg
 
 
+@pytest.mark.parametrize("force", ["bunched", "direct"])
+def test_graph_cache_and_unrolled_steps(force):
+    """A handle keeps its instantiated step graphs: repeated calls with the
+    same parameters replay the kept graph (also over a longer run), and a
+    changed parameter captures anew while the earlier graph stays kept for
+    the next call that needs it.  Every call gives the launched steps' state
+    bit for bit."""
+    sizes = [2000, 600]
+    sys_ = [_plummer(k, 80 + i) for i, k in enumerate(sizes)]
+    m = np.concatenate([s[0] for s in sys_])
+    q = np.concatenate([s[1] + 15.0 * i for i, s in enumerate(sys_)])
+    p = np.concatenate([s[2] for s in sys_])
+    method = 11 if force == "bunched" else 10
+    prec = "mixed" if force == "bunched" else "fp64"
+    tg = OctreeType().pre_init(m, method, counts=sizes, precision=prec)
+    te = OctreeType().pre_init(m, method, counts=sizes, precision=prec).tune(graph=False, split=0)
+    for dt, nsteps in ((1e-3, 6), (1e-3, 6), (1e-3, 6), (1e-3, 70), (2e-3, 6), (1e-3, 6)):
+        tg.set_state(q, p).integrate(dt, nsteps, 0.25, 1.0, 1e-4, force=force)
+        te.set_state(q, p).integrate(dt, nsteps, 0.25, 1.0, 1e-4, force=force)
+        (qg, pg), (qe, pe) = tg.get_state(), te.get_state()
+        assert np.array_equal(qg, qe) and np.array_equal(pg, pe), (dt, nsteps)
+        q, p = qg, pg
+
+
 @pytest.mark.parametrize("split", [0, 1])
 def test_bunched_graph_grid_too_small_reruns_launched(split):
     """A replayed step with more bunches than the captured walk's grid (forced
