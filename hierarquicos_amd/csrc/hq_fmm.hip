@@ -100,8 +100,7 @@ struct FmmTables {
 struct FmmState {
     FmmTables tab;
     DevBuf flag, scan, fid, fnode, Qd, Qf, Ld, Lf, rmax, lvl;
-    DevBuf fa, fb, cnt, lat, gen, p2p, m2p, bins, binpos, lsrc, lcol, phist, ppos, plist;
-    DevBuf mhist, mpos, mlist;
+    DevBuf rec, code, fa, fb, cnt, lat, gen, p2p, m2p, bins, ghist, gpos, gpair, lsrc, lcol, phist, ppos, plist;
     int64_t nf = 0;
     std::vector<int> fls;  // FMM node range per depth
 };
@@ -359,88 +358,115 @@ struct WalkPar {
     int R, nd, nd1, noff0, noff1;
 };
 
-// Classifies the directed pair (a <- b): accept with the MAC, sum directly
-// when both are leaves, else split (both when of one level, else the larger
-// or the non-leaf).  *nch = child pairs produced by a split; *off = lattice
-// offset id for same-level accepts.
-__device__ __forceinline__ int fmm_classify(const FView& v, const double* __restrict__ rmax,
-                                            const WalkPar& w, int a, int b, int* nch, int* off,
-                                            int* split) {
-    *nch = 0;
-    *off = -1;
-    *split = 0;
-    const int fa = v.fid[a], fb = v.fid[b];
-    const bool la = v.flag[a] == 1, lb = v.flag[b] == 1;
-    const int da = v.depth[a], db = v.depth[b];
-    auto nkids = [&](int k) {
-        int n = 0;
-        for (int c = v.cstart[k]; c < v.cstart[k + 1]; ++c) n += v.count[c] > 0;
-        return n;
-    };
+// One FMM cell as the walk reads it (64 bytes, four 16-byte loads): centre,
+// bounding radius, side, children's FMM ids [kid0, kid1) (the non-empty
+// children of an internal cell are FMM cells with consecutive ids), body
+// count, flag | depth << 2.
+struct __align__(16) FRec {
+    double cx, cy, cz, rmax, side;
+    int kid0, kid1, count;
+    uint32_t info;
+};
+
+__global__ __launch_bounds__(256) void k_fmm_records(int nf, FView v, const uint32_t* __restrict__ scan,
+                                                     const double* __restrict__ rmax,
+                                                     FRec* __restrict__ rec) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int k = v.fnode[f];
+    const int flag = v.flag[k];
+    FRec r;
+    r.cx = v.cx[k];
+    r.cy = v.cy[k];
+    r.cz = v.cz[k];
+    r.rmax = rmax[f];
+    r.side = side_of(v, k);
+    r.kid0 = flag == 2 ? (int)scan[v.cstart[k]] : 0;
+    r.kid1 = flag == 2 ? (int)scan[v.cstart[k + 1]] : 0;
+    r.count = v.count[k];
+    r.info = (uint32_t)flag | ((uint32_t)v.depth[k] << 2);
+    rec[f] = r;
+}
+
+// Classifies the directed pair (a <- b) of FMM cells: accept with the MAC,
+// sum directly when both are leaves, else split (both when of one level,
+// else the larger or the non-leaf).  Returns the action code: the action in
+// bits 0-2; for a split, the split (1 = a, 2 = b, 3 = both) in bits 3-4 and
+// the number of child pairs in bits 5-11; for a lattice accept, the lattice
+// offset id from bit 3.
+__device__ __forceinline__ uint32_t fmm_classify(const FRec& ra, const FRec& rb, const WalkPar& w,
+                                                 int a, int b) {
+    if (a < 0) return ACT_NONE;
+    const bool la = (ra.info & 3) == 1, lb = (rb.info & 3) == 1;
+    const int da = (int)(ra.info >> 2), db = (int)(rb.info >> 2);
     if (a != b) {
-        const double dx = v.cx[b] - v.cx[a], dy = v.cy[b] - v.cy[a], dz = v.cz[b] - v.cz[a];
+        const double dx = rb.cx - ra.cx, dy = rb.cy - ra.cy, dz = rb.cz - ra.cz;
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        if (rmax[fa] + rmax[fb] <= w.theta * d) {
+        if (ra.rmax + rb.rmax <= w.theta * d) {
             // a leaf target much larger than its source: the local expansion
             // would carry the larger error, so the source acts on the bodies
             // directly (its multipole, or its bodies when it is a leaf)
-            if (la && (rmax[fa] > 2.0 * rmax[fb] || (rmax[fa] > rmax[fb] && v.count[a] <= 8)))
+            if (la && (ra.rmax > 2.0 * rb.rmax || (ra.rmax > rb.rmax && ra.count <= 8)))
                 return lb ? ACT_P2P : ACT_M2P;
             if (da == db) {
-                const double s = side_of(v, a);
+                const double s = ra.side;
                 const int ox = (int)rint(dx / s), oy = (int)rint(dy / s), oz = (int)rint(dz / s);
-                if (abs(ox) <= w.R && abs(oy) <= w.R && abs(oz) <= w.R) {
-                    *off = ((ox + w.R) * w.nd + (oy + w.R)) * w.nd + (oz + w.R);
-                    return ACT_LATTICE;
-                }
+                if (abs(ox) <= w.R && abs(oy) <= w.R && abs(oz) <= w.R)
+                    return ACT_LATTICE |
+                           ((uint32_t)(((ox + w.R) * w.nd + (oy + w.R)) * w.nd + (oz + w.R)) << 3);
             } else if (da == db - 1 || da == db + 1) {
                 // half-integer offset in units of the finer cube
-                const double s = side_of(v, da > db ? a : b);
+                const double s = da > db ? ra.side : rb.side;
                 const int wx = (int)rint(2.0 * dx / s), wy = (int)rint(2.0 * dy / s),
                           wz = (int)rint(2.0 * dz / s);
                 const int lim = 2 * w.R + 1;
                 if ((wx & 1) && (wy & 1) && (wz & 1) && abs(wx) <= lim && abs(wy) <= lim &&
                     abs(wz) <= lim) {
                     const int ix = (wx + lim) / 2, iy = (wy + lim) / 2, iz = (wz + lim) / 2;
-                    *off = w.noff0 + (da < db ? 0 : w.noff1) + (ix * w.nd1 + iy) * w.nd1 + iz;
-                    return ACT_LATTICE;
+                    const int off = w.noff0 + (da < db ? 0 : w.noff1) + (ix * w.nd1 + iy) * w.nd1 + iz;
+                    return ACT_LATTICE | ((uint32_t)off << 3);
                 }
             }
             return ACT_GENERIC;
         }
     }
     if (la && lb) return ACT_P2P;
-    // split: 1 = a, 2 = b, 3 = both
     int sp;
     if (a == b) sp = 3;
     else if (la) sp = 2;
     else if (lb) sp = 1;
     else if (da == db) sp = 3;
     else sp = da < db ? 1 : 2;
-    *split = sp;
-    const int na = (sp & 1) ? nkids(a) : 1, nb = (sp & 2) ? nkids(b) : 1;
-    *nch = na * nb;
-    return ACT_EXPAND;
+    const int na = (sp & 1) ? ra.kid1 - ra.kid0 : 1, nb = (sp & 2) ? rb.kid1 - rb.kid0 : 1;
+    return ACT_EXPAND | ((uint32_t)sp << 3) | ((uint32_t)(na * nb) << 5);
+}
+
+__device__ __forceinline__ int code_act(uint32_t c) { return (int)(c & 7u); }
+__device__ __forceinline__ uint32_t code_count(uint32_t c) {
+    const int act = code_act(c);
+    return act == ACT_EXPAND ? (c >> 5) & 127u : (act == ACT_NONE ? 0u : 1u);
 }
 
 constexpr int WB = 256;
 
-// per block: counts of next-frontier pairs, lattice, generic, p2p
-__global__ __launch_bounds__(WB) void k_fmm_walk_count(FView v, const double* __restrict__ rmax,
-                                                       WalkPar w, const int2* __restrict__ fr,
-                                                       int64_t nfr, uint32_t* __restrict__ cnt,
-                                                       int nblk) {
+// Classifies every frontier pair once, keeps its code, and stores per block
+// the counts of next-frontier pairs, lattice, generic, near and M2P pairs.
+__global__ __launch_bounds__(WB) void k_fmm_walk_count(const FRec* __restrict__ rec, WalkPar w,
+                                                       const int2* __restrict__ fr, int64_t nfr,
+                                                       uint32_t* __restrict__ code,
+                                                       uint32_t* __restrict__ cnt, int nblk) {
     __shared__ uint32_t ws[NACT][WB / 64];
     const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
-    uint32_t c[NACT] = {0, 0, 0, 0, 0};
+    uint32_t c = ACT_NONE;
     if (i < nfr) {
-        int nch, off, sp;
-        const int act = fmm_classify(v, rmax, w, fr[i].x, fr[i].y, &nch, &off, &sp);
-        if (act == ACT_EXPAND) c[0] = nch;
-        else c[act] = 1;
+        const int a = fr[i].x, b = fr[i].y;
+        c = a < 0 ? (uint32_t)ACT_NONE : fmm_classify(rec[a], rec[b], w, a, b);
+        code[i] = c;
     }
+    const int act = code_act(c);
+    const uint32_t cv = code_count(c);
     for (int q = 0; q < NACT; ++q) {
-        const uint32_t t = wave_sum(c[q]);
+        const uint32_t t = wave_sum(act == q ? cv : 0u);
         if (__lane_id() == 0) ws[q][threadIdx.x >> 6] = t;
     }
     __syncthreads();
@@ -448,12 +474,18 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_count(FView v, const double* __
         uint32_t t = 0;
         for (int k = 0; k < WB / 64; ++k) t += ws[threadIdx.x][k];
         cnt[(int64_t)threadIdx.x * (nblk + 1) + blockIdx.x] = t;
+        if (blockIdx.x == 0) cnt[(int64_t)threadIdx.x * (nblk + 1) + nblk] = 0;
     }
 }
 
-__global__ __launch_bounds__(WB) void k_fmm_walk_emit(FView v, const double* __restrict__ rmax,
-                                                      WalkPar w, const int2* __restrict__ fr,
-                                                      int64_t nfr, const uint32_t* __restrict__ cnt,
+// Writes each pair's output at its place (block prefix from the counts,
+// scanned as one array [category][block], then the wavefront's prefix): a
+// split's child pairs to the next frontier, an accept or near pair to its
+// list.
+__global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ rec,
+                                                      const int2* __restrict__ fr, int64_t nfr,
+                                                      const uint32_t* __restrict__ code,
+                                                      const uint32_t* __restrict__ cnt,
                                                       int nblk, int2* __restrict__ next,
                                                       int3* __restrict__ lat, int64_t lat0,
                                                       int2* __restrict__ gen, int64_t gen0,
@@ -461,88 +493,182 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(FView v, const double* __r
                                                       int2* __restrict__ m2p, int64_t m2p0) {
     __shared__ uint32_t ws[NACT][WB / 64];
     const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
-    int act = ACT_NONE, nch = 0, off = -1, sp = 0, a = 0, b = 0;
+    uint32_t c = ACT_NONE;
+    int a = 0, b = 0;
     if (i < nfr) {
+        c = code[i];
         a = fr[i].x;
         b = fr[i].y;
-        act = fmm_classify(v, rmax, w, a, b, &nch, &off, &sp);
     }
-    uint32_t c[NACT] = {act == ACT_EXPAND ? (uint32_t)nch : 0u, act == ACT_LATTICE ? 1u : 0u,
-                        act == ACT_GENERIC ? 1u : 0u, act == ACT_P2P ? 1u : 0u,
-                        act == ACT_M2P ? 1u : 0u};
+    const int act = code_act(c);
+    const uint32_t cv = code_count(c);
     uint32_t pre[NACT];
     for (int q = 0; q < NACT; ++q) {
-        const uint32_t incl = wave_incl_scan(c[q]);
-        pre[q] = incl - c[q];
+        const uint32_t v = act == q ? cv : 0u;
+        const uint32_t incl = wave_incl_scan(v);
+        pre[q] = incl - v;
         if (__lane_id() == 63) ws[q][threadIdx.x >> 6] = incl;
     }
     __syncthreads();
     const int wv = threadIdx.x >> 6;
     for (int q = 0; q < NACT; ++q) {
-        uint32_t base = cnt[(int64_t)q * (nblk + 1) + blockIdx.x];
+        uint32_t base = cnt[(int64_t)q * (nblk + 1) + blockIdx.x] - cnt[(int64_t)q * (nblk + 1)];
         for (int k = 0; k < wv; ++k) base += ws[q][k];
         pre[q] += base;
     }
     if (act == ACT_EXPAND) {
-        const int a0 = (sp & 1) ? v.cstart[a] : a, a1 = (sp & 1) ? v.cstart[a + 1] : a + 1;
-        const int b0 = (sp & 2) ? v.cstart[b] : b, b1 = (sp & 2) ? v.cstart[b + 1] : b + 1;
+        const int sp = (int)((c >> 3) & 3u);
+        int a0 = a, a1 = a + 1, b0 = b, b1 = b + 1;
+        if (sp & 1) a0 = rec[a].kid0, a1 = rec[a].kid1;
+        if (sp & 2) b0 = rec[b].kid0, b1 = rec[b].kid1;
         int64_t o = pre[0];
-        for (int x = a0; x < a1; ++x) {
-            if ((sp & 1) && v.count[x] == 0) continue;
-            for (int y = b0; y < b1; ++y) {
-                if ((sp & 2) && v.count[y] == 0) continue;
-                next[o++] = make_int2(x, y);
-            }
-        }
+        for (int x = a0; x < a1; ++x)
+            for (int y = b0; y < b1; ++y) next[o++] = make_int2(x, y);
     } else if (act == ACT_LATTICE) {
-        lat[lat0 + pre[1]] = make_int3(v.fid[a], v.fid[b], off);
+        lat[lat0 + pre[1]] = make_int3(a, b, (int)(c >> 3));
     } else if (act == ACT_GENERIC) {
-        gen[gen0 + pre[2]] = make_int2(v.fid[a], v.fid[b]);
+        gen[gen0 + pre[2]] = make_int2(a, b);
     } else if (act == ACT_P2P) {
-        p2p[p2p0 + pre[3]] = make_int2(v.fid[a], v.fid[b]);
+        p2p[p2p0 + pre[3]] = make_int2(a, b);
     } else if (act == ACT_M2P) {
-        m2p[m2p0 + pre[4]] = make_int2(v.fid[a], v.fid[b]);
+        m2p[m2p0 + pre[4]] = make_int2(a, b);
     }
 }
 
 // ------------------------------------------------------------------ binning
-// lattice pairs -> bins (target group, offset); near pairs -> bins (target)
-__global__ __launch_bounds__(256) void k_fmm_lat_hist(const int3* __restrict__ lat, int64_t n,
-                                                      int noff, int gt, uint32_t* __restrict__ hist) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int3 e = lat[i];
-        atomicAdd(&hist[(int64_t)(e.x / gt) * noff + e.z], 1u);
+// Lattice pairs -> bins (target group, offset), without contended global
+// atomics: first by target group (the walk emits a target's pairs, and its
+// siblings', next to each other, so a run of equal groups within a
+// wavefront takes one atomic), then one workgroup per group sorts the
+// group's pairs by offset with an LDS histogram and writes the group's row
+// of bin starts.  Near pairs -> bins (target) the same way as the first step.
+// A run of equal keys within a wavefront: one atomic, made by its first lane.
+struct RunInfo {
+    bool valid, head;
+    int count, hlane;  // run length (at the head), the run's head lane
+};
+
+__device__ __forceinline__ RunInfo run_of(int key, int64_t base, int64_t n) {
+    const int l = __lane_id();
+    const bool valid = base + l < n;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (l == 0 || key != prev);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = (int)min<int64_t>(64, n - base);
+    const uint64_t above = l == 63 ? 0ull : heads & (~0ull << (l + 1));
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : nvalid;
+    const uint64_t le = l == 63 ? heads : heads & ((2ull << l) - 1);
+    const int h = le ? 63 - __clzll((long long)le) : 0;
+    return RunInfo{valid, head, next - l, h};
+}
+
+__global__ __launch_bounds__(256) void k_fmm_lat_ghist(const int3* __restrict__ lat, int64_t n, int gt,
+                                                       uint32_t* __restrict__ ghist) {
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + __lane_id();
+        const int key = i < n ? lat[i].x / gt : -1;
+        const RunInfo r = run_of(key, base, n);
+        if (r.head) atomicAdd(&ghist[key], (uint32_t)r.count);
     }
 }
 
-__global__ __launch_bounds__(256) void k_fmm_lat_scatter(const int3* __restrict__ lat, int64_t n,
-                                                         int noff, int gt, uint32_t* __restrict__ pos,
-                                                         int* __restrict__ src,
-                                                         int16_t* __restrict__ col) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int3 e = lat[i];
-        const uint32_t o = atomicAdd(&pos[(int64_t)(e.x / gt) * noff + e.z], 1u);
-        src[o] = e.y;
-        col[o] = (int16_t)(e.x % gt);
+// pair -> (source, column << 24 | offset) at its group's next free place
+__global__ __launch_bounds__(256) void k_fmm_lat_gscatter(const int3* __restrict__ lat, int64_t n,
+                                                          int gt, uint32_t* __restrict__ gpos,
+                                                          int2* __restrict__ out) {
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + __lane_id();
+        const int3 e = i < n ? lat[i] : make_int3(0, 0, 0);
+        const int key = i < n ? e.x / gt : -1;
+        const RunInfo r = run_of(key, base, n);
+        uint32_t o = 0;
+        if (r.head) o = atomicAdd(&gpos[key], (uint32_t)r.count);
+        o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
+        if (r.valid) out[o] = make_int2(e.y, (int)(((uint32_t)(e.x - key * gt) << 24) | (uint32_t)e.z));
     }
+}
+
+// One workgroup per target group: its pairs [gstart[g], gstart[g+1]) counted
+// by offset in LDS (LSORT_CH offsets per pass), the counts scanned, the row
+// bins[g * noff + o] of bin starts written, and the pairs placed by offset.
+constexpr int LSORT_T = 1024;
+constexpr int LSORT_CH = 32768;
+
+__global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restrict__ gp,
+                                                           const uint32_t* __restrict__ gstart,
+                                                           int ngroups, int noff,
+                                                           uint32_t* __restrict__ bins,
+                                                           int* __restrict__ src,
+                                                           int16_t* __restrict__ col) {
+    __shared__ uint32_t h[LSORT_CH];
+    __shared__ uint32_t ws[LSORT_T / 64];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const uint32_t s0 = gstart[g], s1 = gstart[g + 1];
+    uint32_t* row = bins + (int64_t)g * noff;
+    uint32_t done = 0;  // pairs with offsets below this pass's
+    for (int c0 = 0; c0 < noff; c0 += LSORT_CH) {
+        const int nc = min(LSORT_CH, noff - c0);
+        for (int o = tid; o < nc; o += LSORT_T) h[o] = 0;
+        __syncthreads();
+        for (uint32_t e = s0 + tid; e < s1; e += LSORT_T) {
+            const int o = (int)((uint32_t)gp[e].y & 0xFFFFFFu) - c0;
+            if (o >= 0 && o < nc) atomicAdd(&h[o], 1u);
+        }
+        __syncthreads();
+        // exclusive scan of h[0, nc), each thread over a contiguous run
+        const int per = (nc + LSORT_T - 1) / LSORT_T;
+        const int oa = min(nc, tid * per), ob = min(nc, oa + per);
+        uint32_t sum = 0;
+        for (int o = oa; o < ob; ++o) sum += h[o];
+        uint32_t tot;
+        uint32_t run = block_excl_scan(sum, ws, tot) + done;
+        for (int o = oa; o < ob; ++o) {
+            const uint32_t c = h[o];
+            h[o] = run;
+            row[c0 + o] = s0 + run;
+            run += c;
+        }
+        __syncthreads();
+        for (uint32_t e = s0 + tid; e < s1; e += LSORT_T) {
+            const int2 v = gp[e];
+            const int o = (int)((uint32_t)v.y & 0xFFFFFFu) - c0;
+            if (o >= 0 && o < nc) {
+                const uint32_t p = s0 + atomicAdd(&h[o], 1u);
+                src[p] = v.x;
+                col[p] = (int16_t)((uint32_t)v.y >> 24);
+            }
+        }
+        done += tot;
+        __syncthreads();
+    }
+    if (g == ngroups - 1 && tid == 0) bins[(int64_t)ngroups * noff] = s1;
 }
 
 __global__ __launch_bounds__(256) void k_fmm_p2p_hist(const int2* __restrict__ pr, int64_t n,
                                                       uint32_t* __restrict__ hist) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&hist[pr[i].x], 1u);
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + __lane_id();
+        const int key = i < n ? pr[i].x : -1;
+        const RunInfo r = run_of(key, base, n);
+        if (r.head) atomicAdd(&hist[key], (uint32_t)r.count);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict__ pr, int64_t n,
                                                          uint32_t* __restrict__ pos,
                                                          int* __restrict__ out) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t o = atomicAdd(&pos[pr[i].x], 1u);
-        out[o] = pr[i].y;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + __lane_id();
+        const int2 e = i < n ? pr[i] : make_int2(-1, 0);
+        const RunInfo r = run_of(e.x, base, n);
+        uint32_t o = 0;
+        if (r.head) o = atomicAdd(&pos[e.x], (uint32_t)r.count);
+        o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
+        if (r.valid) out[o] = e.y;
     }
 }
 
@@ -586,74 +712,83 @@ __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
     const uint32_t* bsg = bstart + (int64_t)g * noff;
     float av[RT][KT];
     int nbin = 0;  // running count of non-empty bins: bin k belongs to wavefront k % NW
-    for (int o = o_lo; o < o_hi; ++o) {
-        const uint32_t b0 = bsg[o], b1 = bsg[o + 1];
-        if (b0 == b1) continue;
-        if (nbin++ % NW != wv) continue;
-        const int nt = (int)((b1 - b0 + 15) / 16);
-        const float* to = T + (int64_t)o * Kp * Kq;
-        // the operator and the first tile's sources are fetched together; each
-        // later tile's sources are fetched while the previous tile computes
-        float bn[KT];
-        int cn;
-        bool okn;
-        {
-            const uint32_t e = b0 + (uint32_t)li;
-            okn = e < b1;
-            const int s = okn ? src[e] : 0;
-            cn = okn ? col[e] : 0;
-            // this lane's KT consecutive moments (wide loads); s = 0 for an
-            // empty column reads a valid record
-            const float* q = Qf + (int64_t)s * Kq + lk * KT;
-#pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
-#pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
-        }
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + lk * KT + kt];
-        for (int t = 0; t < nt; ++t) {
-            float bv[KT];
-#pragma unroll
-            for (int kt = 0; kt < KT; ++kt) bv[kt] = bn[kt];
-            const bool ok = okn;
-            const int c = cn;
-            if (t + 1 < nt) {
-                const uint32_t e = b0 + (uint32_t)((t + 1) * 16) + li;
+    // the bin bounds are read 64 offsets at a time, one per lane; the
+    // non-empty ones are taken in order from the ballot
+    for (int ob = o_lo; ob < o_hi; ob += 64) {
+        const int ol = ob + lane;
+        const uint32_t lb0 = ol < o_hi ? bsg[ol] : 0u, lb1 = ol < o_hi ? bsg[ol + 1] : 0u;
+        uint64_t live = __ballot(lb0 != lb1);
+        while (live) {
+            const int l = __ffsll((unsigned long long)live) - 1;
+            live &= live - 1;
+            if (nbin++ % NW != wv) continue;
+            const int o = ob + l;
+            const uint32_t b0 = __shfl(lb0, l, 64), b1 = __shfl(lb1, l, 64);
+            const int nt = (int)((b1 - b0 + 15) / 16);
+            const float* to = T + (int64_t)o * Kp * Kq;
+            // the operator and the first tile's sources are fetched together; each
+            // later tile's sources are fetched while the previous tile computes
+            float bn[KT];
+            int cn;
+            bool okn;
+            {
+                const uint32_t e = b0 + (uint32_t)li;
                 okn = e < b1;
                 const int s = okn ? src[e] : 0;
                 cn = okn ? col[e] : 0;
+                // this lane's KT consecutive moments (wide loads); s = 0 for an
+                // empty column reads a valid record
                 const float* q = Qf + (int64_t)s * Kq + lk * KT;
-#pragma unroll
+    #pragma unroll
                 for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
-#pragma unroll
+    #pragma unroll
                 for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
             }
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                floatx4 d = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kt = 0; kt < KT; ++kt)
-                    d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d, 0, 0, 0);
-                const int row = 16 * rt + 4 * lk;  // rows row .. row + 3 of column li
-                if (!ok) continue;
-                if (PRIV) {
-                    // rows at or past Kq are padding (zero operator rows)
-                    if (row < Kq) {
-                        float4* a = reinterpret_cast<float4*>(mine + c * LS + row);
-                        float4 v = *a;
-                        v.x += d[0];
-                        v.y += d[1];
-                        v.z += d[2];
-                        v.w += d[3];
-                        *a = v;
+    #pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+    #pragma unroll
+                for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + lk * KT + kt];
+            for (int t = 0; t < nt; ++t) {
+                float bv[KT];
+    #pragma unroll
+                for (int kt = 0; kt < KT; ++kt) bv[kt] = bn[kt];
+                const bool ok = okn;
+                const int c = cn;
+                if (t + 1 < nt) {
+                    const uint32_t e = b0 + (uint32_t)((t + 1) * 16) + li;
+                    okn = e < b1;
+                    const int s = okn ? src[e] : 0;
+                    cn = okn ? col[e] : 0;
+                    const float* q = Qf + (int64_t)s * Kq + lk * KT;
+    #pragma unroll
+                    for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
+    #pragma unroll
+                    for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
+                }
+    #pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    floatx4 d = {0.f, 0.f, 0.f, 0.f};
+    #pragma unroll
+                    for (int kt = 0; kt < KT; ++kt)
+                        d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d, 0, 0, 0);
+                    const int row = 16 * rt + 4 * lk;  // rows row .. row + 3 of column li
+                    if (!ok) continue;
+                    if (PRIV) {
+                        // rows at or past Kq are padding (zero operator rows)
+                        if (row < Kq) {
+                            float4* a = reinterpret_cast<float4*>(mine + c * LS + row);
+                            float4 v = *a;
+                            v.x += d[0];
+                            v.y += d[1];
+                            v.z += d[2];
+                            v.w += d[3];
+                            *a = v;
+                        }
+                    } else {
+    #pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (row + r < K) atomicAdd(&mine[c * LS + row + r], d[r]);
                     }
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (row + r < K) atomicAdd(&mine[c * LS + row + r], d[r]);
                 }
             }
         }
@@ -760,9 +895,17 @@ __global__ __launch_bounds__(64) void k_fmm_l2l(int f0, int f1, FView v, const i
 }
 
 // Leaves: L2P (FP64) plus the near field from the leaf's near list, one
-// wavefront per leaf, one lane per body.  FP32 mode sums the near field in
+// wavefront per leaf.  The wavefront splits into 64 / W lane groups of W
+// lanes, W the smallest power of two >= the leaf's count: lane t of group g
+// holds body t, takes the L2P terms n = 1 + g, 1 + g + 64/W, ... and every
+// (64/W)-th staged source body from the g-th, and the groups' sums meet at
+// the end by cross-lane adds (so a leaf of 16 bodies keeps all 64 lanes
+// busy).  Source bodies are staged in LDS, up to NB at a time: the near-list
+// entries are read 64 at a time, one per lane, and their bodies are copied by
+// 16-lane quarters, one entry per quarter.  FP32 mode sums the near field in
 // FP32 on coordinates recentred on the leaf centre, Kahan-compensated per
-// source leaf; FP64 mode in FP64.
+// staged batch; FP64 mode in FP64.  A leaf holds at most 64 bodies (cells of
+// more are split, and a tree leaf holds one).
 template <int P, bool F32>
 __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* __restrict__ sx,
                                                  const double* __restrict__ sy,
@@ -784,9 +927,12 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
     const int k = v.fnode[f];
     if (v.flag[k] != 1) return;
     const int lane = threadIdx.x;
-    const int first = v.first[k], cnt = v.count[k];
-    const bool ok = lane < cnt;
-    const int i = first + (ok ? lane : 0);
+    const int first = v.first[k], cnt = min(v.count[k], 64);
+    int W = 1;
+    while (W < cnt) W <<= 1;
+    const int NG = 64 / W, tl = lane & (W - 1), grp = lane / W;
+    const bool ok = tl < cnt;
+    const int i = first + (ok ? tl : 0);
     const double s = side_of(v, k), c[3] = {v.cx[k], v.cy[k], v.cz[k]};
     const double xi = sx[i], yi = sy[i], zi = sz[i];
     // L2P: a = -(1/s^2) sum_n L~_n n_d u~^(n - e_d); a zero-size cube (one
@@ -800,7 +946,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
     }
     double g[3] = {0.0, 0.0, 0.0};
     const double* L = Ld + (int64_t)f * K;
-    for (int n = 1; n < K; ++n) {
+    for (int n = 1 + grp; n < K; n += NG) {
         const int nx = mi[4 * n], ny = mi[4 * n + 1], nz = mi[4 * n + 2];
         const double l = L[n];
         if (nx) g[0] += l * nx * pw[0][nx - 1] * pw[1][ny] * pw[2][nz];
@@ -809,43 +955,49 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
     }
     const double sc = -si * si;
     double ax = sc * g[0], ay = sc * g[1], az = sc * g[2];
-    // near field: the bodies of consecutive source leaves are staged in LDS,
-    // up to NB at a time, then every lane sums them (FP32 recentred on the
-    // leaf centre with a Kahan-compensated total, or FP64)
     float ks[3] = {0.f, 0.f, 0.f}, kc[3] = {0.f, 0.f, 0.f};
     const float tx = (float)(xi - c[0]), ty = (float)(yi - c[1]), tz = (float)(zi - c[2]);
-    const uint32_t e0 = pstart[f], e1 = pstart[f + 1];
-    uint32_t e = e0;
-    while (e < e1) {
-        int fill = 0;
+    const uint32_t e1 = pstart[f + 1];
+    const int q = lane >> 4, ql = lane & 15;
+    for (uint32_t e = pstart[f]; e < e1;) {
+        // this lane's entry e + lane: its body range
+        const uint32_t ee = e + (uint32_t)lane;
+        int bf = 0, bc = 0;
+        if (ee < e1) {
+            const int kb = v.fnode[plist[ee]];
+            bf = v.first[kb];
+            bc = min(v.count[kb], 64);
+        }
+        // the longest run of entries whose bodies fit in NB (at least one)
+        const int incl = wave_incl_scan(bc);
+        const int take = max(1, __popcll(__ballot(ee < e1 && incl <= NB)));
+        const int fill = min(NB, __shfl(incl, take - 1, 64));
         __syncthreads();  // the previous batch is no longer read
-        while (e < e1) {
-            const int kb = v.fnode[plist[e]];
-            const int bc = v.count[kb];
-            if (fill + bc > NB) break;
-            const int bf = v.first[kb];
-            if (lane < bc) {
-                const int j = bf + lane;
-                if constexpr (F32)
-                    sbuf[fill + lane] = make_float4((float)(sx[j] - c[0]), (float)(sy[j] - c[1]),
-                                                    (float)(sz[j] - c[2]), (float)sm[j]);
-                else
-                    sbuf[fill + lane] = make_double4(sx[j], sy[j], sz[j], sm[j]);
-                sidx[fill + lane] = j;
-            }
-            fill += bc;
-            ++e;
+        for (int j0 = 0; j0 < take; j0 += 4) {
+            const int j = j0 + q;
+            const int jf = __shfl(bf, j & 63, 64), jc = __shfl(bc, j & 63, 64);
+            const int jp = __shfl(incl - bc, j & 63, 64);
+            if (j < take)
+                for (int b = ql; b < jc && jp + b < NB; b += 16) {
+                    const int jj = jf + b;
+                    if constexpr (F32)
+                        sbuf[jp + b] = make_float4((float)(sx[jj] - c[0]), (float)(sy[jj] - c[1]),
+                                                   (float)(sz[jj] - c[2]), (float)sm[jj]);
+                    else
+                        sbuf[jp + b] = make_double4(sx[jj], sy[jj], sz[jj], sm[jj]);
+                    sidx[jp + b] = jj;
+                }
         }
         __syncthreads();
         if constexpr (F32) {
             float p[3] = {0.f, 0.f, 0.f};
             const float e2 = (float)eps2;
-            for (int j = 0; j < fill; ++j) {
-                const float4 q = sbuf[j];
-                const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+            for (int j = grp; j < fill; j += NG) {
+                const float4 sq = sbuf[j];
+                const float dx = sq.x - tx, dy = sq.y - ty, dz = sq.z - tz;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
                 const float ri = __frsqrt_rn(d2);
-                const float w = sidx[j] == i ? 0.0f : q.w * ri * ri * ri;
+                const float w = sidx[j] == i ? 0.0f : sq.w * ri * ri * ri;
                 p[0] = fmaf(w, dx, p[0]);
                 p[1] = fmaf(w, dy, p[1]);
                 p[2] = fmaf(w, dz, p[2]);
@@ -857,24 +1009,30 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
                 ks[d] = tv;
             }
         } else {
-            for (int j = 0; j < fill; ++j) {
+            for (int j = grp; j < fill; j += NG) {
                 if (sidx[j] == i) continue;
-                const V4 q = sbuf[j];
-                const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const V4 sq = sbuf[j];
+                const double dx = sq.x - xi, dy = sq.y - yi, dz = sq.z - zi;
                 const double d2 = dx * dx + dy * dy + dz * dz + eps2;
-                const double w = q.w / (sqrt(d2) * d2);
+                const double w = sq.w / (sqrt(d2) * d2);
                 ax += w * dx;
                 ay += w * dy;
                 az += w * dz;
             }
         }
+        e += (uint32_t)take;
     }
     if (F32) {
         ax += (double)ks[0] - (double)kc[0];
         ay += (double)ks[1] - (double)kc[1];
         az += (double)ks[2] - (double)kc[2];
     }
-    if (ok) {
+    for (int m = W; m < 64; m <<= 1) {
+        ax += __shfl_xor(ax, m, 64);
+        ay += __shfl_xor(ay, m, 64);
+        az += __shfl_xor(az, m, 64);
+    }
+    if (ok && grp == 0) {
         const double gm = G * sm[i];
         fx[i] = gm * ax;
         fy[i] = gm * ay;
@@ -883,37 +1041,32 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
 }
 
 // M2P of the lopsided pairs (a leaf target much larger than its source cell):
-// one wavefront per target leaf, one lane per body, added to its force.
+// one wavefront per pair, one lane per target body, added to its force with
+// atomic adds (a target's pairs run in parallel).
 template <int P>
-__global__ __launch_bounds__(64) void k_fmm_m2p(int nf, FView v, const double* __restrict__ sx,
+__global__ __launch_bounds__(64) void k_fmm_m2p(const int2* __restrict__ pr, int64_t npr, FView v,
+                                                const double* __restrict__ sx,
                                                 const double* __restrict__ sy,
                                                 const double* __restrict__ sz,
                                                 const double* __restrict__ sm,
                                                 const int* __restrict__ mi2,
                                                 const int* __restrict__ rec2,
-                                                const double* __restrict__ Qd,
-                                                const uint32_t* __restrict__ mstart,
-                                                const int* __restrict__ mlist, double G,
+                                                const double* __restrict__ Qd, double G,
                                                 double* __restrict__ fx, double* __restrict__ fy,
                                                 double* __restrict__ fz) {
     constexpr int K = ncoef(P), K1 = ncoef(P + 1);
     __shared__ double at[K1][64];  // per-lane Taylor coefficients
-    const int f = blockIdx.x;
-    if (f >= nf || mstart[f] == mstart[f + 1]) return;
-    const int k = v.fnode[f];
     const int lane = threadIdx.x;
-    const int first = v.first[k], cnt = v.count[k];
-    const bool ok = lane < cnt;
-    const int i = first + (ok ? lane : 0);
-    const double xi = sx[i], yi = sy[i], zi = sz[i];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    // M2P: a_d = s^-2 sum_m (-1)^|m| Q~_m (m_d + 1) a_{m+e_d}((x - c_B) / s),
-    // accumulated as each a_k (|k| <= p+1) is produced
-    for (uint32_t e = mstart[f]; e < mstart[f + 1]; ++e) {
-        const int fb = mlist[e];
-        const int kb = v.fnode[fb];
+    for (int64_t e = blockIdx.x; e < npr; e += gridDim.x) {
+        const int fa = pr[e].x, fb = pr[e].y;
+        const int k = v.fnode[fa], kb = v.fnode[fb];
+        const int first = v.first[k], cnt = v.count[k];
+        const bool ok = lane < cnt;
+        const int i = first + (ok ? lane : 0);
+        // M2P: a_d = s^-2 sum_m (-1)^|m| Q~_m (m_d + 1) a_{m+e_d}((x - c_B) / s),
+        // accumulated as each a_k (|k| <= p+1) is produced
         const double sb = side_of(v, kb);
-        const double R[3] = {(xi - v.cx[kb]) / sb, (yi - v.cy[kb]) / sb, (zi - v.cz[kb]) / sb};
+        const double R[3] = {(sx[i] - v.cx[kb]) / sb, (sy[i] - v.cy[kb]) / sb, (sz[i] - v.cz[kb]) / sb};
         const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
         const double* q = Qd + (int64_t)fb * K;
         double g2[3] = {0.0, 0.0, 0.0};
@@ -936,22 +1089,21 @@ __global__ __launch_bounds__(64) void k_fmm_m2p(int nf, FView v, const double* _
                 }
             }
         }
-        const double sc2 = 1.0 / (sb * sb);
-        ax += sc2 * g2[0];
-        ay += sc2 * g2[1];
-        az += sc2 * g2[2];
-    }
-    if (ok) {
-        const double gm = G * sm[i];
-        fx[i] += gm * ax;
-        fy[i] += gm * ay;
-        fz[i] += gm * az;
+        if (ok) {
+            const double gm = G * sm[i] / (sb * sb);
+            atomicAdd(&fx[i], gm * g2[0]);
+            atomicAdd(&fy[i], gm * g2[1]);
+            atomicAdd(&fz[i], gm * g2[2]);
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void k_fmm_roots(int nsys, int2* __restrict__ fr) {
+// the roots (tree nodes 0 .. nsys-1) as FMM cell pairs; an empty system's
+// root is no FMM cell (id -1) and produces nothing
+__global__ __launch_bounds__(256) void k_fmm_roots(int nsys, const int* __restrict__ fid,
+                                                   int2* __restrict__ fr) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nsys) fr[k] = make_int2(k, k);
+    if (k < nsys) fr[k] = make_int2(fid[k], fid[k]);
 }
 
 // ------------------------------------------------------------------ driver
@@ -1024,13 +1176,18 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
                                                 st.rmax.as<double>());
     }
     k_fmm_qf<P><<<grid_for(nf * kq(P), 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
+    if ((rc = st.rec.ensure(std::max<int64_t>(nf, 1) * sizeof(FRec)))) return rc;
+    k_fmm_records<<<grid_for(std::max<int64_t>(nf, 1), 256), 256, 0, s>>>(
+        (int)nf, v, st.scan.as<uint32_t>(), st.rmax.as<double>(), st.rec.as<FRec>());
     HQ_CHECK_HIP(hipMemsetAsync(st.Ld.p, 0, nf * K * 8, s));
     // 3. dual walk: frontier ping-pong in fa / fb; lists appended
     WalkPar w{theta, tb.R, tb.nd, tb.nd1, tb.noff0, tb.noff1};
     int64_t nfr = t->nsys, nlat = 0, ngen = 0, np2p = 0, nm2p = 0;
-    size_t lat_cap = 0, gen_cap = 0, p2p_cap = 0, m2p_cap = 0;
+    // the lists keep their capacity from one evaluation to the next
+    size_t lat_cap = st.lat.bytes / sizeof(int3), gen_cap = st.gen.bytes / sizeof(int2),
+           p2p_cap = st.p2p.bytes / sizeof(int2), m2p_cap = st.m2p.bytes / sizeof(int2);
     if ((rc = st.fa.ensure(std::max<int64_t>(nfr, 1024) * sizeof(int2)))) return rc;
-    k_fmm_roots<<<grid_for(nfr, 256), 256, 0, s>>>((int)nfr, st.fa.as<int2>());
+    k_fmm_roots<<<grid_for(nfr, 256), 256, 0, s>>>((int)nfr, st.fid.as<int>(), st.fa.as<int2>());
     DevBuf* cur = &st.fa;
     DevBuf* nxt = &st.fb;
     auto grow = [&](DevBuf& b, size_t& cap, int64_t used, int64_t need, size_t esz) -> int {
@@ -1050,26 +1207,31 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         if (nfr > ((int64_t)1 << 31))
             return fail(HQ_ERR_OVERFLOW, "FMM pair frontier of %lld pairs", (long long)nfr);
         const int nblk = grid_for(nfr, WB);
-        if ((rc = st.cnt.ensure((size_t)NACT * (nblk + 1) * 4))) return rc;
+        // [category][block], one end slot per category, and the grand total
+        if ((rc = st.cnt.ensure(((size_t)NACT * (nblk + 1) + 1) * 4)) || (rc = st.code.ensure(nfr * 4)))
+            return rc;
         uint32_t* cnt = st.cnt.as<uint32_t>();
-        k_fmm_walk_count<<<nblk, WB, 0, s>>>(v, st.rmax.as<double>(), w, cur->as<int2>(), nfr, cnt, nblk);
-        for (int q = 0; q < NACT; ++q)
-            if ((rc = scan_excl_buf(cnt + (int64_t)q * (nblk + 1), nblk, t->scanpart, s))) return rc;
-        uint32_t tot[NACT];
-        for (int q = 0; q < NACT; ++q)
-            HQ_CHECK_HIP(hipMemcpyAsync(&tot[q], cnt + (int64_t)q * (nblk + 1) + nblk, 4,
+        k_fmm_walk_count<<<nblk, WB, 0, s>>>(st.rec.as<FRec>(), w, cur->as<int2>(), nfr,
+                                             st.code.as<uint32_t>(), cnt, nblk);
+        // one scan over [category][block]: category q's slice starts at its base
+        if ((rc = scan_excl_buf(cnt, (int64_t)NACT * (nblk + 1), t->scanpart, s))) return rc;
+        uint32_t edge[NACT + 1];
+        for (int q = 0; q <= NACT; ++q)
+            HQ_CHECK_HIP(hipMemcpyAsync(&edge[q], cnt + (int64_t)q * (nblk + 1), 4,
                                         hipMemcpyDeviceToHost, s));
         HQ_CHECK_HIP(hipStreamSynchronize(s));
+        uint32_t tot[NACT];
+        for (int q = 0; q < NACT; ++q) tot[q] = edge[q + 1] - edge[q];
         if ((rc = nxt->ensure(std::max<int64_t>(tot[0], 1) * sizeof(int2))) ||
             (rc = grow(st.lat, lat_cap, nlat, nlat + tot[1] + 1, sizeof(int3))) ||
             (rc = grow(st.gen, gen_cap, ngen, ngen + tot[2] + 1, sizeof(int2))) ||
             (rc = grow(st.p2p, p2p_cap, np2p, np2p + tot[3] + 1, sizeof(int2))) ||
             (rc = grow(st.m2p, m2p_cap, nm2p, nm2p + tot[4] + 1, sizeof(int2))))
             return rc;
-        k_fmm_walk_emit<<<nblk, WB, 0, s>>>(v, st.rmax.as<double>(), w, cur->as<int2>(), nfr, cnt,
-                                            nblk, nxt->as<int2>(), st.lat.as<int3>(), nlat,
-                                            st.gen.as<int2>(), ngen, st.p2p.as<int2>(), np2p,
-                                            st.m2p.as<int2>(), nm2p);
+        k_fmm_walk_emit<<<nblk, WB, 0, s>>>(st.rec.as<FRec>(), cur->as<int2>(), nfr,
+                                            st.code.as<uint32_t>(), cnt, nblk, nxt->as<int2>(),
+                                            st.lat.as<int3>(), nlat, st.gen.as<int2>(), ngen,
+                                            st.p2p.as<int2>(), np2p, st.m2p.as<int2>(), nm2p);
         nlat += tot[1];
         ngen += tot[2];
         np2p += tot[3];
@@ -1085,20 +1247,24 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         const int GT = group_of(P);
         const int ngroups = grid_for(nf, GT);
         const int64_t nbin = (int64_t)ngroups * tb.noff;
-        if ((rc = st.bins.ensure((nbin + 1) * 4)) || (rc = st.binpos.ensure((nbin + 1) * 4)) ||
+        if ((rc = st.bins.ensure((nbin + 1) * 4)) || (rc = st.ghist.ensure((ngroups + 1) * 4)) ||
+            (rc = st.gpos.ensure((ngroups + 1) * 4)) ||
+            (rc = st.gpair.ensure(std::max<int64_t>(nlat, 1) * sizeof(int2))) ||
             (rc = st.lsrc.ensure(std::max<int64_t>(nlat, 1) * 4)) ||
             (rc = st.lcol.ensure(std::max<int64_t>(nlat, 1) * 2)))
             return rc;
-        HQ_CHECK_HIP(hipMemsetAsync(st.bins.p, 0, (nbin + 1) * 4, s));
+        HQ_CHECK_HIP(hipMemsetAsync(st.ghist.p, 0, (ngroups + 1) * 4, s));
         if (nlat)
-            k_fmm_lat_hist<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, tb.noff, GT,
-                                                              st.bins.as<uint32_t>());
-        if ((rc = scan_excl_buf(st.bins.as<uint32_t>(), nbin, t->scanpart, s))) return rc;
-        HQ_CHECK_HIP(hipMemcpyAsync(st.binpos.p, st.bins.p, (nbin + 1) * 4, hipMemcpyDeviceToDevice, s));
+            k_fmm_lat_ghist<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, GT,
+                                                         st.ghist.as<uint32_t>());
+        if ((rc = scan_excl_buf(st.ghist.as<uint32_t>(), ngroups, t->scanpart, s))) return rc;
+        HQ_CHECK_HIP(hipMemcpyAsync(st.gpos.p, st.ghist.p, (ngroups + 1) * 4, hipMemcpyDeviceToDevice, s));
         if (nlat)
-            k_fmm_lat_scatter<<<capped(nlat), 256, 0, s>>>(
-                st.lat.as<int3>(), nlat, tb.noff, GT, st.binpos.as<uint32_t>(), st.lsrc.as<int>(),
-                st.lcol.as<int16_t>());
+            k_fmm_lat_gscatter<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, GT,
+                                                            st.gpos.as<uint32_t>(), st.gpair.as<int2>());
+        k_fmm_lat_lsort<<<ngroups, LSORT_T, 0, s>>>(st.gpair.as<int2>(), st.ghist.as<uint32_t>(),
+                                                    ngroups, tb.noff, st.bins.as<uint32_t>(),
+                                                    st.lsrc.as<int>(), st.lcol.as<int16_t>());
         // enough workgroups to fill the device: split the offsets of a group
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
         if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
@@ -1143,20 +1309,6 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         k_fmm_p2p_scatter<<<capped(np2p), 256, 0, s>>>(st.p2p.as<int2>(), np2p,
                                                              st.ppos.as<uint32_t>(), st.plist.as<int>());
     }
-    // M2P lists by target, the same way
-    if ((rc = st.mhist.ensure((nf + 1) * 4)) || (rc = st.mpos.ensure((nf + 1) * 4)) ||
-        (rc = st.mlist.ensure(std::max<int64_t>(nm2p, 1) * 4)))
-        return rc;
-    HQ_CHECK_HIP(hipMemsetAsync(st.mhist.p, 0, (nf + 1) * 4, s));
-    if (nm2p)
-        k_fmm_p2p_hist<<<capped(nm2p), 256, 0, s>>>(st.m2p.as<int2>(), nm2p,
-                                                          st.mhist.as<uint32_t>());
-    if ((rc = scan_excl_buf(st.mhist.as<uint32_t>(), nf, t->scanpart, s))) return rc;
-    HQ_CHECK_HIP(hipMemcpyAsync(st.mpos.p, st.mhist.p, (nf + 1) * 4, hipMemcpyDeviceToDevice, s));
-    if (nm2p) {
-        k_fmm_p2p_scatter<<<capped(nm2p), 256, 0, s>>>(st.m2p.as<int2>(), nm2p,
-                                                             st.mpos.as<uint32_t>(), st.mlist.as<int>());
-    }
     const size_t b8 = n * sizeof(double);
     if ((rc = t->fx.ensure(b8)) || (rc = t->fy.ensure(b8)) || (rc = t->fz.ensure(b8))) return rc;
     auto leaf = [&](auto kern) {
@@ -1169,11 +1321,10 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     if (mfma) leaf(k_fmm_leaf<P, true>);
     else leaf(k_fmm_leaf<P, false>);
     if (nm2p)
-        k_fmm_m2p<P><<<(int)nf, 64, 0, s>>>((int)nf, v, t->sx.as<double>(), t->sy.as<double>(),
-                                            t->sz.as<double>(), t->sm.as<double>(), tb.mi2.as<int>(),
-                                            tb.rec2.as<int>(), st.Qd.as<double>(),
-                                            st.mhist.as<uint32_t>(), st.mlist.as<int>(), G,
-                                            t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>());
+        k_fmm_m2p<P><<<(int)std::min<int64_t>(nm2p, 1 << 16), 64, 0, s>>>(
+            st.m2p.as<int2>(), nm2p, v, t->sx.as<double>(), t->sy.as<double>(), t->sz.as<double>(),
+            t->sm.as<double>(), tb.mi2.as<int>(), tb.rec2.as<int>(), st.Qd.as<double>(), G,
+            t->fx.as<double>(), t->fy.as<double>(), t->fz.as<double>());
     HQ_CHECK_HIP(hipEventRecord(t->ev[8], s));
     HQ_CHECK_HIP(hipStreamSynchronize(s));
     HQ_CHECK_HIP(hipGetLastError());
