@@ -95,12 +95,18 @@ struct FmmTables {
     int p = 0, K = 0, Kp = 0, K2 = 0, R = 0, nd = 0, noff0 = 0, nd1 = 0, noff1 = 0, noff = 0;
     double theta = -1.0;
     DevBuf mi, mi2, rec2, m2lidx, m2lcoef, m2m, l2l, T;
+    // order 4 builds its operators on the device: the FP32 Taylor
+    // coefficients a_k (|k| <= 2p) of each offset, and for each operator
+    // element (n, m) of the padded Kp x Kq block its coefficient
+    // -(-1)^|m| C(n+m, n) (0 in the padding) and the index of n+m
+    DevBuf A, cf, ix, deg;
 };
 
 struct FmmState {
     FmmTables tab;
     DevBuf flag, scan, fid, fnode, Qd, Qf, Ld, Lf, rmax, lvl;
-    DevBuf rec, code, fa, fb, cnt, lat, gen, p2p, m2p, bins, ghist, gpos, gpair, lsrc, lcol, phist, ppos, plist;
+    DevBuf rec, code, fa, fb, cnt, lat, gen, p2p, m2p, blist, gnb, ghist, gpos, gpair, lsrc, lcol;
+    DevBuf phist, ppos, plist;
     int64_t nf = 0;
     std::vector<int> fls;  // FMM node range per depth
 };
@@ -163,12 +169,14 @@ static int build_tables(FmmTables& tb, int p, double theta) {
     const int nd1 = 2 * R + 2, noff1 = nd1 * nd1 * nd1;
     const int noff = noff0 + 2 * noff1;
     std::vector<MI> ks2(k2.begin(), k2.begin() + K2);
-    std::vector<float> T((size_t)noff * Kp * Kq, 0.0f);
+    const int K2A = (K2 + 3) / 4 * 4;  // a row of A, padded to whole float4s
+    std::vector<float> T((size_t)noff * Kp * Kq, 0.0f), A((size_t)noff * K2A, 0.0f);
     std::vector<double> a;
     auto fill = [&](int o, const double u[3], int cls) {
         // u is the source's offset from the target; the expansion needs r = -u
         const double Ro[3] = {-u[0], -u[1], -u[2]};
         taylor(Ro, ks2, rec, a);
+        for (int k = 0; k < K2; ++k) A[(size_t)o * K2A + k] = (float)a[k];
         float* t = T.data() + (size_t)o * Kp * Kq;
         for (int n = 0; n < K; ++n)
             for (int m = 0; m < K; ++m) {
@@ -199,6 +207,16 @@ static int build_tables(FmmTables& tb, int p, double theta) {
         mi2[4 * i] = k2[i].x, mi2[4 * i + 1] = k2[i].y, mi2[4 * i + 2] = k2[i].z;
         mi2[4 * i + 3] = k2[i].x + k2[i].y + k2[i].z;
     }
+    std::vector<float> cf((size_t)Kp * Kq, 0.0f);
+    std::vector<uint16_t> ix((size_t)Kp * Kq, 0);
+    std::vector<int8_t> deg(128, 0);
+    for (int n = 0; n < K; ++n) {
+        deg[n] = (int8_t)(k1[n].x + k1[n].y + k1[n].z);
+        for (int m = 0; m < K; ++m) {
+            cf[(size_t)n * Kq + m] = (float)mcoef[n * K + m];
+            ix[(size_t)n * Kq + m] = (uint16_t)midx[n * K + m];
+        }
+    }
     int rc;
     auto up = [&](DevBuf& d, const void* h, size_t b) -> int {
         int r;
@@ -211,7 +229,9 @@ static int build_tables(FmmTables& tb, int p, double theta) {
         (rc = up(tb.m2lidx, midx.data(), midx.size() * 4)) ||
         (rc = up(tb.m2lcoef, mcoef.data(), mcoef.size() * 8)) ||
         (rc = up(tb.m2m, m2m.data(), m2m.size() * 8)) || (rc = up(tb.l2l, l2l.data(), l2l.size() * 8)) ||
-        (rc = up(tb.T, T.data(), T.size() * 4)))
+        (rc = up(tb.T, T.data(), T.size() * 4)) || (rc = up(tb.A, A.data(), A.size() * 4)) ||
+        (rc = up(tb.cf, cf.data(), cf.size() * 4)) || (rc = up(tb.ix, ix.data(), ix.size() * 2)) ||
+        (rc = up(tb.deg, deg.data(), deg.size())))
         return rc;
     tb.p = p, tb.K = K, tb.Kp = Kp, tb.K2 = K2, tb.R = R, tb.nd = nd, tb.noff0 = noff0;
     tb.nd1 = nd1, tb.noff1 = noff1, tb.noff = noff, tb.theta = theta;
@@ -343,10 +363,11 @@ __global__ __launch_bounds__(256) void k_fmm_qf(int64_t nf, const double* __rest
                                                 float* __restrict__ Qf) {
     constexpr int K = ncoef(P), Kq = kq(P);
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= nf * Kq) return;
+    if (i >= (nf + 1) * Kq) return;
     const int64_t f = i / Kq;
     const int n = (int)(i % Kq);
-    Qf[i] = n < K ? (float)Qd[f * K + n] : 0.0f;
+    // record nf is all zero: the M2L reads it for the empty columns of a tile
+    Qf[i] = n < K && f < nf ? (float)Qd[f * K + n] : 0.0f;
 }
 
 // ------------------------------------------------------------------ dual walk
@@ -591,23 +612,25 @@ __global__ __launch_bounds__(256) void k_fmm_lat_gscatter(const int3* __restrict
 }
 
 // One workgroup per target group: its pairs [gstart[g], gstart[g+1]) counted
-// by offset in LDS (LSORT_CH offsets per pass), the counts scanned, the row
-// bins[g * noff + o] of bin starts written, and the pairs placed by offset.
+// by offset in LDS (LSORT_CH offsets per pass), the counts scanned, the
+// group's non-empty bins listed in offset order as (offset, first pair) at
+// blist[g * noff + r], r < gnb[g] (a bin ends where the next one starts, the
+// last at gstart[g+1]), and the pairs placed by offset.
 constexpr int LSORT_T = 1024;
 constexpr int LSORT_CH = 32768;
 
 __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restrict__ gp,
                                                            const uint32_t* __restrict__ gstart,
-                                                           int ngroups, int noff,
-                                                           uint32_t* __restrict__ bins,
+                                                           int noff, int2* __restrict__ blist,
+                                                           uint32_t* __restrict__ gnb,
                                                            int* __restrict__ src,
                                                            int16_t* __restrict__ col) {
     __shared__ uint32_t h[LSORT_CH];
     __shared__ uint32_t ws[LSORT_T / 64];
     const int g = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = gstart[g], s1 = gstart[g + 1];
-    uint32_t* row = bins + (int64_t)g * noff;
-    uint32_t done = 0;  // pairs with offsets below this pass's
+    int2* row = blist + (int64_t)g * noff;
+    uint32_t done = 0, listed = 0;  // pairs and non-empty bins of the earlier passes
     for (int c0 = 0; c0 < noff; c0 += LSORT_CH) {
         const int nc = min(LSORT_CH, noff - c0);
         for (int o = tid; o < nc; o += LSORT_T) h[o] = 0;
@@ -617,17 +640,22 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
             if (o >= 0 && o < nc) atomicAdd(&h[o], 1u);
         }
         __syncthreads();
-        // exclusive scan of h[0, nc), each thread over a contiguous run
+        // exclusive scans of the counts and of the non-empty flags of
+        // h[0, nc), each thread over a contiguous run
         const int per = (nc + LSORT_T - 1) / LSORT_T;
         const int oa = min(nc, tid * per), ob = min(nc, oa + per);
-        uint32_t sum = 0;
-        for (int o = oa; o < ob; ++o) sum += h[o];
-        uint32_t tot;
+        uint32_t sum = 0, ne = 0;
+        for (int o = oa; o < ob; ++o) {
+            sum += h[o];
+            ne += h[o] ? 1u : 0u;
+        }
+        uint32_t tot, ntot;
         uint32_t run = block_excl_scan(sum, ws, tot) + done;
+        uint32_t rank = block_excl_scan(ne, ws, ntot) + listed;
         for (int o = oa; o < ob; ++o) {
             const uint32_t c = h[o];
             h[o] = run;
-            row[c0 + o] = s0 + run;
+            if (c) row[rank++] = make_int2(c0 + o, (int)(s0 + run));
             run += c;
         }
         __syncthreads();
@@ -641,9 +669,10 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
             }
         }
         done += tot;
+        listed += ntot;
         __syncthreads();
     }
-    if (g == ngroups - 1 && tid == 0) bins[(int64_t)ngroups * noff] = s1;
+    if (tid == 0) gnb[g] = listed;
 }
 
 __global__ __launch_bounds__(256) void k_fmm_p2p_hist(const int2* __restrict__ pr, int64_t n,
@@ -683,13 +712,330 @@ __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict_
 // once per bin and no other wavefront writes that copy; the copies are summed
 // in wavefront order at the end.  LDS float atomics, which the shared copy of
 // order 6 uses, cost about 135 cycles per instruction.
+// Adds one tile's products into the group's FP32 locals in LDS: column li of
+// the tile is target c (c < 0: an empty column), this lane's rows
+// 16 rt + 4 lk .. + 3.  A private copy takes a plain 16-byte read-add-write
+// (rows at or past Kq are operator padding); the shared copy atomic adds.
+template <bool PRIV, int RT, int Kq, int K, int LS>
+__device__ __forceinline__ void m2l_accumulate(float* mine, int c, int lk, const floatx4* d) {
+    if (c < 0) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int row = 16 * rt + 4 * lk;
+        if (PRIV) {
+            if (row < Kq) {
+                float4* a = reinterpret_cast<float4*>(mine + c * LS + row);
+                float4 v = *a;
+                v.x += d[rt][0];
+                v.y += d[rt][1];
+                v.z += d[rt][2];
+                v.w += d[rt][3];
+                *a = v;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (row + r < K) atomicAdd(&mine[c * LS + row + r], d[rt][r]);
+        }
+    }
+}
+
+// The same for a private copy in two halves: the reads are issued first, the
+// adds and writes later, so that the read latency overlaps other work.
+template <int RT, int Kq, int LS>
+__device__ __forceinline__ void m2l_acc_read(const float* mine, int c, int lk, float4* v) {
+    if (c < 0) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int row = 16 * rt + 4 * lk;
+        if (row < Kq) v[rt] = *reinterpret_cast<const float4*>(mine + c * LS + row);
+    }
+}
+
+template <int RT, int Kq, int LS>
+__device__ __forceinline__ void m2l_acc_write(float* mine, int c, int lk, float4* v,
+                                              const floatx4* d) {
+    if (c < 0) return;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int row = 16 * rt + 4 * lk;
+        if (row < Kq) {
+            v[rt].x += d[rt][0];
+            v[rt].y += d[rt][1];
+            v[rt].z += d[rt][2];
+            v[rt].w += d[rt][3];
+            *reinterpret_cast<float4*>(mine + c * LS + row) = v[rt];
+        }
+    }
+}
+
+// Order-4 lattice M2L on the matrix cores.  One workgroup of M2L4_WAVES
+// wavefronts per group of 128 target cubes (with nsplit > 1, per share of
+// the group's bins); wavefront w takes a contiguous share of the bins (in
+// offset order) and cuts each into tiles of 16 pairs, the columns.  A tile's
+// product D = T_o Q (48 x 16: the operator padded to 16-row tiles, the
+// contraction to 36) runs on v_mfma_f32_16x16x4f32 and is added into the
+// wavefront's private FP32 copy of the group's locals in LDS (a plain 16-byte
+// read-add-write: a target appears once per bin); the copies are summed in
+// order at the end.  A bin's operator is built in registers from the FP32
+// Taylor coefficients a_k of its offset, staged in LDS:
+// T[n][m] = c[n][m] a_{n+m} 2^e, with e the class scaling (rows by 2^(|n|+1)
+// for a coarser target, columns by 2^|m| for a finer one).  The loads are
+// software pipelined over the wavefront's stream of tiles: the step of tile
+// j fetches the moments, columns and Taylor coefficients of tile j+2 and the
+// source ids of tile j+4, so every load has two steps to arrive.  Every step
+// issues the same loads (past the end of the stream they read valid dummy
+// records), which keeps the wait counts the same from step to step.
+constexpr int M2L4_WAVES = 4;
+constexpr int M2L4_GT = 128;
+
+struct M2lTile {   // one tile of a wavefront's stream (wave-uniform)
+    int o;         // offset id; -1 past the end of the stream
+    uint32_t e0, b1;  // its pairs: [e0, min(e0 + 16, b1))
+    int first;     // the first tile of its bin
+};
+
+struct M2lStream {  // the wavefront's bins [k, k1) of a group's list, tile by tile
+    const int2* list;
+    int k, k1, nbg;
+    uint32_t s1, b0, b1;
+    int o, t, nt;
+    __device__ __forceinline__ M2lTile next() {
+        if (t >= nt) {
+            if (k >= k1) return M2lTile{-1, 0u, 0u, 0};
+            // the same for every lane: kept in scalar registers
+            o = __builtin_amdgcn_readfirstlane(list[k].x);
+            b0 = (uint32_t)__builtin_amdgcn_readfirstlane(list[k].y);
+            b1 = k + 1 < nbg ? (uint32_t)__builtin_amdgcn_readfirstlane(list[k + 1].y) : s1;
+            nt = (int)((b1 - b0 + 15) / 16);
+            t = 0;
+            ++k;
+        }
+        const M2lTile r{o, b0 + 16u * (uint32_t)t, b1, t == 0};
+        ++t;
+        return r;
+    }
+};
+
+template <int P>
+__global__ __launch_bounds__(64 * M2L4_WAVES) void k_fmm_m2l4(
+    int64_t nf, int noff, int noff0, int noff1, int nsplit, const int2* __restrict__ blist,
+    const uint32_t* __restrict__ gnb, const uint32_t* __restrict__ gstart,
+    const int* __restrict__ src, const int16_t* __restrict__ col, const float* __restrict__ A,
+    const float* __restrict__ cf, const uint16_t* __restrict__ ix, const int8_t* __restrict__ deg,
+    const float* __restrict__ Qf, float* __restrict__ Lf) {
+    constexpr int K = ncoef(P), K2 = ncoef(2 * P), Kp = kpad(P), Kq = kq(P);
+    constexpr int RT = Kp / 16, KT = Kq / 4, NW = M2L4_WAVES, NT = 64 * NW, GT = M2L4_GT, LS = Kq;
+    constexpr int K2V = (K2 + 3) / 4;  // float4 slots of one offset's Taylor coefficients
+    static_assert(K2V <= 64, "one float4 of Taylor coefficients per lane");
+    __shared__ float4 acc4[NW * GT * LS / 4];
+    __shared__ float4 stage4[NW][K2V];
+    float* acc = reinterpret_cast<float*>(acc4);
+    const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, lk = lane >> 4;
+    for (int i = tid; i < NW * GT * LS / 4; i += NT) acc4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // this lane's operator elements (n, m) = (16 rt + li, lk KT + kt):
+    // coefficients, indices of n+m (four 8-bit indices per word), degrees
+    float cfl[RT][KT];
+    uint32_t ixw[(RT * KT + 3) / 4];
+    int rdeg[RT], cdeg[KT];
+#pragma unroll
+    for (int w = 0; w < (RT * KT + 3) / 4; ++w) ixw[w] = 0u;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        rdeg[rt] = deg[16 * rt + li] + 1;
+#pragma unroll
+        for (int kt = 0; kt < KT; ++kt) {
+            const int n = 16 * rt + li, m = lk * KT + kt, q = rt * KT + kt;
+            cfl[rt][kt] = cf[n * Kq + m];
+            ixw[q / 4] |= (uint32_t)ix[n * Kq + m] << (8 * (q % 4));
+        }
+    }
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) cdeg[kt] = deg[lk * KT + kt];
+    __syncthreads();
+    float* __restrict__ mine = acc + wv * GT * LS;
+    float4* stg4 = stage4[wv];
+    const float* stg = reinterpret_cast<const float*>(stage4[wv]);
+    // this wavefront's bins: its share of the workgroup's share of the group
+    const int nbg = __builtin_amdgcn_readfirstlane((int)gnb[g]);
+    const int64_t kb = (int64_t)nbg * sp / nsplit, ke = (int64_t)nbg * (sp + 1) / nsplit;
+    M2lStream st;
+    st.list = blist + (int64_t)g * noff;
+    st.k = (int)(kb + (ke - kb) * wv / NW);
+    st.k1 = (int)(kb + (ke - kb) * (wv + 1) / NW);
+    st.nbg = nbg;
+    st.s1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)gstart[g + 1]);
+    st.b0 = st.b1 = 0u;
+    st.o = 0;
+    st.t = st.nt = 0;
+    M2lTile t0 = st.next(), t1 = st.next(), t2 = st.next(), t3 = st.next(), t4 = st.next();
+    // a lane's pair in tile t: index, or -1 (past the bin or the stream)
+    auto pair_of = [&](const M2lTile& t) -> int64_t {
+        const uint32_t e = t.e0 + (uint32_t)li;
+        return (t.o >= 0 && e < t.b1) ? (int64_t)e : -1;
+    };
+    // loads of a step: moments, column and Taylor coefficients of a tile (its
+    // source id fetched two steps before), the source id of a later tile.
+    // They go to rings of three slots; the loop is unrolled three times so
+    // that a slot is never copied while its load is in flight.
+    float bq0[KT], bq1[KT], bq2[KT];
+    float4 ak0, ak1, ak2;
+    int cl0, cl1, cl2, sr0, sr1, sr2;
+    // ring slot R of a ring of three named registers (R known at compile time)
+    auto slot = [](auto RS, auto& x0, auto& x1, auto& x2) -> auto& {
+        constexpr int R = decltype(RS)::value % 3;
+        if constexpr (R == 0) return x0;
+        else if constexpr (R == 1) return x1;
+        else return x2;
+    };
+    auto load_q = [&](const M2lTile& t, int sid, float (&q3)[KT], int& c) {
+        const int64_t e = pair_of(t);
+        const float* q = Qf + (e >= 0 ? (int64_t)sid : nf) * Kq + lk * KT;
+#pragma unroll
+        for (int kt = 0; kt < KT; ++kt) q3[kt] = q[kt];
+        c = col[e >= 0 ? e : 0];
+    };
+    auto load_a = [&](const M2lTile& t) -> float4 {
+        return reinterpret_cast<const float4*>(A + (int64_t)(t.o >= 0 ? t.o : 0) * (4 * K2V))[lane < K2V ? lane : 0];
+    };
+    auto load_s = [&](const M2lTile& t) -> int {
+        const int64_t e = pair_of(t);
+        return src[e >= 0 ? e : 0];
+    };
+    // the operator of tile t from its Taylor coefficients, staged in LDS: the
+    // elements are read first (build_read), then scaled in place (build_scale)
+    const auto build_read = [&](float (&avx)[RT][KT]) {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) {
+                const int q = rt * KT + kt;
+                avx[rt][kt] = stg[(ixw[q / 4] >> (8 * (q % 4))) & 0xFFu];
+            }
+    };
+    const auto build_scale = [&](const M2lTile& t, float (&avx)[RT][KT]) {
+        if (t.o < noff0) {
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt) avx[rt][kt] *= cfl[rt][kt];
+        } else {
+            const int f1 = t.o < noff0 + noff1, f2 = 1 - f1;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt)
+                    avx[rt][kt] = ldexpf(cfl[rt][kt] * avx[rt][kt], f1 * rdeg[rt] + f2 * cdeg[kt]);
+        }
+    };
+    const auto stage = [&](const float4& a) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < K2V) stg4[lane] = a;
+        __builtin_amdgcn_wave_barrier();
+    };
+    float av[RT][KT];
+    {
+        // in the order of two steps, so that the loop's first wait is the
+        // same as its later ones
+        const int s0 = load_s(t0), s1 = load_s(t1);
+        __builtin_amdgcn_sched_barrier(0);
+        load_q(t0, s0, bq0, cl0);
+        ak0 = load_a(t0);
+        sr2 = load_s(t2);
+        __builtin_amdgcn_sched_barrier(0);
+        load_q(t1, s1, bq1, cl1);
+        ak1 = load_a(t1);
+        sr0 = load_s(t3);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    floatx4 dp[RT];
+    int cp = -1;
+    bool have_prev = false;
+    // step for tile t0 in ring slot R (its loads made two steps before):
+    // fetch tile t2's moments, column and Taylor coefficients into slot R + 2
+    // (its source id in slot R + 2 of the source ring) and the source id of
+    // tile t4 into slot R + 1 (mod 3)
+    auto step = [&](auto RS) {
+        using C0 = std::integral_constant<int, decltype(RS)::value>;
+        using C1 = std::integral_constant<int, decltype(RS)::value + 1>;
+        using C2 = std::integral_constant<int, decltype(RS)::value + 2>;
+        float(&bqR)[KT] = slot(C0{}, bq0, bq1, bq2);
+        load_q(t2, slot(C2{}, sr0, sr1, sr2), slot(C2{}, bq0, bq1, bq2), slot(C2{}, cl0, cl1, cl2));
+        slot(C2{}, ak0, ak1, ak2) = load_a(t2);
+        slot(C1{}, sr0, sr1, sr2) = load_s(t4);
+        __builtin_amdgcn_sched_barrier(0);
+        if (t0.first) {
+            stage(slot(C0{}, ak0, ak1, ak2));
+            build_read(av);
+            build_scale(t0, av);
+        }
+        float4 pv[RT];
+        if (have_prev) m2l_acc_read<RT, Kq, LS>(mine, cp, lk, pv);
+        floatx4 d[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) d[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < KT / 2; ++kt)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+                d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bqR[kt], d[rt], 0, 0, 0);
+        // the previous tile's sums go into the locals while the second half
+        // of this tile's products runs
+        __builtin_amdgcn_sched_barrier(0);
+        if (have_prev) m2l_acc_write<RT, Kq, LS>(mine, cp, lk, pv, dp);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int kt = KT / 2; kt < KT; ++kt)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+                d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bqR[kt], d[rt], 0, 0, 0);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) dp[rt] = d[rt];
+        cp = pair_of(t0) >= 0 ? slot(C0{}, cl0, cl1, cl2) : -1;
+        have_prev = true;
+        t0 = t1;
+        t1 = t2;
+        t2 = t3;
+        t3 = t4;
+        t4 = st.next();
+    };
+    // three steps per pass, checked at the end: up to two steps past the end
+    // of the stream run on the zero record and add nothing
+    if (t0.o >= 0) do {
+        step(std::integral_constant<int, 0>{});
+        step(std::integral_constant<int, 1>{});
+        step(std::integral_constant<int, 2>{});
+    } while (t0.o >= 0);
+    if (have_prev) {
+        float4 pv[RT];
+        m2l_acc_read<RT, Kq, LS>(mine, cp, lk, pv);
+        m2l_acc_write<RT, Kq, LS>(mine, cp, lk, pv, dp);
+    }
+    __syncthreads();
+    const int64_t f0 = (int64_t)g * GT;
+    for (int i = tid; i < GT * K; i += NT) {
+        const int tc = i / K, k = i % K;
+        const int64_t f = f0 + tc;
+        if (f >= nf) break;
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) v += acc[(w * GT + tc) * LS + k];
+        float* dst = &Lf[f * Kp + k];
+        if (nsplit == 1) *dst = v;
+        else if (v != 0.0f) atomicAdd(dst, v);
+    }
+}
+
 // Operand layout (wave64): A[i][k] from lane i + 16k, B[k][j] from lane
 // j + 16k, D[4q + r][j] in lane j + 16q, element r.  The contraction index of
 // step kt in lane group lk is lk * KT + kt, so that each lane reads KT
 // consecutive moments of its source (wide loads).
 template <int P>
 __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
-    int64_t nf, int noff, int nsplit, const uint32_t* __restrict__ bstart,
+    int64_t nf, int noff, int nsplit, const int2* __restrict__ blist,
+    const uint32_t* __restrict__ gnb, const uint32_t* __restrict__ gstart,
     const int* __restrict__ src, const int16_t* __restrict__ col, const float* __restrict__ T,
     const float* __restrict__ Qf, float* __restrict__ Lf) {
     constexpr int K = ncoef(P), Kp = kpad(P), Kq = kq(P), RT = Kp / 16, KT = Kq / 4;
@@ -697,100 +1043,99 @@ __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
     constexpr bool PRIV = m2l_private(P);
     constexpr int NC = PRIV ? NW : 1;      // copies of the locals
     constexpr int LS = PRIV ? Kq : K;      // floats per target in a copy
+    constexpr int TC = 8;                  // tiles of a bin whose sources are loaded together
     __shared__ float4 acc4[(NC * GT * LS + 3) / 4];
     float* acc = reinterpret_cast<float*>(acc4);
-    // block (g, sp): target group g, offsets [o_lo, o_hi); with several
-    // splits per group the partial sums meet in Lf by atomic adds
+    // block (g, sp): target group g, share sp of its bins; with several
+    // shares per group the partial sums meet in Lf by atomic adds
     const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
-    const int per = (noff + nsplit - 1) / nsplit;
-    const int o_lo = sp * per, o_hi = min(noff, o_lo + per);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     for (int i = tid; i < (NC * GT * LS + 3) / 4; i += NT) acc4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
     float* __restrict__ mine = acc + (PRIV ? wv * GT * LS : 0);
-    const uint32_t* bsg = bstart + (int64_t)g * noff;
+    // this workgroup's share of the group's non-empty bins; bin k of it
+    // belongs to wavefront k % NW
+    const int2* list = blist + (int64_t)g * noff;
+    const int nbg = (int)gnb[g];
+    const uint32_t s1 = gstart[g + 1];
+    const int kb = (int)((int64_t)nbg * sp / nsplit), ke = (int)((int64_t)nbg * (sp + 1) / nsplit);
     float av[RT][KT];
-    int nbin = 0;  // running count of non-empty bins: bin k belongs to wavefront k % NW
-    // the bin bounds are read 64 offsets at a time, one per lane; the
-    // non-empty ones are taken in order from the ballot
-    for (int ob = o_lo; ob < o_hi; ob += 64) {
-        const int ol = ob + lane;
-        const uint32_t lb0 = ol < o_hi ? bsg[ol] : 0u, lb1 = ol < o_hi ? bsg[ol + 1] : 0u;
-        uint64_t live = __ballot(lb0 != lb1);
-        while (live) {
-            const int l = __ffsll((unsigned long long)live) - 1;
-            live &= live - 1;
-            if (nbin++ % NW != wv) continue;
-            const int o = ob + l;
-            const uint32_t b0 = __shfl(lb0, l, 64), b1 = __shfl(lb1, l, 64);
-            const int nt = (int)((b1 - b0 + 15) / 16);
-            const float* to = T + (int64_t)o * Kp * Kq;
-            // the operator and the first tile's sources are fetched together; each
-            // later tile's sources are fetched while the previous tile computes
-            float bn[KT];
-            int cn;
-            bool okn;
+    for (int k = kb + wv; k < ke; k += NW) {
+        const int o = list[k].x;
+        const uint32_t b0 = (uint32_t)list[k].y, b1 = k + 1 < nbg ? (uint32_t)list[k + 1].y : s1;
+        const int nt = (int)((b1 - b0 + 15) / 16);
+        const float* to = T + (int64_t)o * Kp * Kq;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + lk * KT + kt];
+        for (int t0 = 0; t0 < nt; t0 += TC) {
+            // this lane's source and column in each tile of the chunk,
+            // loaded together; an empty column reads the zero record nf
+            int sr[TC], cr[TC];
+#pragma unroll
+            for (int u = 0; u < TC; ++u) {
+                const uint32_t e = b0 + (uint32_t)((t0 + u) * 16 + li);
+                const bool okc = e < b1;
+                const uint32_t ee = okc ? e : b0;
+                const int sv = src[ee], cv = col[ee];
+                sr[u] = okc ? sv : (int)nf;
+                cr[u] = okc ? cv : -1;
+            }
+            const int ntc = min(TC, nt - t0);
+            // tile u's moments are fetched during tile u - 1, and tile
+            // u - 1's sums are added into the locals while tile u's
+            // matrix products run
+            float bv[KT];
             {
-                const uint32_t e = b0 + (uint32_t)li;
-                okn = e < b1;
-                const int s = okn ? src[e] : 0;
-                cn = okn ? col[e] : 0;
-                // this lane's KT consecutive moments (wide loads); s = 0 for an
-                // empty column reads a valid record
-                const float* q = Qf + (int64_t)s * Kq + lk * KT;
-    #pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
-    #pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
+                const float* q = Qf + (int64_t)sr[0] * Kq + lk * KT;
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt) bv[kt] = q[kt];
             }
-    #pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-    #pragma unroll
-                for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + lk * KT + kt];
-            for (int t = 0; t < nt; ++t) {
-                float bv[KT];
-    #pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bv[kt] = bn[kt];
-                const bool ok = okn;
-                const int c = cn;
-                if (t + 1 < nt) {
-                    const uint32_t e = b0 + (uint32_t)((t + 1) * 16) + li;
-                    okn = e < b1;
-                    const int s = okn ? src[e] : 0;
-                    cn = okn ? col[e] : 0;
-                    const float* q = Qf + (int64_t)s * Kq + lk * KT;
-    #pragma unroll
+            floatx4 dp[RT];
+            int cp = -1;
+#pragma unroll
+            for (int u = 0; u < TC; ++u) {
+                if (u >= ntc) break;
+                float bn[KT];
+                {
+                    const float* q = Qf + (int64_t)sr[u + 1 < TC ? u + 1 : u] * Kq + lk * KT;
+#pragma unroll
                     for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
-    #pragma unroll
-                    for (int kt = 0; kt < KT; ++kt) bn[kt] = okn ? bn[kt] : 0.0f;
                 }
-    #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) {
-                    floatx4 d = {0.f, 0.f, 0.f, 0.f};
-    #pragma unroll
-                    for (int kt = 0; kt < KT; ++kt)
-                        d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d, 0, 0, 0);
-                    const int row = 16 * rt + 4 * lk;  // rows row .. row + 3 of column li
-                    if (!ok) continue;
-                    if (PRIV) {
-                        // rows at or past Kq are padding (zero operator rows)
-                        if (row < Kq) {
-                            float4* a = reinterpret_cast<float4*>(mine + c * LS + row);
-                            float4 v = *a;
-                            v.x += d[0];
-                            v.y += d[1];
-                            v.z += d[2];
-                            v.w += d[3];
-                            *a = v;
-                        }
-                    } else {
-    #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (row + r < K) atomicAdd(&mine[c * LS + row + r], d[r]);
-                    }
+                // the loads above are issued before the products start
+                __builtin_amdgcn_sched_barrier(0);
+                float4 pv[RT];
+                if (PRIV && u > 0) m2l_acc_read<RT, Kq, LS>(mine, cp, lk, pv);
+                floatx4 d[RT];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) d[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kt = 0; kt < KT / 2; ++kt)
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt)
+                        d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d[rt], 0, 0, 0);
+                // the previous tile's sums go into the locals while the
+                // second half of this tile's products runs
+                __builtin_amdgcn_sched_barrier(0);
+                if (u > 0) {
+                    if (PRIV) m2l_acc_write<RT, Kq, LS>(mine, cp, lk, pv, dp);
+                    else m2l_accumulate<PRIV, RT, Kq, K, LS>(mine, cp, lk, dp);
                 }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int kt = KT / 2; kt < KT; ++kt)
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt)
+                        d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d[rt], 0, 0, 0);
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) dp[rt] = d[rt];
+                cp = cr[u];
+#pragma unroll
+                for (int kt = 0; kt < KT; ++kt) bv[kt] = bn[kt];
             }
+            m2l_accumulate<PRIV, RT, Kq, K, LS>(mine, cp, lk, dp);
         }
     }
     __syncthreads();
@@ -1162,7 +1507,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     v.cz = t->ncz.as<double>();
     v.sg = t->geom.as<double>() + G_SYS;
     // 2. upward
-    if ((rc = st.Qd.ensure(nf * K * 8)) || (rc = st.Qf.ensure(nf * kq(P) * 4)) ||
+    if ((rc = st.Qd.ensure(nf * K * 8)) || (rc = st.Qf.ensure((nf + 1) * kq(P) * 4)) ||
         (rc = st.Ld.ensure(nf * K * 8)) || (rc = st.Lf.ensure(nf * Kp * 4)) ||
         (rc = st.rmax.ensure(nf * 8)))
         return rc;
@@ -1175,7 +1520,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
             k_fmm_m2m<P><<<f1 - f0, 64, 0, s>>>(f0, f1, v, tb.m2m.as<double>(), st.Qd.as<double>(),
                                                 st.rmax.as<double>());
     }
-    k_fmm_qf<P><<<grid_for(nf * kq(P), 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
+    k_fmm_qf<P><<<grid_for((nf + 1) * kq(P), 256), 256, 0, s>>>(nf, st.Qd.as<double>(), st.Qf.as<float>());
     if ((rc = st.rec.ensure(std::max<int64_t>(nf, 1) * sizeof(FRec)))) return rc;
     k_fmm_records<<<grid_for(std::max<int64_t>(nf, 1), 256), 256, 0, s>>>(
         (int)nf, v, st.scan.as<uint32_t>(), st.rmax.as<double>(), st.rec.as<FRec>());
@@ -1245,9 +1590,11 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     // 4. M2L
     if (mfma) {
         const int GT = group_of(P);
+        static_assert(P != 4 || group_of(4) == M2L4_GT, "order-4 groups");
         const int ngroups = grid_for(nf, GT);
         const int64_t nbin = (int64_t)ngroups * tb.noff;
-        if ((rc = st.bins.ensure((nbin + 1) * 4)) || (rc = st.ghist.ensure((ngroups + 1) * 4)) ||
+        if ((rc = st.blist.ensure(std::max<int64_t>(nbin, 1) * sizeof(int2))) ||
+            (rc = st.gnb.ensure((ngroups + 1) * 4)) || (rc = st.ghist.ensure((ngroups + 1) * 4)) ||
             (rc = st.gpos.ensure((ngroups + 1) * 4)) ||
             (rc = st.gpair.ensure(std::max<int64_t>(nlat, 1) * sizeof(int2))) ||
             (rc = st.lsrc.ensure(std::max<int64_t>(nlat, 1) * 4)) ||
@@ -1263,15 +1610,23 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
             k_fmm_lat_gscatter<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, GT,
                                                             st.gpos.as<uint32_t>(), st.gpair.as<int2>());
         k_fmm_lat_lsort<<<ngroups, LSORT_T, 0, s>>>(st.gpair.as<int2>(), st.ghist.as<uint32_t>(),
-                                                    ngroups, tb.noff, st.bins.as<uint32_t>(),
+                                                    tb.noff, st.blist.as<int2>(), st.gnb.as<uint32_t>(),
                                                     st.lsrc.as<int>(), st.lcol.as<int16_t>());
-        // enough workgroups to fill the device: split the offsets of a group
+        // enough workgroups to fill the device: split the bins of a group
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
         if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
         HQ_CHECK_HIP(hipEventRecord(t->ev[6], s));
-        k_fmm_m2l_mfma<P><<<ngroups * nsplit, 64 * M2L_WAVES, 0, s>>>(
-            nf, tb.noff, nsplit, st.bins.as<uint32_t>(), st.lsrc.as<int>(), st.lcol.as<int16_t>(),
-            tb.T.as<float>(), st.Qf.as<float>(), st.Lf.as<float>());
+        if constexpr (P == 4)
+            k_fmm_m2l4<P><<<ngroups * nsplit, 64 * M2L4_WAVES, 0, s>>>(
+                nf, tb.noff, tb.noff0, tb.noff1, nsplit, st.blist.as<int2>(), st.gnb.as<uint32_t>(),
+                st.ghist.as<uint32_t>(), st.lsrc.as<int>(), st.lcol.as<int16_t>(), tb.A.as<float>(),
+                tb.cf.as<float>(), tb.ix.as<uint16_t>(), tb.deg.as<int8_t>(), st.Qf.as<float>(),
+                st.Lf.as<float>());
+        else
+            k_fmm_m2l_mfma<P><<<ngroups * nsplit, 64 * M2L_WAVES, 0, s>>>(
+                nf, tb.noff, nsplit, st.blist.as<int2>(), st.gnb.as<uint32_t>(), st.ghist.as<uint32_t>(),
+                st.lsrc.as<int>(), st.lcol.as<int16_t>(), tb.T.as<float>(), st.Qf.as<float>(),
+                st.Lf.as<float>());
         HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
         if (ngen)
             k_fmm_m2l_fp64<P><<<(int)std::min<int64_t>(ngen, 1 << 18), 64, 0, s>>>(
