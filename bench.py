@@ -275,7 +275,9 @@ def run_fmm(args):
     K0 = (head["order"] + 1) * (head["order"] + 2) * (head["order"] + 3) // 6
     workload = f"fmm_{head['profile']}_n{N}_p{head['order']}_theta{args.theta}"
     traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
-    tf = os.path.join(REPO, "profiles", f"traffic_k_fmm_m2l_mfma_{head['profile']}_n{N}_p{head['order']}.json")
+    # the matrix-core M2L kernel of this order
+    kname = "k_fmm_m2l4" if head["order"] == 4 else "k_fmm_m2l_mfma"
+    tf = os.path.join(REPO, "profiles", f"traffic_{kname}_{head['profile']}_n{N}_p{head['order']}.json")
     if os.path.exists(tf) and head["m2l"] == "mfma":
         rec = json.load(open(tf))
         if rec.get("workload") == workload:
@@ -288,7 +290,7 @@ def run_fmm(args):
         "data": "synthetic (seeded uniform cube and Plummer sphere)",
         "config": {"workload": workload, "N": N, "theta": args.theta, "leaf": 32},
         "rms_rel_err": head["rms_rel_err"], "max_rel_err": head["max_rel_err"],
-        "roofline": {"bound": "mfma", "kernel": f"k_fmm_m2l_mfma<{head['order']}>",
+        "roofline": {"bound": "mfma", "kernel": f"{kname}<{head['order']}>",
                      "achieved": head["m2l_tflops"], "peak": peak, "unit": "TFLOP/s",
                      "frac": head["m2l_tflops"] / peak, "traffic": traffic,
                      "flops_per_launch": 2.0 * K0 * K0 * head["same_level_m2l_pairs"],
