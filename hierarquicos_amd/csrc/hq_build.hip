@@ -11,8 +11,9 @@
 //                    (c + sign*h/2, x >= cx, y < cy, z < cz), so boundary
 //                    cases fall exactly as in the reference
 //   3. radix sort    stable LSD radix sort (8-bit digits, wavefront ballot
-//                    ranking) of the 63-bit high key word; equal high words
-//                    are then ordered by the 12-bit low word (levels 22-25)
+//                    ranking) of the 63-bit high key word above its lowest
+//                    16 bits; runs equal there are then ordered by the rest
+//                    of the high word and the 12-bit low word (levels 22-25)
 //   4. k_node_*      nodes from the sorted keys: with D_i the number of
 //                    common levels of particles i and i+1, particle i starts
 //                    the nodes of depths D_{i-1}+1 .. max(D_{i-1},D_i)+1 (the
@@ -20,7 +21,8 @@
 //                    in level order; children of a node are contiguous.
 //   5. k_parents,    parent and "next" (skip) pointers for stackless walks,
 //      k_next        one launch each (the skip pointer by an ancestor walk)
-//   6. k_upward      level-synchronous M2M from depth 25 to 0, children summed
+//   6. k_upward      level-synchronous M2M from depth 25 to 0, a wavefront per
+//                    64 nodes staging their children in LDS; children summed
 //                    in the reference's child order (octant index 1..8), with
 //                    contraction off, so mass/COM/quad/oct match bit for bit
 #include <math.h>
@@ -363,18 +365,25 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     }
 }
 
-// Runs of equal high words (particles sharing levels 1..21) are ordered by
-// the low word (levels 22..25), stably, one run per thread.  Such runs are
-// rare and short; a longer one flags the build, which then sorts by the full
-// key instead (full_key_sort).
+// The radix sort of the tree's keys orders them by the high word without its
+// lowest TIE_SKIP_BITS bits (levels 1..16 and a third of 17: two passes fewer).
+// Runs of keys equal in those bits (particles sharing a cube 2^-16 of the root
+// wide, rare and short) are then ordered by the rest of the high word and the
+// low word (levels 22..25), stably, one run per thread: the order of a full
+// sort by (high word, low word).  A run longer than TIE_RUN_MAX flags the
+// build, which then sorts by the full key instead (full_key_sort).
+constexpr int TIE_SKIP_BITS = 16;
 constexpr int64_t TIE_RUN_MAX = 64;
-__global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __restrict__ klo,
+__global__ void k_tie_fix(uint64_t* __restrict__ khi, const uint32_t* __restrict__ klo,
                           int* __restrict__ idx, const int* __restrict__ sysid, int64_t n,
                           int64_t* __restrict__ status) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i + 1 >= n) return;
+    // the sorted bits of a run's keys are equal, so these tests read the same
+    // whatever the run's own thread has moved so far
     auto same = [&](int64_t a, int64_t b) {
-        return khi[a] == khi[b] && (!sysid || sysid[idx[a]] == sysid[idx[b]]);
+        return (khi[a] >> TIE_SKIP_BITS) == (khi[b] >> TIE_SKIP_BITS) &&
+               (!sysid || sysid[idx[a]] == sysid[idx[b]]);
     };
     if (!same(i, i + 1)) return;
     if (i > 0 && same(i - 1, i)) return;
@@ -385,13 +394,16 @@ __global__ void k_tie_fix(const uint64_t* __restrict__ khi, const uint32_t* __re
         return;
     }
     for (int64_t a = i + 1; a <= j; ++a) {
-        int v = idx[a];
-        uint32_t kv = klo[v];
+        const int v = idx[a];
+        const uint64_t kh = khi[a];
+        const uint32_t kl = klo[v];
         int64_t b = a - 1;
-        while (b >= i && klo[idx[b]] > kv) {
+        while (b >= i && (khi[b] > kh || (khi[b] == kh && klo[idx[b]] > kl))) {
+            khi[b + 1] = khi[b];
             idx[b + 1] = idx[b];
             --b;
         }
+        khi[b + 1] = kh;
         idx[b + 1] = v;
     }
 }
@@ -639,13 +651,265 @@ struct UpArgs {
     const int* nfirst;
     const int* ncstart;
     const uint8_t* nmask;
-    const double *sx, *sy, *sz, *sm;
+    const uint8_t* ntwig;
+    const double4* sbody;  // sorted bodies as (x, y, z, m)
     double *nmass, *ncomx, *ncomy, *ncomz, *nquad, *noct;
     int* ncount;
-    int64_t nn;
+    int64_t nn;  // stride of the moment arrays (the capacity)
 };
 
-// Moments of node k at `depth` from its children (or its particle).
+// Children of one round of parents, staged in LDS by the wavefront: every
+// child's mass, centre of mass and count (slot i = the round's i-th child in
+// level order), and the moments of its twig children only (compacted; tslot
+// maps a child's slot to its twig slot, -1 for a leaf).
+constexpr int UP_KIDS = 128, UP_TWIGS = 64;
+template <int MP>
+struct UpStage {
+    double m[UP_KIDS], x[UP_KIDS], y[UP_KIDS], z[UP_KIDS];
+    int n[UP_KIDS];
+    int8_t tslot[UP_KIDS];
+    int tnode[UP_TWIGS];
+    double q[MP >= 4 ? 6 : 1][MP >= 4 ? UP_TWIGS : 1];
+    double o[MP > 4 ? 10 : 1][MP > 4 ? UP_TWIGS : 1];
+};
+
+// Segment size (log2, 8..64 nodes) of the upward pass at a level of `nodes`
+// nodes whose children number `kids`: the largest that keeps a segment's
+// children near 50 on average (at the top levels, 5-8 us per level against
+// 13-15 us with segments of twice as many children).
+__host__ __device__ inline int up_segsh(int64_t nodes, int64_t kids) {
+    int sh = 6;
+    while (sh > 3 && (kids << sh) > 50 * nodes) --sh;
+    return sh;
+}
+
+// Inclusive prefix sum over the wavefront's lanes.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Moments of the nodes of one level, one 64-thread workgroup (one wavefront)
+// per segment of 2^segsh consecutive nodes.  A segment is done in rounds: a
+// round takes the longest run of its remaining nodes with at most UP_KIDS
+// children and UP_TWIGS twig children (at least 8 nodes).  The lanes load the
+// round's children -- mass, centre of mass and count of every child, the
+// moments of the twig children, consecutive lanes at consecutive addresses,
+// all in flight together -- into LDS (which children are twigs comes from
+// the parents' masks, so no load waits on another); lane j then forms node
+// j's moments from its children there, and loads its body if the node is a
+// leaf.  The sums are those of evaluate_multipole (fortran/octree_default.f90
+// :744-882) and incorporate_particles (:582-628): children in the reference's
+// child order r = 0..7 (octant index 1 + x + 2(1-y) + 4(1-z), :527-531, of
+// digit 4x + 2y + z), contraction off, a leaf child's (zero) higher moments
+// skipped, so every bit matches.  Bounds come from (a, b), or from lv[depth],
+// lv[depth + 1] on the device when lv is given (graph mode, which also takes
+// the segment size from the level sizes); the workgroups stride over the
+// segments.
+template <int MP>
+__global__ __launch_bounds__(64) void k_upward(int64_t a, int64_t b,
+                                               const int64_t* __restrict__ lv, int depth,
+                                               int segsh, UpArgs u) {
+#pragma clang fp contract(off)
+    __shared__ UpStage<MP> st;
+    if (lv) {
+        a = lv[depth];
+        b = lv[depth + 1];
+        segsh = up_segsh(b - a, depth + 2 <= MAX_DEPTH + 1 ? lv[depth + 2] - b : 0);
+    }
+    const int lane = threadIdx.x;
+    const int64_t S = u.nn;
+    const int64_t nseg = (b - a + (1 << segsh) - 1) >> segsh;
+    for (int64_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        int64_t p0 = a + (seg << segsh);
+        const int64_t pend = min(b, p0 + (1 << segsh));
+        while (p0 < pend) {
+            const int64_t p = p0 + lane;
+            const bool act = p < pend;
+            int type = 0, cs = 0, ce = 0, first = 0;
+            unsigned present = 0u, tw = 0u;
+            if (act) {
+                type = u.ntype[p];
+                cs = u.ncstart[p];
+                ce = u.ncstart[p + 1];
+                first = u.nfirst[p];
+                present = u.nmask[p];
+                tw = u.ntwig[p];
+            }
+            const int c0 = __shfl(cs, 0);
+            const int tend = wave_incl_scan((int)__popc(tw));  // twig children up to this node
+            // a prefix of the lanes (both counts grow with the lane)
+            const int J = __popcll(__ballot(act && ce - c0 <= UP_KIDS && tend <= UP_TWIGS));
+            const int nc = __shfl(ce, J - 1) - c0;
+            const int ntw = __shfl(tend, J - 1);
+            const bool mine = lane < J;
+            double4 body = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (mine && type == 1) body = u.sbody[first];
+            if (mine) {
+                int ts = tend - (int)__popc(tw);
+                for (int i = 0; i < ce - cs; ++i) {
+                    const bool t = (tw >> i) & 1u;
+                    st.tslot[cs - c0 + i] = (int8_t)(t ? ts : -1);
+                    if (t) st.tnode[ts++] = cs + i;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < UP_KIDS / 64; ++h) {
+                const int i = lane + 64 * h;
+                if (i < nc) {
+                    const int64_t c = c0 + i;
+                    st.m[i] = u.nmass[c];
+                    st.x[i] = u.ncomx[c];
+                    st.y[i] = u.ncomy[c];
+                    st.z[i] = u.ncomz[c];
+                    st.n[i] = u.ncount[c];
+                }
+            }
+            if (MP >= 4 && lane < ntw) {
+                const int64_t c = st.tnode[lane];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) st.q[j][lane] = u.nquad[j * S + c];
+                if (MP > 4)
+#pragma unroll
+                    for (int j = 0; j < 10; ++j) st.o[MP > 4 ? j : 0][lane] = u.noct[j * S + c];
+            }
+            __syncthreads();
+            if (mine) {
+                double Q[6] = {0, 0, 0, 0, 0, 0};
+                double O[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                double mass = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
+                int count = 0;
+                if (type == 1) {
+                    mass = body.w;
+                    qx = body.x;
+                    qy = body.y;
+                    qz = body.z;
+                    count = 1;
+                } else if (depth == MAX_DEPTH) {
+                    // max-depth twig: evaluate_multipole skips it (:772-774,
+                    // stays 0); incorporate_particles divides 0/0 (:614-616)
+                    if (MP == 1) {
+                        qx = qx / mass;
+                        qy = qy / mass;
+                        qz = qz / mass;
+                    }
+                } else {
+                    const int base = cs - c0;
+                    int sl[8];
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        const unsigned d = 4u * (unsigned)(r & 1) + 2u * (1u - ((r >> 1) & 1u)) +
+                                           (1u - ((r >> 2) & 1u));
+                        sl[r] = ((present >> d) & 1u) ? base + __popc(present & ((1u << d) - 1u))
+                                                      : -1;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        const int c = sl[r];
+                        if (c < 0) continue;
+                        const double mc = st.m[c];
+                        mass = mass + mc;
+                        qx = qx + mc * st.x[c];
+                        qy = qy + mc * st.y[c];
+                        qz = qz + mc * st.z[c];
+                        count += st.n[c];
+                    }
+                    qx = qx / mass;
+                    qy = qy / mass;
+                    qz = qz / mass;
+                    if (MP >= 4) {
+                        constexpr bool oct = MP > 4;
+#pragma unroll
+                        for (int r = 0; r < 8; ++r) {
+                            const int c = sl[r];
+                            if (c < 0) continue;
+                            const double pm = st.m[c];
+                            const double dxi = st.x[c] - qx;
+                            const double dyi = st.y[c] - qy;
+                            const double dzi = st.z[c] - qz;
+                            // O[10..13] are recomputed from O[0..9] below, so
+                            // children's are not summed
+                            const int ts = st.tslot[c];
+                            const bool twig = ts >= 0;
+                            double Qc[6] = {0, 0, 0, 0, 0, 0};
+                            if (twig) {
+#pragma unroll
+                                for (int j = 0; j < 6; ++j) Qc[j] = st.q[j][ts];
+#pragma unroll
+                                for (int j = 0; j < 6; ++j) Q[j] = Q[j] + Qc[j];
+                                if (oct)
+#pragma unroll
+                                    for (int j = 0; j < 10; ++j) O[j] = O[j] + st.o[oct ? j : 0][ts];
+                            }
+                            Q[0] = Q[0] + pm * (dxi * dxi);
+                            Q[1] = Q[1] + pm * (dyi * dyi);
+                            Q[2] = Q[2] + pm * (dzi * dzi);
+                            Q[3] = Q[3] + pm * dxi * dyi;
+                            Q[4] = Q[4] + pm * dxi * dzi;
+                            Q[5] = Q[5] + pm * dyi * dzi;
+                            if (oct) {
+                                O[0] = O[0] + pm * dxi * (dxi * dxi);
+                                O[1] = O[1] + pm * dxi * (dyi * dyi);
+                                O[2] = O[2] + pm * dxi * (dzi * dzi);
+                                O[3] = O[3] + pm * dyi * (dxi * dxi);
+                                O[4] = O[4] + pm * dyi * (dyi * dyi);
+                                O[5] = O[5] + pm * dyi * (dzi * dzi);
+                                O[6] = O[6] + pm * dzi * (dxi * dxi);
+                                O[7] = O[7] + pm * dzi * (dyi * dyi);
+                                O[8] = O[8] + pm * dzi * (dzi * dzi);
+                                O[9] = O[9] + pm * dxi * dyi * dzi;
+                                if (twig) {
+                                    O[0] = O[0] + 3.0 * dxi * Qc[0];
+                                    O[1] = O[1] + 2.0 * dyi * Qc[3] + dxi * Qc[1];
+                                    O[2] = O[2] + 2.0 * dzi * Qc[4] + dxi * Qc[2];
+                                    O[3] = O[3] + 2.0 * dxi * Qc[3] + dyi * Qc[0];
+                                    O[4] = O[4] + 3.0 * dyi * Qc[1];
+                                    O[5] = O[5] + 2.0 * dzi * Qc[5] + dyi * Qc[2];
+                                    O[6] = O[6] + 2.0 * dxi * Qc[4] + dzi * Qc[0];
+                                    O[7] = O[7] + 2.0 * dyi * Qc[5] + dzi * Qc[1];
+                                    O[8] = O[8] + 3.0 * dzi * Qc[2];
+                                    O[9] = O[9] + dxi * Qc[5] + dyi * Qc[4] + dzi * Qc[3];
+                                }
+                            }
+                        }
+                        if (oct) {
+                            double s = 0.0;
+                            for (int j = 0; j < 10; ++j) s = s + O[j];
+                            O[10] = s;
+                            O[11] = (O[0] + O[1]) + O[2];
+                            O[12] = (O[3] + O[4]) + O[5];
+                            O[13] = (O[6] + O[7]) + O[8];
+                        }
+                    }
+                }
+                u.nmass[p] = mass;
+                u.ncomx[p] = qx;
+                u.ncomy[p] = qy;
+                u.ncomz[p] = qz;
+                u.ncount[p] = count;
+                if (MP >= 4)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) u.nquad[j * S + p] = Q[j];
+                if (MP > 4)
+#pragma unroll
+                    for (int j = 0; j < 14; ++j) u.noct[j * S + p] = O[j];
+            }
+            __syncthreads();
+            p0 += J;
+        }
+    }
+}
+
+// Thread-per-node form of the same sums, for levels whose nodes have more
+// children than nodes (a thread's loads of its children are then mostly
+// distinct and the arithmetic keeps its lanes busy): node k at `depth` from
+// its children (or its body).
 __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int depth) {
 #pragma clang fp contract(off)
     const int multipole = u.multipole;
@@ -653,10 +917,6 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
     const int* __restrict__ nfirst = u.nfirst;
     const int* __restrict__ ncstart = u.ncstart;
     const uint8_t* __restrict__ nmask = u.nmask;
-    const double* __restrict__ sx = u.sx;
-    const double* __restrict__ sy = u.sy;
-    const double* __restrict__ sz = u.sz;
-    const double* __restrict__ sm = u.sm;
     double* __restrict__ nmass = u.nmass;
     double* __restrict__ ncomx = u.ncomx;
     double* __restrict__ ncomy = u.ncomy;
@@ -670,11 +930,11 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
     double mass = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
     int count = 0;
     if (ntype[k] == 1) {
-        int p = nfirst[k];
-        mass = sm[p];
-        qx = sx[p];
-        qy = sy[p];
-        qz = sz[p];
+        const double4 body = u.sbody[nfirst[k]];
+        mass = body.w;
+        qx = body.x;
+        qy = body.y;
+        qz = body.z;
         count = 1;
     } else if (depth == MAX_DEPTH) {
         // max-depth twig: evaluate_multipole skips it (:772-774, stays 0);
@@ -796,20 +1056,47 @@ __device__ __forceinline__ void upward_node(const UpArgs& u, int64_t k, int dept
         for (int j = 0; j < 14; ++j) noct[(int64_t)j * nn + k] = O[j];
 }
 
-// One launch per level.
-__global__ __launch_bounds__(128) void k_upward(int64_t a, int64_t b, int depth, UpArgs u) {
-    const int64_t k = a + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k < b) upward_node(u, k, depth);
-}
 
-// Graph mode: level `depth` with its bounds lv[depth], lv[depth + 1] read from
-// the device, grid-stride.
-__global__ __launch_bounds__(128) void k_upward_dev(const int64_t* __restrict__ lv, int depth,
-                                                    UpArgs u) {
-    const int64_t a = lv[depth], b = lv[depth + 1];
+// One launch per level, a thread per node: bounds (a, b), or lv[depth],
+// lv[depth + 1] from the device (graph mode), grid-stride.
+__global__ __launch_bounds__(128) void k_upward_t(int64_t a, int64_t b,
+                                                  const int64_t* __restrict__ lv, int depth,
+                                                  UpArgs u) {
+    if (lv) {
+        a = lv[depth];
+        b = lv[depth + 1];
+    }
     for (int64_t k = a + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < b;
          k += (int64_t)gridDim.x * blockDim.x)
         upward_node(u, k, depth);
+}
+
+// One launch per level: a segment per workgroup, or (graph mode) `grid`
+// workgroups striding over the level's device-held bounds.
+// Which form of the upward pass a level of `nodes` nodes with `kids`
+// children takes: the staged one (k_upward) where most nodes are leaves or
+// the level is small, the thread-per-node one (k_upward_t) elsewhere.
+// Measured per level at 1M Plummer: the staged form 44 against 52 us at the
+// densest leaf-heavy level and 5-8 against 7-11 us at the top levels; the
+// thread-per-node form 60 against 64-76 us where nodes have 1.4 children.
+static bool up_staged(int64_t nodes, int64_t kids) { return kids < nodes || nodes < 4096; }
+
+// grid: for graph mode (lv given), in workgroups of the chosen form
+static void launch_upward(const UpArgs& u, int64_t a, int64_t b, int64_t kids,
+                          const int64_t* lv, int depth, bool staged, int grid, hipStream_t s) {
+    if (!staged) {
+        if (!lv) grid = grid_for(b - a, 128);
+        k_upward_t<<<grid, 128, 0, s>>>(a, b, lv, depth, u);
+        return;
+    }
+    int segsh = 6;
+    if (!lv) {
+        segsh = up_segsh(b - a, kids);
+        grid = (int)((b - a + (1 << segsh) - 1) >> segsh);
+    }
+    if (u.multipole == 1) k_upward<1><<<grid, 64, 0, s>>>(a, b, lv, depth, segsh, u);
+    else if (u.multipole == 4) k_upward<4><<<grid, 64, 0, s>>>(a, b, lv, depth, segsh, u);
+    else k_upward<8><<<grid, 64, 0, s>>>(a, b, lv, depth, segsh, u);
 }
 
 // Graph mode: the checks the host makes after node emission, as flags that
@@ -841,24 +1128,30 @@ __global__ void k_safe_nn(int64_t* __restrict__ st, int64_t cap, int nsys) {
     st[ST_SAFENN] = ok ? nn : 0;
 }
 
-// Parent pointers of every node (roots, the first nroots nodes, get -1) and
-// the mask of the octant digits of each node's children.
+// Parent pointers of every node (roots, the first nroots nodes, get -1), the
+// mask of the octant digits of each node's children and the mask of which of
+// them (bit i: the i-th child in level order) are twigs.
 __global__ __launch_bounds__(256) void k_parents(int64_t nn, int64_t nroots,
                                                  const int* __restrict__ ncstart,
                                                  const uint8_t* __restrict__ ndigit,
+                                                 const int8_t* __restrict__ ntype,
                                                  int* __restrict__ nparent,
                                                  uint8_t* __restrict__ nmask,
+                                                 uint8_t* __restrict__ ntwig,
                                                  const int64_t* __restrict__ dnn) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (dnn) nn = *dnn;  // the count is on the device (0 when the tree is unusable)
     if (k >= nn) return;
     if (k < nroots) nparent[k] = -1;
-    unsigned m = 0u;
-    for (int c = ncstart[k]; c < ncstart[k + 1]; ++c) {
+    unsigned m = 0u, tw = 0u;
+    const int cs = ncstart[k], ce = ncstart[k + 1];
+    for (int c = cs; c < ce; ++c) {
         nparent[c] = (int)k;
         m |= 1u << ndigit[c];
+        tw |= (ntype[c] == 2 ? 1u : 0u) << (c - cs);
     }
     nmask[k] = (uint8_t)m;
+    ntwig[k] = (uint8_t)tw;
 }
 
 // Skip pointer of every node: its next sibling, else that of the nearest
@@ -887,8 +1180,8 @@ __global__ __launch_bounds__(256) void k_next(int64_t nn, const int* __restrict_
 // ------------------------------------------------------------------ driver
 static UpArgs up_args(const hq_tree* t) {
     return UpArgs{t->multipole, t->ntype.as<int8_t>(), t->nfirst.as<int>(), t->ncstart.as<int>(),
-                  t->nmask.as<uint8_t>(), t->sx.as<double>(), t->sy.as<double>(),
-                  t->sz.as<double>(), t->sm.as<double>(), t->nmass.as<double>(),
+                  t->nmask.as<uint8_t>(), t->ntwig.as<uint8_t>(), t->sbody.as<double4>(),
+                  t->nmass.as<double>(),
                   t->ncomx.as<double>(), t->ncomy.as<double>(), t->ncomz.as<double>(),
                   t->multipole >= 4 ? t->nquad.as<double>() : nullptr,
                   t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), t->cap};
@@ -899,8 +1192,9 @@ static UpArgs up_args(const hq_tree* t) {
 static void launch_pointers(hq_tree* t, int64_t nn, int nsys, const int64_t* dnn) {
     hipStream_t s = t->stream;
     k_parents<<<grid_for(nn, 256), 256, 0, s>>>(nn, nsys, t->ncstart.as<int>(),
-                                                t->ndigit.as<uint8_t>(), t->nparent.as<int>(),
-                                                t->nmask.as<uint8_t>(), dnn);
+                                                t->ndigit.as<uint8_t>(), t->ntype.as<int8_t>(),
+                                                t->nparent.as<int>(), t->nmask.as<uint8_t>(),
+                                                t->ntwig.as<uint8_t>(), dnn);
     k_next<<<grid_for(nn, 256), 256, 0, s>>>(nn, t->ncstart.as<int>(), t->nparent.as<int>(),
                                              t->nnext.as<int>(), dnn);
 }
@@ -917,7 +1211,8 @@ static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
     // whatever the level holds at replay)
     const UpArgs ua = up_args(t);
     for (int d = NDEP - 1; d >= 0; --d)
-        k_upward_dev<<<std::max(1, t->graph_lgrid[d]), 128, 0, s>>>(gs + GS_LEVEL0, d, ua);
+        launch_upward(ua, 0, 0, 0, gs + GS_LEVEL0, d, t->graph_lgrid[d] < 0,
+                      std::max(1, std::abs(t->graph_lgrid[d])), s);
     HQ_CHECK_HIP(hipGetLastError());
     return HQ_OK;
 }
@@ -928,8 +1223,15 @@ static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
 void graph_grids(hq_tree* t) {
     for (int d = 0; d < NDEP; ++d) {
         const int64_t sz = t->level_start[d + 1] - t->level_start[d];
-        const int g = (int)std::min<int64_t>(grid_for(sz + sz / 4 + 128, 128), grid_for(t->cap, 128));
-        t->graph_lgrid[d] = std::max(t->graph_lgrid[d], g);
+        const int64_t kids = d + 2 <= NDEP ? t->level_start[d + 2] - t->level_start[d + 1] : 0;
+        // the form is fixed by the first capture (the sign: negative for the
+        // staged form); the grid only grows
+        const bool staged = t->graph_lgrid[d] != 0 ? t->graph_lgrid[d] < 0 : up_staged(sz, kids);
+        const int unit = staged ? (1 << up_segsh(sz, kids)) : 128;
+        const int g = (int)std::min<int64_t>(grid_for(sz + sz / 4 + 128, unit),
+                                             grid_for(t->cap, staged ? 8 : 128));
+        const int prev = std::abs(t->graph_lgrid[d]);
+        t->graph_lgrid[d] = (staged ? -1 : 1) * std::max(prev, g);
     }
 }
 
@@ -963,14 +1265,14 @@ int scan_excl_buf(uint32_t* a, int64_t len, DevBuf& part, hipStream_t s) {
 
 template <int ITEMS>
 static int rs_passes(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
-                     uint32_t* hist, hipStream_t s) {
+                     int shift0, uint32_t* hist, hipStream_t s) {
     const int nblk = grid_for(n, RS_THREADS * ITEMS);
     uint32_t* dtot = hist + (size_t)256 * nblk;  // per-digit totals
     uint64_t *ka = keys, *kb = ktmp;
     int *va = vals, *vb = vtmp;
-    const int passes = (bits + 7) / 8;
+    const int passes = (bits - shift0 + 7) / 8;
     for (int pass = 0; pass < passes; ++pass) {
-        const int shift = 8 * pass;
+        const int shift = shift0 + 8 * pass;
         k_rs_hist<ITEMS><<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist, nblk);
         k_rs_scan<<<256, RS_THREADS, 0, s>>>(hist, nblk, dtot);
         k_rs_scatter<ITEMS><<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblk, dtot);
@@ -995,11 +1297,17 @@ size_t rs_hist_words(int64_t n) {
     return (size_t)256 * grid_for(n, RS_THREADS * rs_items(n)) + 256;
 }
 
+// Bits [shift0, bits) of the keys; the lower ones are left in input order.
+static int radix_sort_bits(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n,
+                           int bits, int shift0, uint32_t* hist, hipStream_t s) {
+    if (n <= 1) return HQ_OK;
+    if (rs_items(n) == 2) return rs_passes<2>(keys, vals, ktmp, vtmp, n, bits, shift0, hist, s);
+    return rs_passes<8>(keys, vals, ktmp, vtmp, n, bits, shift0, hist, s);
+}
+
 int radix_sort_pairs_ws(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
                         uint32_t* hist, hipStream_t s) {
-    if (n <= 1) return HQ_OK;
-    if (rs_items(n) == 2) return rs_passes<2>(keys, vals, ktmp, vtmp, n, bits, hist, s);
-    return rs_passes<8>(keys, vals, ktmp, vtmp, n, bits, hist, s);
+    return radix_sort_bits(keys, vals, ktmp, vtmp, n, bits, 0, hist, s);
 }
 
 int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64_t n, int bits,
@@ -1011,12 +1319,17 @@ int radix_sort_pairs(uint64_t* keys, int* vals, uint64_t* ktmp, int* vtmp, int64
     return radix_sort_pairs_ws(keys, vals, ktmp, vtmp, n, bits, hist.as<uint32_t>(), s);
 }
 
+// The tree's keys by their high word above the lowest TIE_SKIP_BITS bits;
+// k_tie_fix orders the rest.
 static int radix_sort(hq_tree* t) {
     int rc;
     if ((rc = t->khi2.ensure(t->n * sizeof(uint64_t)))) return rc;
     if ((rc = t->idx2.ensure(t->n * sizeof(int)))) return rc;
-    return radix_sort_pairs(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
-                            t->idx2.as<int>(), t->n, 63, t->hist, t->scanpart, t->stream);
+    if (t->n <= 1) return HQ_OK;
+    if ((rc = t->hist.ensure(rs_hist_words(t->n) * sizeof(uint32_t)))) return rc;
+    return radix_sort_bits(t->khi.as<uint64_t>(), t->idx.as<int>(), t->khi2.as<uint64_t>(),
+                           t->idx2.as<int>(), t->n, 63, TIE_SKIP_BITS, t->hist.as<uint32_t>(),
+                           t->stream);
 }
 
 // Gather the sorted bodies, then the nodes from the common prefixes of
@@ -1146,7 +1459,7 @@ int tree_build(hq_tree* t) {
         (rc = t->nfirst.ensure(cap * 4)) || (rc = t->ncount.ensure(cap * 4)) ||
         (rc = t->ncstart.ensure((cap + 1) * 4)) || (rc = t->nnext.ensure(cap * 4)) ||
         (rc = t->nparent.ensure(cap * 4)) || (rc = t->ndigit.ensure(cap)) ||
-        (rc = t->nmask.ensure(cap)) ||
+        (rc = t->nmask.ensure(cap)) || (rc = t->ntwig.ensure(cap)) ||
         (rc = t->nmass.ensure(cap * 8)) || (rc = t->ncomx.ensure(cap * 8)) ||
         (rc = t->ncomy.ensure(cap * 8)) || (rc = t->ncomz.ensure(cap * 8)) ||
         (rc = t->ncx.ensure(cap * 8)) || (rc = t->ncy.ensure(cap * 8)) ||
@@ -1208,9 +1521,10 @@ int tree_build(hq_tree* t) {
     int64_t flags = t->hstatus[ST_FLAGS];
     if (flags & 1) return fail(HQ_ERR_NONFINITE, "non-finite input position");
     if (flags & 4) {
-        // a run of more than TIE_RUN_MAX bodies shares levels 1..21 (a cell
-        // 2^-21 of the root wide): sort by the full 25-level key instead --
-        // levels 22..25 first, then 1..21 -- and build the nodes again
+        // a run of more than TIE_RUN_MAX bodies shares the sorted bits of the
+        // high word (a cube 2^-16 of the root wide): sort by the full
+        // 25-level key instead -- levels 22..25 first, then 1..21 -- and
+        // build the nodes again
         if ((rc = full_key_sort(t))) return rc;
         HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
         if ((rc = gather_and_emit(t, cap, mark))) return rc;
@@ -1237,7 +1551,8 @@ int tree_build(hq_tree* t) {
     const UpArgs ua = up_args(t);
     for (int d = t->most_depth; d >= 0; --d) {
         const int64_t a = t->level_start[d], b = t->level_start[d + 1];
-        if (b > a) k_upward<<<grid_for(b - a, 128), 128, 0, s>>>(a, b, d, ua);
+        const int64_t kids = d + 2 <= NDEP ? t->level_start[d + 2] - b : 0;
+        if (b > a) launch_upward(ua, a, b, kids, nullptr, d, up_staged(b - a, kids), 0, s);
     }
     HQ_CHECK_HIP(hipEventRecord(t->ev[4], s));
     HQ_CHECK_HIP(hipGetLastError());

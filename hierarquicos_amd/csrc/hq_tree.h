@@ -44,6 +44,7 @@ struct hq_tree {
     // node store (level order)
     int64_t nn = 0, cap = 0;
     hq::DevBuf ndepth, ntype, nfirst, ncount, ncstart, nnext, nparent, ndigit, nmask;
+    hq::DevBuf ntwig;  // per node: which children (bit i: the i-th) are twigs
     hq::DevBuf nmass, ncomx, ncomy, ncomz, nquad, noct, ncx, ncy, ncz, nkhi, nklo;
     hq::DevBuf mom32;  // float moment records for the mixed-precision walk
     hq::DevBuf wnode;  // packed per-node records for the mixed walk's traversal
@@ -87,7 +88,8 @@ struct hq_tree {
     bool graph_mode = false;
     int64_t graph_nbun = 0;  // grid of the walk in graph mode (upper bound of the bunches;
                              // only grows, so that repeated calls capture the same graph)
-    int graph_lgrid[hq::MAX_DEPTH + 2] = {};  // grids of the captured upward pass (only grow)
+    int graph_lgrid[hq::MAX_DEPTH + 2] = {};  // grids of the captured upward pass (only grow;
+                                              // negative: the staged form)
     // instantiated step graphs of integrate, kept for calls with the same key
     struct GraphEntry {
         uint64_t key = 0, used = 0;
