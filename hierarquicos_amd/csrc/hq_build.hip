@@ -19,8 +19,9 @@
 //                    the nodes of depths D_{i-1}+1 .. max(D_{i-1},D_i)+1 (the
 //                    last one its leaf).  Two ballot passes place every node
 //                    in level order; children of a node are contiguous.
-//   5. k_parents,    parent and "next" (skip) pointers for stackless walks,
-//      k_next        one launch each (the skip pointer by an ancestor walk)
+//   5. k_parents,    parent pointers and child masks; "next" (skip) pointers
+//      k_next        for the stackless per-target walk, by an ancestor walk
+//                    per node (a launched build leaves them to that walk)
 //   6. k_upward      level-synchronous M2M from depth 25 to 0, a wavefront per
 //                    64 nodes staging their children in LDS; children summed
 //                    in the reference's child order (octant index 1..8), with
@@ -38,11 +39,28 @@ namespace hq {
 enum { G_CX = 0, G_CY, G_CZ, G_SIDE, G_HALF, G_XMIN, G_XMAX, G_YMIN, G_YMAX, G_ZMIN, G_ZMAX };
 
 // ------------------------------------------------------------------ bbox
+// A depth-d node has side ldexp(side, -d): add_node gets the parent's
+// halfside as its side (:508), and L2 = side^2 (:475).  Lane d of the calling
+// wavefront writes depth d of a system's table g.
+__device__ __forceinline__ void geom_table(double* __restrict__ g, double side, int d) {
+#pragma clang fp contract(off)
+    if (d <= MAX_DEPTH) {
+        const double sd = ldexp(side, -d);
+        g[S_L2 + d] = sd * sd;
+        g[S_HALFD + d] = sd / 2.0;
+    }
+}
+
+// Bounding box of one system: each workgroup reduces a grid-stride share to
+// part[6 * block]; k_bbox_final reduces the shares.  (Finishing in the last
+// workgroup to arrive instead took 74 us at 1M: the device-wide fence each
+// workgroup needs before its arrival writes back its L2.)
 __global__ __launch_bounds__(256) void k_bbox_partial(const double* __restrict__ x,
                                                       const double* __restrict__ y,
                                                       const double* __restrict__ z, int64_t n,
                                                       double* __restrict__ part,
                                                       int64_t* __restrict__ status) {
+#pragma clang fp contract(off)
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     bool bad = false;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -77,6 +95,10 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double* __restrict__
     }
 }
 
+// The shares of k_bbox_partial (one wavefront): the root cube --
+// node_size_center (octree_default.f90:424-443) and the root add_node
+// (:345-347) -- and the per-depth tables.  min and max are exact, so the
+// result does not depend on the order of the shares.
 __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ part, int nb,
                                                    double* __restrict__ g) {
 #pragma clang fp contract(off)
@@ -91,28 +113,28 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
             mn[c] = fmin(mn[c], __shfl_xor(mn[c], o, 64));
             mx[c] = fmax(mx[c], __shfl_xor(mx[c], o, 64));
         }
+    double ext = mx[0] - mn[0];
+    ext = fmax(ext, mx[1] - mn[1]);
+    ext = fmax(ext, mx[2] - mn[2]);
+    const double side = 1.2 * ext;
+    double* g0 = g + G_SYS;  // system 0 of the per-system table
     if (threadIdx.x == 0) {
-        // node_size_center (octree_default.f90:424-443) and the root add_node (:345-347)
         g[G_CX] = 0.5 * (mn[0] + mx[0]);
         g[G_CY] = 0.5 * (mn[1] + mx[1]);
         g[G_CZ] = 0.5 * (mn[2] + mx[2]);
-        double ext = mx[0] - mn[0];
-        ext = fmax(ext, mx[1] - mn[1]);
-        ext = fmax(ext, mx[2] - mn[2]);
-        double side = 1.2 * ext;
         g[G_SIDE] = side;
         g[G_HALF] = side / 2.0;
         g[G_XMIN] = mn[0]; g[G_XMAX] = mx[0];
         g[G_YMIN] = mn[1]; g[G_YMAX] = mx[1];
         g[G_ZMIN] = mn[2]; g[G_ZMAX] = mx[2];
-        double* g0 = g + G_SYS;  // system 0 of the per-system table
         g0[S_CX] = g[G_CX]; g0[S_CY] = g[G_CY]; g0[S_CZ] = g[G_CZ];
         g0[S_SIDE] = side; g0[S_HALF] = side / 2.0;
     }
+    geom_table(g0, side, threadIdx.x);
 }
 
 // One workgroup per system of a batch: the system's root cube (same
-// arithmetic as k_bbox_final) into sg[SG * s + ...].
+// arithmetic as k_bbox) and per-depth tables into sg[SG * s + ...].
 __global__ __launch_bounds__(256) void k_sys_bbox(const double* __restrict__ x,
                                                   const double* __restrict__ y,
                                                   const double* __restrict__ z,
@@ -142,36 +164,25 @@ __global__ __launch_bounds__(256) void k_sys_bbox(const double* __restrict__ x,
     if (__lane_id() == 0)
         for (int c = 0; c < 3; ++c) { sh[w][2 * c] = mn[c]; sh[w][2 * c + 1] = mx[c]; }
     __syncthreads();
+    if (threadIdx.x >= 64) return;
+    for (int k = 1; k < 4; ++k)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], sh[k][2 * c]);
+            mx[c] = fmax(mx[c], sh[k][2 * c + 1]);
+        }
+    double* g = sg + SG * sys;
+    double ext = mx[0] - mn[0];
+    ext = fmax(ext, mx[1] - mn[1]);
+    ext = fmax(ext, mx[2] - mn[2]);
+    const double side = 1.2 * ext;
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; ++k)
-            for (int c = 0; c < 3; ++c) {
-                mn[c] = fmin(mn[c], sh[k][2 * c]);
-                mx[c] = fmax(mx[c], sh[k][2 * c + 1]);
-            }
-        double* g = sg + SG * sys;
         g[S_CX] = 0.5 * (mn[0] + mx[0]);
         g[S_CY] = 0.5 * (mn[1] + mx[1]);
         g[S_CZ] = 0.5 * (mn[2] + mx[2]);
-        double ext = mx[0] - mn[0];
-        ext = fmax(ext, mx[1] - mn[1]);
-        ext = fmax(ext, mx[2] - mn[2]);
-        const double side = 1.2 * ext;
         g[S_SIDE] = side;
         g[S_HALF] = side / 2.0;
     }
-}
-
-// A depth-d node has side ldexp(side, -d): add_node gets the parent's
-// halfside as its side (:508), and L2 = side^2 (:475).  One workgroup per system.
-__global__ void k_geom_tables(double* __restrict__ sg) {
-#pragma clang fp contract(off)
-    double* g = sg + SG * blockIdx.x;
-    const int d = threadIdx.x;
-    if (d <= MAX_DEPTH) {
-        const double sd = ldexp(g[S_SIDE], -d);
-        g[S_L2 + d] = sd * sd;
-        g[S_HALFD + d] = sd / 2.0;
-    }
+    geom_table(g, side, threadIdx.x);
 }
 
 // system id of every particle of a batch (original order)
@@ -1187,16 +1198,31 @@ static UpArgs up_args(const hq_tree* t) {
                   t->multipole > 4 ? t->noct.as<double>() : nullptr, t->ncount.as<int>(), t->cap};
 }
 
-// Parent, child-mask and skip pointers of the nodes; with dnn the count is
-// read on the device (grids over the capacity).
-static void launch_pointers(hq_tree* t, int64_t nn, int nsys, const int64_t* dnn) {
+// Parent pointers and child masks of the nodes, and (with_next) their skip
+// pointers; with dnn the count is read on the device (grids over the
+// capacity).  A launched build leaves the skip pointers, which only the
+// per-target walk reads, to ensure_next.
+static void launch_pointers(hq_tree* t, int64_t nn, int nsys, const int64_t* dnn, bool with_next) {
     hipStream_t s = t->stream;
     k_parents<<<grid_for(nn, 256), 256, 0, s>>>(nn, nsys, t->ncstart.as<int>(),
                                                 t->ndigit.as<uint8_t>(), t->ntype.as<int8_t>(),
                                                 t->nparent.as<int>(), t->nmask.as<uint8_t>(),
                                                 t->ntwig.as<uint8_t>(), dnn);
-    k_next<<<grid_for(nn, 256), 256, 0, s>>>(nn, t->ncstart.as<int>(), t->nparent.as<int>(),
-                                             t->nnext.as<int>(), dnn);
+    if (with_next)
+        k_next<<<grid_for(nn, 256), 256, 0, s>>>(nn, t->ncstart.as<int>(), t->nparent.as<int>(),
+                                                 t->nnext.as<int>(), dnn);
+    t->next_valid = with_next;
+}
+
+// The skip pointers of the built tree, once per build (on its stream).
+int ensure_next(hq_tree* t) {
+    if (t->next_valid || t->nn <= 0) return HQ_OK;
+    k_next<<<grid_for(t->nn, 256), 256, 0, t->stream>>>(t->nn, t->ncstart.as<int>(),
+                                                        t->nparent.as<int>(), t->nnext.as<int>(),
+                                                        nullptr);
+    HQ_CHECK_HIP(hipGetLastError());
+    t->next_valid = true;
+    return HQ_OK;
 }
 
 // Graph mode, after node emission: the checks on the device, then pointers and
@@ -1206,7 +1232,7 @@ static int build_graph_tail(hq_tree* t, int64_t cap, int nsys) {
     hipStream_t s = t->stream;
     int64_t* gs = t->gstate.as<int64_t>();
     k_graph_build_check<<<1, 64, 0, s>>>(t->status.as<int64_t>(), cap, nsys, gs);
-    launch_pointers(t, cap, nsys, gs + GS_NN);
+    launch_pointers(t, cap, nsys, gs + GS_NN, true);
     // grids set by graph_grids before the capture (the kernel strides over
     // whatever the level holds at replay)
     const UpArgs ua = up_args(t);
@@ -1490,7 +1516,6 @@ int tree_build(hq_tree* t) {
         k_sys_bbox<<<nsys, 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(), t->z.as<double>(),
                                         t->sysoff.as<int64_t>(), sg, t->status.as<int64_t>());
     }
-    k_geom_tables<<<nsys, 64, 0, s>>>(sg);
     // 2. keys
     k_keys<<<grid_for(n, 256), 256, 0, s>>>(t->x.as<double>(), t->y.as<double>(),
                                             t->z.as<double>(), n, sg, sysid,
@@ -1513,7 +1538,7 @@ int tree_build(hq_tree* t) {
     HQ_CHECK_HIP(hipMemcpyAsync(t->hstatus, t->status.p, ST_SIZE * sizeof(int64_t),
                                 hipMemcpyDeviceToHost, s));
     HQ_CHECK_HIP(hipEventRecord(t->ev[11], s));
-    launch_pointers(t, cap, nsys, t->status.as<int64_t>() + ST_SAFENN);
+    launch_pointers(t, cap, nsys, t->status.as<int64_t>() + ST_SAFENN, false);
     HQ_CHECK_HIP(hipEventSynchronize(t->ev[11]));
     HQ_CHECK_HIP(hipGetLastError());
     collect_build_times(t);
@@ -1546,7 +1571,7 @@ int tree_build(hq_tree* t) {
         if (t->level_start[d + 1] > t->level_start[d]) t->most_depth = d;
     // 5. parent and skip pointers (the walks' threading; the roots end every
     // walk), unless queued above
-    if (!pointers) launch_pointers(t, t->nn, nsys, nullptr);
+    if (!pointers) launch_pointers(t, t->nn, nsys, nullptr, false);
     // 6. moments, deepest level first
     const UpArgs ua = up_args(t);
     for (int d = t->most_depth; d >= 0; --d) {

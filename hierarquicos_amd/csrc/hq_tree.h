@@ -32,6 +32,7 @@ struct hq_tree {
     hipStream_t stream = nullptr;
     hipEvent_t ev[14] = {};
     bool have_masses = false, built = false;
+    bool next_valid = false;  // the skip pointers (nnext) match the built tree
 
     // inputs, original order
     hq::DevBuf m, x, y, z;
@@ -235,6 +236,7 @@ extern thread_local bool g_capturing;
 __global__ void k_fill_i64(int64_t* __restrict__ p, int64_t v, int64_t n);
 __global__ void k_fill_i32(int* __restrict__ p, int v, int64_t n);
 int tree_build(hq_tree* t);
+int ensure_next(hq_tree* t);  // skip pointers of the last launched build
 int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2);
 int tree_forces_groups(hq_tree* t, double theta2, double G, double eps2, int precision, bool bunched);
 int tree_forces_dehnen(hq_tree* t, double theta2, double G, double eps2);

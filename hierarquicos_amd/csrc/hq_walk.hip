@@ -1388,7 +1388,7 @@ static int ensure_forces(hq_tree* t) {
 
 int tree_forces_bh(hq_tree* t, double theta2, double G, double eps2) {
     int rc;
-    if ((rc = ensure_forces(t))) return rc;
+    if ((rc = ensure_forces(t)) || (rc = ensure_next(t))) return rc;
     hipStream_t s = t->stream;
     HQ_CHECK_HIP(hipMemsetAsync(t->status.p, 0, ST_SIZE * sizeof(int64_t), s));
     HQ_CHECK_HIP(hipEventRecord(t->ev[5], s));
