@@ -1420,68 +1420,52 @@ __global__ __launch_bounds__(256) void k_mark_groups(NodeView nv, const int* __r
 }
 
 // bunches: in every twig with more than `gs` bodies, runs of adjacent children
-// that hold <= gs bodies each are packed greedily into bunches of <= gs bodies;
-// a tree with <= gs bodies is one bunch.  Marks each bunch's first particle.
-__global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, int gs,
+// that hold <= gs bodies each are packed greedily into super-groups of <= gs
+// bodies; a tree with <= gs bodies is one super-group.  Each super-group of
+// c bodies is cut into ceil(c / B) bunches of near-equal size, and the first
+// particle of each bunch is marked.  (Super-groups tile the bodies: bodies
+// lost at max depth, in nodes of count 0, stop a bunched walk before this.)
+__global__ __launch_bounds__(256) void k_mark_bunches(NodeView nv, int64_t nn, int gs, int B,
                                                       int* __restrict__ bmark,
                                                       const int64_t* __restrict__ dnn) {
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (dnn) nn = *dnn;
     if (k >= nn) return;
+    auto emit = [&](int s0, int c) {
+        const int nb = (c + B - 1) / B;
+        if (nb <= 0) return;
+        const int base = c / nb, rem = c % nb;
+        for (int j = 0; j < nb; ++j) {
+            const int p = s0 + j * base + (j < rem ? j : rem);
+            bmark[p] = p;
+        }
+    };
     const int cnt = nv.count[k];
     if (nv.depth[k] == 0 && cnt <= gs) {  // a root (one per system) small enough to bunch whole
-        bmark[nv.first[k]] = nv.first[k];
+        emit(nv.first[k], cnt);
         return;
     }
     if (nv.type[k] != 2 || cnt <= gs) return;
-    int run = 0;
+    int run = 0, rs = 0;
     bool prev = false;
     for (int c = nv.cstart[k]; c < nv.cstart[k + 1]; ++c) {
         const int cc = nv.count[c];
         if (cc == 0 || cc > gs) {
+            if (prev) emit(rs, run);
             prev = false;
             continue;
         }
         if (prev && run + cc <= gs) {
             run += cc;
         } else {
-            bmark[nv.first[c]] = nv.first[c];
+            if (prev) emit(rs, run);
+            rs = nv.first[c];
             run = cc;
         }
         prev = true;
     }
+    if (prev) emit(rs, run);
 }
-
-// split each super-group [sg[i], sg[i+1]) into ceil(c / B) bunches of near-equal
-// size (contiguous in tree order)
-// (graph mode: nsg from the device, *dnsg, and entries nsg .. len-1 of cnt zeroed)
-__global__ __launch_bounds__(256) void k_bunch_count(const int* __restrict__ sg, int nsg, int B,
-                                                     uint32_t* __restrict__ cnt,
-                                                     const uint32_t* __restrict__ dnsg, int len) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (dnsg) {
-        nsg = (int)*dnsg;
-        if (i >= nsg && i < len) cnt[i] = 0u;
-    }
-    if (i >= nsg) return;
-    const int c = sg[i + 1] - sg[i];
-    cnt[i] = (uint32_t)((c + B - 1) / B);
-}
-
-__global__ __launch_bounds__(256) void k_bunch_emit(const int* __restrict__ sg, int nsg, int B,
-                                                    const uint32_t* __restrict__ off,
-                                                    int* __restrict__ out,
-                                                    const uint32_t* __restrict__ dnsg) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (dnsg) nsg = (int)*dnsg;
-    if (i >= nsg) return;
-    const int c = sg[i + 1] - sg[i];
-    const int nb = (c + B - 1) / B;
-    const int base = c / nb, rem = c % nb;
-    const uint32_t o = off[i];
-    for (int j = 0; j < nb; ++j) out[o + j] = sg[i] + j * base + (j < rem ? j : rem);
-}
-
 __global__ __launch_bounds__(256) void k_flag_count(const int* __restrict__ gmark, int64_t n,
                                                     uint32_t* __restrict__ bcnt) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1936,41 +1920,33 @@ int tree_make_groups(hq_tree* t, bool bunched) {
     const int64_t nnb = gm ? t->cap : t->nn;  // graph mode: the count is on the device
     const int64_t* dnn = gm ? t->gstate.as<int64_t>() + GS_NN : nullptr;
     if (bunched)
-        k_mark_bunches<<<grid_for(nnb, 256), 256, 0, s>>>(node_view(t), nnb, S,
+        k_mark_bunches<<<grid_for(nnb, 256), 256, 0, s>>>(node_view(t), nnb, S, B,
                                                          t->gflag.as<int>(), dnn);
     else
         k_mark_groups<<<grid_for(t->nn, 256), 256, 0, s>>>(node_view(t), t->nparent.as<int>(),
                                                            t->nn, t->group_size, t->gflag.as<int>());
     const int nb = grid_for(n, 256);
     if ((rc = t->hist.ensure(((size_t)nb + 1) * sizeof(uint32_t)))) return rc;
+    if (bunched && (rc = t->bstart.ensure(((size_t)n + 1) * sizeof(int)))) return rc;
+    // the marked first particles in order: group starts, or bunch starts
+    int* out = bunched ? t->bstart.as<int>() : t->groups.as<int>();
     k_flag_count<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>());
     if ((rc = scan_excl_buf(t->hist.as<uint32_t>(), nb, t->scanpart, s))) return rc;
-    k_flag_write<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>(),
-                                    t->groups.as<int>());
+    k_flag_write<<<nb, 256, 0, s>>>(t->gflag.as<int>(), n, t->hist.as<uint32_t>(), out);
     if (bunched) {
-        // super-groups -> bunches with the counts on the device: the super-group
-        // count bounded by n, the bunch table by n + 1 (one read-back at the
-        // end when launched, none in graph mode)
-        int* sg = t->groups.as<int>();
-        const uint32_t* dnsg = t->hist.as<uint32_t>() + nb;
-        if ((rc = t->gflag.ensure(((size_t)n + 1) * sizeof(int))) ||
-            (rc = t->bstart.ensure(((size_t)n + 1) * sizeof(int))))
-            return rc;
-        uint32_t* cnt = reinterpret_cast<uint32_t*>(t->gflag.as<int>());
-        k_set_end<<<1, 64, 0, s>>>(sg, dnsg, (int)n);
-        k_bunch_count<<<grid_for(n, 256), 256, 0, s>>>(sg, 0, B, cnt, dnsg, (int)n);
-        if ((rc = scan_excl_buf(cnt, n, t->scanpart, s))) return rc;
-        k_bunch_emit<<<grid_for(n, 256), 256, 0, s>>>(sg, 0, B, cnt, t->bstart.as<int>(), dnsg);
-        k_set_end<<<1, 64, 0, s>>>(t->bstart.as<int>(), cnt + n, (int)n);
+        // the bunch count stays on the device (one read-back at the end when
+        // launched, none in graph mode); the table ends with n
+        const uint32_t* dnb = t->hist.as<uint32_t>() + nb;
+        k_set_end<<<1, 64, 0, s>>>(t->bstart.as<int>(), dnb, (int)n);
         if (gm) {
-            k_graph_bunches<<<1, 64, 0, s>>>(cnt + n, t->graph_nbun, t->gstate.as<int64_t>());
+            k_graph_bunches<<<1, 64, 0, s>>>(dnb, t->graph_nbun, t->gstate.as<int64_t>());
             HQ_CHECK_HIP(hipGetLastError());
             return HQ_OK;
         }
         // the count comes back through a pinned slot; the caller may queue
         // work that does not need it (k_mom32) before finish_groups waits
         uint32_t* slot = reinterpret_cast<uint32_t*>(t->hstatus + ST_HOST_BUNCHES);
-        HQ_CHECK_HIP(hipMemcpyAsync(slot, cnt + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HQ_CHECK_HIP(hipMemcpyAsync(slot, dnb, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HQ_CHECK_HIP(hipEventRecord(t->ev[12], s));
         t->groups_pending = true;
         return HQ_OK;
