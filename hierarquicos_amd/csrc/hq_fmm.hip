@@ -106,7 +106,7 @@ struct FmmState {
     FmmTables tab;
     DevBuf flag, scan, fid, fnode, Qd, Qf, Ld, Lf, rmax, lvl;
     DevBuf rec, code, fa, fb, cnt, lat, gen, p2p, m2p, blist, gnb, ghist, gpos, gpair, lsrc, lcol;
-    DevBuf phist, ppos, plist;
+    DevBuf phist, ppos, plist, edges;
     int64_t nf = 0;
     std::vector<int> fls;  // FMM node range per depth
 };
@@ -373,6 +373,15 @@ __global__ __launch_bounds__(256) void k_fmm_qf(int64_t nf, const double* __rest
 // ------------------------------------------------------------------ dual walk
 enum { ACT_EXPAND = 0, ACT_LATTICE = 1, ACT_GENERIC = 2, ACT_P2P = 3, ACT_M2P = 4, ACT_NONE = 5 };
 constexpr int NACT = 5;
+
+// The dual walk's per-category totals (the category starts of the scanned
+// [category][block] counts, then the grand total), gathered into one small
+// array so that they return to the host in one copy.
+__global__ void k_fmm_edges(const uint32_t* __restrict__ cnt, int nblk, uint32_t* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q <= NACT) out[q] = cnt[(int64_t)q * (nblk + 1)];
+}
+
 
 struct WalkPar {
     double theta;
@@ -1531,7 +1540,9 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     // the lists keep their capacity from one evaluation to the next
     size_t lat_cap = st.lat.bytes / sizeof(int3), gen_cap = st.gen.bytes / sizeof(int2),
            p2p_cap = st.p2p.bytes / sizeof(int2), m2p_cap = st.m2p.bytes / sizeof(int2);
-    if ((rc = st.fa.ensure(std::max<int64_t>(nfr, 1024) * sizeof(int2)))) return rc;
+    if ((rc = st.fa.ensure(std::max<int64_t>(nfr, 1024) * sizeof(int2))) ||
+        (rc = st.edges.ensure(8 * sizeof(uint32_t))))
+        return rc;
     k_fmm_roots<<<grid_for(nfr, 256), 256, 0, s>>>((int)nfr, st.fid.as<int>(), st.fa.as<int2>());
     DevBuf* cur = &st.fa;
     DevBuf* nxt = &st.fb;
@@ -1560,10 +1571,12 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
                                              st.code.as<uint32_t>(), cnt, nblk);
         // one scan over [category][block]: category q's slice starts at its base
         if ((rc = scan_excl_buf(cnt, (int64_t)NACT * (nblk + 1), t->scanpart, s))) return rc;
-        uint32_t edge[NACT + 1];
-        for (int q = 0; q <= NACT; ++q)
-            HQ_CHECK_HIP(hipMemcpyAsync(&edge[q], cnt + (int64_t)q * (nblk + 1), 4,
-                                        hipMemcpyDeviceToHost, s));
+        // the totals come back in one copy, into host-only slots of the
+        // handle's pinned status block
+        uint32_t* edge = reinterpret_cast<uint32_t*>(t->hstatus + ST_HOST_FMM);
+        k_fmm_edges<<<1, 64, 0, s>>>(cnt, nblk, st.edges.as<uint32_t>());
+        HQ_CHECK_HIP(hipMemcpyAsync(edge, st.edges.p, (NACT + 1) * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, s));
         HQ_CHECK_HIP(hipStreamSynchronize(s));
         uint32_t tot[NACT];
         for (int q = 0; q < NACT; ++q) tot[q] = edge[q + 1] - edge[q];

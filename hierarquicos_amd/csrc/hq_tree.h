@@ -145,6 +145,7 @@ enum {
     ST_SIZE = 48,
     // host-only slots after the status block (the status copy leaves them alone)
     ST_HOST_BUNCHES = 48,  // bunch count, read back while k_mom32 runs
+    ST_HOST_FMM = 50,      // FMM dual walk: per-category totals (6 uint32, slots 50..52)
     ST_HOST_SIZE = 56
 };
 
