@@ -8,6 +8,9 @@ the build (SURVEY's failure handling: capacity, depth, ties, lost bodies).
 * Many bodies sharing all 21 levels of the high key word: the build sorts by
   the full 25-level key and matches the oracle's insertion tree (the
   reference drops bodies that still share a cell at depth 25, :570-579).
+* Runs of bodies equal in the sorted bits of the key (a cube 2^-16 of the
+  root wide) but separated at levels 17-21: the tie pass, or for a long run
+  the full-key sort, gives the oracle's tree.
 * Morton depth outside 1..21 or keys not distinct at 21 levels: HQ_ERR_DEPTH.
 """
 import numpy as np
@@ -75,6 +78,49 @@ def test_long_runs_of_shared_high_keys(ncluster, side):
     ok = np.all(np.isfinite(Fo), axis=1) & (np.linalg.norm(np.nan_to_num(Fo), axis=1) > 0)
     rel = np.linalg.norm(F - Fo, axis=1)[ok] / np.linalg.norm(Fo, axis=1)[ok]
     assert ok.sum() > 2000 and float(rel.max()) < 1e-10
+
+
+@pytest.mark.parametrize("ncluster", [40, 200])
+def test_runs_sharing_the_sorted_key_bits(ncluster):
+    """The key sort orders the high word above its lowest 16 bits (levels
+    1-16 and a third of 17) and the tie pass orders runs equal there by the
+    rest of the key.  A cluster inside one cube 2^-16 of the root wide whose
+    bodies separate at levels 17-21 makes such a run: 40 bodies take the tie
+    pass, 200 the full-key sort.  Either way topology, moments and FP64 forces
+    equal the oracle's."""
+    rng = np.random.default_rng(ncluster + 1)
+    m0, q0, _ = ics.plummer(2000, seed=7)
+    # the middle half of the depth-16 cube around (0.3, 0.3, 0.3) of the root
+    # cube the build derives (centre of the bounding box, side 1.2 x extent)
+    lo, hi = q0.min(0), q0.max(0)
+    side = 1.2 * float(np.max(hi - lo))
+    corner = 0.5 * (lo + hi) - 0.5 * side
+    cell = side / 2.0**16
+    base = corner + np.floor((0.3 - corner) / cell) * cell
+    qc = base + cell * (0.25 + 0.5 * rng.random((ncluster, 3)))
+    q = np.concatenate([q0, qc])
+    m = np.full(len(q), 1.0 / len(q))
+    x, y, z = (np.ascontiguousarray(q[:, k]) for k in range(3))
+    t = OctreeType().pre_init(m, 12).init(x, y, z)
+    e = t.export()
+    r = O.Tree(m, 12).init(x, y, z).export()
+    cg = np.lexsort((e["key_lo"], e["key_hi"], e["depth"]))
+    cr = np.lexsort((r["klo"], r["khi"], r["depth"]))
+    assert len(cg) == len(cr)
+    for a, b in (("depth", "depth"), ("key_hi", "khi"), ("key_lo", "klo"), ("type", "type"),
+                 ("particle", "particle"), ("mass", "mass"), ("quad", "quad"), ("oct", "oct")):
+        assert np.array_equal(np.asarray(e[a])[cg], np.asarray(r[b])[cr]), a
+    # the cluster shares the sorted bits of the high word and separates below
+    leaves = np.asarray(e["type"]) == 1
+    own = np.isin(np.asarray(e["particle"]), np.arange(2000, 2000 + ncluster)) & leaves
+    khi = np.asarray(e["key_hi"], dtype=np.uint64)[own]
+    d = np.asarray(e["depth"])[own]
+    assert len(np.unique(khi >> np.uint64(16))) == 1 and d.min() >= 17
+    assert t.info()["lost"] == 0
+    F = t.forces_all(0.25, 1.0, 1e-6).T
+    Fo = O.Tree(m, 12).init(x, y, z).bh_forces(0.25, 1.0, 1e-6)
+    rel = np.linalg.norm(F - Fo, axis=1) / np.linalg.norm(Fo, axis=1)
+    assert float(rel.max()) < 1e-10
 
 
 def test_morton_depth_errors():
