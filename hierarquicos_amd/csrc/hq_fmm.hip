@@ -592,14 +592,29 @@ __device__ __forceinline__ RunInfo run_of(int key, int64_t base, int64_t n) {
     return RunInfo{valid, head, next - l, h};
 }
 
+// The binning kernels below take RUN_U wavefront-chunks of 64 entries per
+// loop round: the round's loads are issued before its atomics, so that each
+// thread has RUN_U loads in flight.
+constexpr int RUN_U = 4;
+
 __global__ __launch_bounds__(256) void k_fmm_lat_ghist(const int3* __restrict__ lat, int64_t n, int gt,
                                                        uint32_t* __restrict__ ghist) {
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + __lane_id();
-        const int key = i < n ? lat[i].x / gt : -1;
-        const RunInfo r = run_of(key, base, n);
-        if (r.head) atomicAdd(&ghist[key], (uint32_t)r.count);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base0 < n;
+         base0 += stride * RUN_U) {
+        int key[RUN_U];
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t i = base0 + u * stride + __lane_id();
+            key[u] = i < n ? lat[i].x / gt : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t base = base0 + u * stride;  // the same in the whole wavefront
+            if (base >= n) break;
+            const RunInfo r = run_of(key[u], base, n);
+            if (r.head) atomicAdd(&ghist[key[u]], (uint32_t)r.count);
+        }
     }
 }
 
@@ -607,16 +622,27 @@ __global__ __launch_bounds__(256) void k_fmm_lat_ghist(const int3* __restrict__ 
 __global__ __launch_bounds__(256) void k_fmm_lat_gscatter(const int3* __restrict__ lat, int64_t n,
                                                           int gt, uint32_t* __restrict__ gpos,
                                                           int2* __restrict__ out) {
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + __lane_id();
-        const int3 e = i < n ? lat[i] : make_int3(0, 0, 0);
-        const int key = i < n ? e.x / gt : -1;
-        const RunInfo r = run_of(key, base, n);
-        uint32_t o = 0;
-        if (r.head) o = atomicAdd(&gpos[key], (uint32_t)r.count);
-        o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
-        if (r.valid) out[o] = make_int2(e.y, (int)(((uint32_t)(e.x - key * gt) << 24) | (uint32_t)e.z));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base0 < n;
+         base0 += stride * RUN_U) {
+        int3 e[RUN_U];
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t i = base0 + u * stride + __lane_id();
+            e[u] = i < n ? lat[i] : make_int3(0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t base = base0 + u * stride;
+            if (base >= n) break;
+            const int key = base + __lane_id() < n ? e[u].x / gt : -1;
+            const RunInfo r = run_of(key, base, n);
+            uint32_t o = 0;
+            if (r.head) o = atomicAdd(&gpos[key], (uint32_t)r.count);
+            o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
+            if (r.valid)
+                out[o] = make_int2(e[u].y, (int)(((uint32_t)(e[u].x - key * gt) << 24) | (uint32_t)e[u].z));
+        }
     }
 }
 
@@ -627,6 +653,7 @@ __global__ __launch_bounds__(256) void k_fmm_lat_gscatter(const int3* __restrict
 // last at gstart[g+1]), and the pairs placed by offset.
 constexpr int LSORT_T = 1024;
 constexpr int LSORT_CH = 32768;
+constexpr int LSORT_U = 8;
 
 __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restrict__ gp,
                                                            const uint32_t* __restrict__ gstart,
@@ -644,9 +671,18 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
         const int nc = min(LSORT_CH, noff - c0);
         for (int o = tid; o < nc; o += LSORT_T) h[o] = 0;
         __syncthreads();
-        for (uint32_t e = s0 + tid; e < s1; e += LSORT_T) {
-            const int o = (int)((uint32_t)gp[e].y & 0xFFFFFFu) - c0;
-            if (o >= 0 && o < nc) atomicAdd(&h[o], 1u);
+        // LSORT_U pairs per thread in flight: the loads of a round are
+        // issued before its atomics
+        for (uint32_t e0 = s0 + tid; e0 < s1; e0 += LSORT_T * LSORT_U) {
+            int o[LSORT_U];
+#pragma unroll
+            for (int u = 0; u < LSORT_U; ++u) {
+                const uint32_t e = e0 + u * LSORT_T;
+                o[u] = e < s1 ? (int)((uint32_t)gp[e].y & 0xFFFFFFu) - c0 : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < LSORT_U; ++u)
+                if (o[u] >= 0 && o[u] < nc) atomicAdd(&h[o[u]], 1u);
         }
         __syncthreads();
         // exclusive scans of the counts and of the non-empty flags of
@@ -668,13 +704,24 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
             run += c;
         }
         __syncthreads();
-        for (uint32_t e = s0 + tid; e < s1; e += LSORT_T) {
-            const int2 v = gp[e];
-            const int o = (int)((uint32_t)v.y & 0xFFFFFFu) - c0;
-            if (o >= 0 && o < nc) {
-                const uint32_t p = s0 + atomicAdd(&h[o], 1u);
-                src[p] = v.x;
-                col[p] = (int16_t)((uint32_t)v.y >> 24);
+        for (uint32_t e0 = s0 + tid; e0 < s1; e0 += LSORT_T * LSORT_U) {
+            int2 v[LSORT_U];
+#pragma unroll
+            for (int u = 0; u < LSORT_U; ++u) {
+                const uint32_t e = e0 + u * LSORT_T;
+                v[u] = e < s1 ? gp[e] : make_int2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < LSORT_U; ++u) {
+                // (column << 24 | offset) may have its sign bit set: the
+                // pair's place decides whether it is one
+                const bool ok = e0 + u * LSORT_T < s1;
+                const int o = ok ? (int)((uint32_t)v[u].y & 0xFFFFFFu) - c0 : -1;
+                if (o >= 0 && o < nc) {
+                    const uint32_t p = s0 + atomicAdd(&h[o], 1u);
+                    src[p] = v[u].x;
+                    col[p] = (int16_t)((uint32_t)v[u].y >> 24);
+                }
             }
         }
         done += tot;
@@ -686,27 +733,47 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
 
 __global__ __launch_bounds__(256) void k_fmm_p2p_hist(const int2* __restrict__ pr, int64_t n,
                                                       uint32_t* __restrict__ hist) {
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + __lane_id();
-        const int key = i < n ? pr[i].x : -1;
-        const RunInfo r = run_of(key, base, n);
-        if (r.head) atomicAdd(&hist[key], (uint32_t)r.count);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base0 < n;
+         base0 += stride * RUN_U) {
+        int key[RUN_U];
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t i = base0 + u * stride + __lane_id();
+            key[u] = i < n ? pr[i].x : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t base = base0 + u * stride;
+            if (base >= n) break;
+            const RunInfo r = run_of(key[u], base, n);
+            if (r.head) atomicAdd(&hist[key[u]], (uint32_t)r.count);
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict__ pr, int64_t n,
                                                          uint32_t* __restrict__ pos,
                                                          int* __restrict__ out) {
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + __lane_id();
-        const int2 e = i < n ? pr[i] : make_int2(-1, 0);
-        const RunInfo r = run_of(e.x, base, n);
-        uint32_t o = 0;
-        if (r.head) o = atomicAdd(&pos[e.x], (uint32_t)r.count);
-        o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
-        if (r.valid) out[o] = e.y;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base0 < n;
+         base0 += stride * RUN_U) {
+        int2 e[RUN_U];
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t i = base0 + u * stride + __lane_id();
+            e[u] = i < n ? pr[i] : make_int2(-1, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < RUN_U; ++u) {
+            const int64_t base = base0 + u * stride;
+            if (base >= n) break;
+            const RunInfo r = run_of(e[u].x, base, n);
+            uint32_t o = 0;
+            if (r.head) o = atomicAdd(&pos[e[u].x], (uint32_t)r.count);
+            o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
+            if (r.valid) out[o] = e[u].y;
+        }
     }
 }
 
