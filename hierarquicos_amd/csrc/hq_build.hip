@@ -694,17 +694,6 @@ __host__ __device__ inline int up_segsh(int64_t nodes, int64_t kids) {
     return sh;
 }
 
-// Inclusive prefix sum over the wavefront's lanes.
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
 // Moments of the nodes of one level, one 64-thread workgroup (one wavefront)
 // per segment of 2^segsh consecutive nodes.  A segment is done in rounds: a
 // round takes the longest run of its remaining nodes with at most UP_KIDS

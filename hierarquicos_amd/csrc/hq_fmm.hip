@@ -522,6 +522,8 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ r
                                                       int2* __restrict__ p2p, int64_t p2p0,
                                                       int2* __restrict__ m2p, int64_t m2p0) {
     __shared__ uint32_t ws[NACT][WB / 64];
+    __shared__ uint32_t xstart[WB / 64][64];
+    __shared__ int4 xpar[WB / 64][64];
     const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
     uint32_t c = ACT_NONE;
     int a = 0, b = 0;
@@ -546,15 +548,50 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ r
         for (int k = 0; k < wv; ++k) base += ws[q][k];
         pre[q] += base;
     }
-    if (act == ACT_EXPAND) {
-        const int sp = (int)((c >> 3) & 3u);
-        int a0 = a, a1 = a + 1, b0 = b, b1 = b + 1;
-        if (sp & 1) a0 = rec[a].kid0, a1 = rec[a].kid1;
-        if (sp & 2) b0 = rec[b].kid0, b1 = rec[b].kid1;
-        int64_t o = pre[0];
-        for (int x = a0; x < a1; ++x)
-            for (int y = b0; y < b1; ++y) next[o++] = make_int2(x, y);
-    } else if (act == ACT_LATTICE) {
+    // a split's child pairs: the wavefront writes them together, consecutive
+    // lanes at consecutive places (a pair may have up to 64 children): lane
+    // j's pairs are [ex[j], ex[j] + cv) of the wavefront's run, which starts
+    // at pre[0] of its first lane
+    {
+        const int lane = __lane_id();
+        int a0 = a, b0 = b, nbk = 1;
+        if (act == ACT_EXPAND) {
+            const int sp = (int)((c >> 3) & 3u);
+            if (sp & 1) a0 = rec[a].kid0;
+            if (sp & 2) {
+                b0 = rec[b].kid0;
+                nbk = rec[b].kid1 - b0;
+            }
+        }
+        const uint32_t ne = act == ACT_EXPAND ? cv : 0u;
+        const uint32_t incl = wave_incl_scan(ne);
+        const uint32_t tot = __shfl(incl, 63, 64);
+        if (tot > 0) {
+            uint32_t* xs = &xstart[wv][0];
+            int4* xp = &xpar[wv][0];
+            xs[lane] = incl - ne;
+            xp[lane] = make_int4(a0, b0, nbk, (int)ne);
+            const int64_t o0 = (int64_t)__shfl((int)(pre[0] - (incl - ne)), 0, 64);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t k = lane; k < tot; k += 64) {
+                // the last lane whose run starts at or before k (lanes with no
+                // children have empty runs and are skipped)
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (xs[mid] <= k) lo = mid;
+                    else hi = mid - 1;
+                }
+                while (xp[lo].w == 0 || xs[lo] + (uint32_t)xp[lo].w <= k) --lo;
+                const int4 q = xp[lo];
+                const int r = (int)(k - xs[lo]);
+                next[o0 + k] = make_int2(q.x + r / q.z, q.y + r % q.z);
+            }
+        }
+    }
+    if (act == ACT_LATTICE) {
         lat[lat0 + pre[1]] = make_int3(a, b, (int)(c >> 3));
     } else if (act == ACT_GENERIC) {
         gen[gen0 + pre[2]] = make_int2(a, b);
