@@ -1369,6 +1369,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
                                                  const double* __restrict__ sy,
                                                  const double* __restrict__ sz,
                                                  const double* __restrict__ sm,
+                                                 const double4* __restrict__ sb,
                                                  const int* __restrict__ mi,
                                                  const double* __restrict__ Ld,
                                                  const uint32_t* __restrict__ pstart,
@@ -1438,11 +1439,12 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
             if (j < take)
                 for (int b = ql; b < jc && jp + b < NB; b += 16) {
                     const int jj = jf + b;
+                    const double4 r = sb[jj];  // the sorted body as one (x, y, z, m) record
                     if constexpr (F32)
-                        sbuf[jp + b] = make_float4((float)(sx[jj] - c[0]), (float)(sy[jj] - c[1]),
-                                                   (float)(sz[jj] - c[2]), (float)sm[jj]);
+                        sbuf[jp + b] = make_float4((float)(r.x - c[0]), (float)(r.y - c[1]),
+                                                   (float)(r.z - c[2]), (float)r.w);
                     else
-                        sbuf[jp + b] = make_double4(sx[jj], sy[jj], sz[jj], sm[jj]);
+                        sbuf[jp + b] = r;
                     sidx[jp + b] = jj;
                 }
         }
@@ -1785,7 +1787,8 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     if ((rc = t->fx.ensure(b8)) || (rc = t->fy.ensure(b8)) || (rc = t->fz.ensure(b8))) return rc;
     auto leaf = [&](auto kern) {
         kern<<<(int)nf, 64, 0, s>>>((int)nf, v, t->sx.as<double>(), t->sy.as<double>(),
-                                    t->sz.as<double>(), t->sm.as<double>(), tb.mi.as<int>(),
+                                    t->sz.as<double>(), t->sm.as<double>(),
+                                    t->sbody.as<double4>(), tb.mi.as<int>(),
                                     st.Ld.as<double>(), st.phist.as<uint32_t>(), st.plist.as<int>(),
                                     G, eps2, t->fx.as<double>(), t->fy.as<double>(),
                                     t->fz.as<double>());
