@@ -39,14 +39,12 @@ __host__ __device__ constexpr int kpad(int p) { return (ncoef(p) + 15) / 16 * 16
 // the M2L contraction length: the moments padded to the MFMA's k-step of 4
 // (rows of the operator and the locals are padded to 16 instead)
 __host__ __device__ constexpr int kq(int p) { return (ncoef(p) + 3) / 4 * 4; }
-// M2L workgroup of M2L_WAVES wavefronts and group_of(p) target cubes whose FP32
-// locals live in LDS.  At order 4 each wavefront keeps its own copy of them
-// (plain adds, no atomics; 8 x 128 targets x 36 floats = 144 KiB).  At order 6
-// the operators are 4.6x larger, so groups stay at 256 targets for their reuse,
-// and the wavefronts share one copy through LDS atomic adds.
-constexpr int M2L_WAVES = 8;
-__host__ __device__ constexpr bool m2l_private(int p) { return p <= 4; }
-__host__ __device__ constexpr int group_of(int p) { return m2l_private(p) ? 128 : 256; }
+// Target cubes per group of the matrix-core M2L, whose FP32 locals live in
+// LDS: at order 4 each wavefront keeps its own copy of them (k_fmm_m2l4), at
+// order 6 each wavefront owns 16 rows of one shared copy (k_fmm_m2l6).  128
+// at both; at order 6, 96 took 74 ms and 192 took 79 ms against 67 (4M cube).
+constexpr int M2L6_GT = 128;
+__host__ __device__ constexpr int group_of(int p) { return p <= 4 ? 128 : M2L6_GT; }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -815,46 +813,20 @@ __global__ __launch_bounds__(256) void k_fmm_p2p_scatter(const int2* __restrict_
 }
 
 // ------------------------------------------------------------------ M2L
-// Lattice M2L on the matrix cores.  One workgroup per group of GT target
-// cubes keeps their FP32 locals in LDS.  The group's pairs are binned by
-// operator; wavefront w takes the non-empty bins w, w+8, ..., loads the bin's
-// operator T (Kp x Kq) into registers once, multiplies it with the bin's
-// source moments 16 columns at a time by v_mfma_f32_16x16x4f32 and adds each
-// column into its target's locals.  With private copies (order 4) that add is
-// a plain 16-byte read-add-write in the wavefront's own copy: a target appears
-// once per bin and no other wavefront writes that copy; the copies are summed
-// in wavefront order at the end.  LDS float atomics, which the shared copy of
-// order 6 uses, cost about 135 cycles per instruction.
-// Adds one tile's products into the group's FP32 locals in LDS: column li of
-// the tile is target c (c < 0: an empty column), this lane's rows
-// 16 rt + 4 lk .. + 3.  A private copy takes a plain 16-byte read-add-write
-// (rows at or past Kq are operator padding); the shared copy atomic adds.
-template <bool PRIV, int RT, int Kq, int K, int LS>
-__device__ __forceinline__ void m2l_accumulate(float* mine, int c, int lk, const floatx4* d) {
-    if (c < 0) return;
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-        const int row = 16 * rt + 4 * lk;
-        if (PRIV) {
-            if (row < Kq) {
-                float4* a = reinterpret_cast<float4*>(mine + c * LS + row);
-                float4 v = *a;
-                v.x += d[rt][0];
-                v.y += d[rt][1];
-                v.z += d[rt][2];
-                v.w += d[rt][3];
-                *a = v;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (row + r < K) atomicAdd(&mine[c * LS + row + r], d[rt][r]);
-        }
-    }
-}
+// Lattice M2L on the matrix cores.  One workgroup per group of target cubes
+// keeps their FP32 locals in LDS.  The group's pairs are binned by operator
+// (offset); a bin's product D = T_o Q runs 16 columns (pairs) at a time on
+// v_mfma_f32_16x16x4f32 and each column is added into its target's locals.
+// A target appears once per bin, so the adds need no atomics as long as no
+// two wavefronts write the same locals at once: at order 4 each wavefront
+// takes its own bins and keeps a private copy of the locals (summed at the
+// end); at order 6 (too large for private copies) every wavefront takes
+// every tile but only its own 16 rows of the product.  (LDS float atomics
+// into one shared copy cost about 135 cycles per instruction.)
 
-// The same for a private copy in two halves: the reads are issued first, the
-// adds and writes later, so that the read latency overlaps other work.
+// Adds a tile's column into a private copy in two halves: the reads are
+// issued first, the adds and writes later, so that the read latency
+// overlaps other work.
 template <int RT, int Kq, int LS>
 __device__ __forceinline__ void m2l_acc_read(const float* mine, int c, int lk, float4* v) {
     if (c < 0) return;
@@ -1145,111 +1117,126 @@ __global__ __launch_bounds__(64 * M2L4_WAVES) void k_fmm_m2l4(
 // j + 16k, D[4q + r][j] in lane j + 16q, element r.  The contraction index of
 // step kt in lane group lk is lk * KT + kt, so that each lane reads KT
 // consecutive moments of its source (wide loads).
+
+// Order-6 lattice M2L on the matrix cores.  One workgroup of RT = 6
+// wavefronts per group of M2L6_GT target cubes (with nsplit > 1, per share of
+// the group's bins); all six follow the same stream of tiles of 16 pairs
+// (bins in offset order), and wavefront r computes rows 16 r .. 16 r + 15 of
+// each tile's product D = T_o Q (96 x 16, contraction 84) with
+// v_mfma_f32_16x16x4f32.  The wavefronts thus own disjoint rows of the
+// group's FP32 locals in LDS and add their rows with plain read-add-writes,
+// no atomics (the LDS float atomics of a shared copy kept the LDS busy 87% of
+// the time).  The tile's 16 source records are loaded once for all six into
+// LDS, double-buffered: the next tile's records are fetched while the
+// current one runs.  A wavefront's operator rows (21 values per lane) stay
+// in registers over a bin; the next bin's are read from the operator table
+// one tile ahead.  (Building them from the offset's 455 Taylor coefficients
+// staged in LDS, as at order 4, took 85 ms instead of 67 at 4M: the build
+// then sits between the barrier and the products.)
 template <int P>
-__global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
+__global__ __launch_bounds__(64 * (kpad(P) / 16)) void k_fmm_m2l6(
     int64_t nf, int noff, int nsplit, const int2* __restrict__ blist,
     const uint32_t* __restrict__ gnb, const uint32_t* __restrict__ gstart,
     const int* __restrict__ src, const int16_t* __restrict__ col, const float* __restrict__ T,
     const float* __restrict__ Qf, float* __restrict__ Lf) {
     constexpr int K = ncoef(P), Kp = kpad(P), Kq = kq(P), RT = Kp / 16, KT = Kq / 4;
-    constexpr int NW = M2L_WAVES, NT = 64 * NW, GT = group_of(P);
-    constexpr bool PRIV = m2l_private(P);
-    constexpr int NC = PRIV ? NW : 1;      // copies of the locals
-    constexpr int LS = PRIV ? Kq : K;      // floats per target in a copy
-    constexpr int TC = 8;                  // tiles of a bin whose sources are loaded together
-    __shared__ float4 acc4[(NC * GT * LS + 3) / 4];
-    float* acc = reinterpret_cast<float*>(acc4);
-    // block (g, sp): target group g, share sp of its bins; with several
-    // shares per group the partial sums meet in Lf by atomic adds
+    constexpr int NT = 64 * RT, GT = M2L6_GT, KQ4 = Kq / 4, NS = 16 * KQ4;
+    static_assert(K == Kq && K % 4 == 0 && NS <= NT, "whole float4 rows; one record slice per thread");
+    __shared__ float4 loc4[GT * K / 4];
+    __shared__ float4 bst4[2][NS];
+    __shared__ int bcol[2][16];
+    float* loc = reinterpret_cast<float*>(loc4);
+    const float* bstf = reinterpret_cast<const float*>(&bst4[0][0]);
     const int g = blockIdx.x / nsplit, sp = blockIdx.x % nsplit;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int rt = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    for (int i = tid; i < (NC * GT * LS + 3) / 4; i += NT) acc4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    __syncthreads();
-    float* __restrict__ mine = acc + (PRIV ? wv * GT * LS : 0);
-    // this workgroup's share of the group's non-empty bins; bin k of it
-    // belongs to wavefront k % NW
-    const int2* list = blist + (int64_t)g * noff;
-    const int nbg = (int)gnb[g];
-    const uint32_t s1 = gstart[g + 1];
-    const int kb = (int)((int64_t)nbg * sp / nsplit), ke = (int)((int64_t)nbg * (sp + 1) / nsplit);
-    float av[RT][KT];
-    for (int k = kb + wv; k < ke; k += NW) {
-        const int o = list[k].x;
-        const uint32_t b0 = (uint32_t)list[k].y, b1 = k + 1 < nbg ? (uint32_t)list[k + 1].y : s1;
-        const int nt = (int)((b1 - b0 + 15) / 16);
-        const float* to = T + (int64_t)o * Kp * Kq;
+    const int sj = tid / KQ4, sq = tid % KQ4;  // staging: this thread's column and record slice
+    for (int i = tid; i < GT * K / 4; i += NT) loc4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int nbg = __builtin_amdgcn_readfirstlane((int)gnb[g]);
+    M2lStream st;
+    st.list = blist + (int64_t)g * noff;
+    st.k = (int)((int64_t)nbg * sp / nsplit);
+    st.k1 = (int)((int64_t)nbg * (sp + 1) / nsplit);
+    st.nbg = nbg;
+    st.s1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)gstart[g + 1]);
+    st.b0 = st.b1 = 0u;
+    st.o = 0;
+    st.t = st.nt = 0;
+    // the staging thread's pair of tile t, or -1 (not a staging thread, past
+    // the bin or past the stream)
+    auto pair_of = [&](const M2lTile& t) -> int64_t {
+        const uint32_t e = t.e0 + (uint32_t)sj;
+        return (tid < NS && t.o >= 0 && e < t.b1) ? (int64_t)e : -1;
+    };
+    // an empty column reads the zero record nf
+    auto load_src = [&](const M2lTile& t) -> int {
+        const int64_t e = pair_of(t);
+        return e >= 0 ? src[e] : (int)nf;
+    };
+    auto load_rec = [&](int sid) -> float4 {
+        return reinterpret_cast<const float4*>(Qf + (int64_t)sid * Kq)[tid < NS ? sq : 0];
+    };
+    auto load_col = [&](const M2lTile& t) -> int {
+        const int64_t e = pair_of(t);
+        return e >= 0 ? (int)col[e] : -1;
+    };
+    // this wavefront's operator rows of offset o: lane (li, lk) holds
+    // T[16 rt + li][lk KT + kt], kt < KT
+    auto load_op = [&](int o, float (&a)[KT]) {
+        const float* to = T + ((int64_t)o * Kp + 16 * rt + li) * Kq + lk * KT;
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
+        for (int kt = 0; kt < KT; ++kt) a[kt] = to[kt];
+    };
+    M2lTile t0 = st.next(), t1 = st.next(), t2 = st.next();
+    float av[KT], an[KT];
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) av[rt][kt] = to[(16 * rt + li) * Kq + lk * KT + kt];
-        for (int t0 = 0; t0 < nt; t0 += TC) {
-            // this lane's source and column in each tile of the chunk,
-            // loaded together; an empty column reads the zero record nf
-            int sr[TC], cr[TC];
+    for (int kt = 0; kt < KT; ++kt) av[kt] = an[kt] = 0.0f;
+    if (t0.o >= 0) load_op(t0.o, av);
+    {
+        const int s0 = load_src(t0);
+        const int c0 = load_col(t0);
+        const float4 r0 = load_rec(s0);
+        if (tid < NS) bst4[0][tid] = r0;
+        if (tid < NS && sq == 0) bcol[0][sj] = c0;
+    }
+    int s1 = load_src(t1);
+    int b = 0;
+    while (t0.o >= 0) {
+        // the next tile's records and column (and operator rows at a bin
+        // start), the source ids of the tile after it
+        const float4 r1 = load_rec(s1);
+        const int c1 = load_col(t1);
+        const int s2 = load_src(t2);
+        if (t1.o >= 0 && t1.first) load_op(t1.o, an);
+        __syncthreads();  // buffer b holds tile t0; buffer 1-b is no longer read
+        floatx4 d = {0.f, 0.f, 0.f, 0.f};
+        const float* bq = bstf + b * NS * 4 + li * Kq + lk * KT;
 #pragma unroll
-            for (int u = 0; u < TC; ++u) {
-                const uint32_t e = b0 + (uint32_t)((t0 + u) * 16 + li);
-                const bool okc = e < b1;
-                const uint32_t ee = okc ? e : b0;
-                const int sv = src[ee], cv = col[ee];
-                sr[u] = okc ? sv : (int)nf;
-                cr[u] = okc ? cv : -1;
-            }
-            const int ntc = min(TC, nt - t0);
-            // tile u's moments are fetched during tile u - 1, and tile
-            // u - 1's sums are added into the locals while tile u's
-            // matrix products run
-            float bv[KT];
-            {
-                const float* q = Qf + (int64_t)sr[0] * Kq + lk * KT;
-#pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bv[kt] = q[kt];
-            }
-            floatx4 dp[RT];
-            int cp = -1;
-#pragma unroll
-            for (int u = 0; u < TC; ++u) {
-                if (u >= ntc) break;
-                float bn[KT];
-                {
-                    const float* q = Qf + (int64_t)sr[u + 1 < TC ? u + 1 : u] * Kq + lk * KT;
-#pragma unroll
-                    for (int kt = 0; kt < KT; ++kt) bn[kt] = q[kt];
-                }
-                // the loads above are issued before the products start
-                __builtin_amdgcn_sched_barrier(0);
-                float4 pv[RT];
-                if (PRIV && u > 0) m2l_acc_read<RT, Kq, LS>(mine, cp, lk, pv);
-                floatx4 d[RT];
-#pragma unroll
-                for (int rt = 0; rt < RT; ++rt) d[rt] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kt = 0; kt < KT / 2; ++kt)
-#pragma unroll
-                    for (int rt = 0; rt < RT; ++rt)
-                        d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d[rt], 0, 0, 0);
-                // the previous tile's sums go into the locals while the
-                // second half of this tile's products runs
-                __builtin_amdgcn_sched_barrier(0);
-                if (u > 0) {
-                    if (PRIV) m2l_acc_write<RT, Kq, LS>(mine, cp, lk, pv, dp);
-                    else m2l_accumulate<PRIV, RT, Kq, K, LS>(mine, cp, lk, dp);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int kt = KT / 2; kt < KT; ++kt)
-#pragma unroll
-                    for (int rt = 0; rt < RT; ++rt)
-                        d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][kt], bv[kt], d[rt], 0, 0, 0);
-#pragma unroll
-                for (int rt = 0; rt < RT; ++rt) dp[rt] = d[rt];
-                cp = cr[u];
-#pragma unroll
-                for (int kt = 0; kt < KT; ++kt) bv[kt] = bn[kt];
-            }
-            m2l_accumulate<PRIV, RT, Kq, K, LS>(mine, cp, lk, dp);
+        for (int kt = 0; kt < KT; ++kt)
+            d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kt], bq[kt], d, 0, 0, 0);
+        const int c = bcol[b][li];
+        const int row = 16 * rt + 4 * lk;
+        if (c >= 0 && row < K) {
+            float4* p = reinterpret_cast<float4*>(loc + c * K + row);
+            float4 v = *p;
+            v.x += d[0];
+            v.y += d[1];
+            v.z += d[2];
+            v.w += d[3];
+            *p = v;
         }
+        if (tid < NS) bst4[1 - b][tid] = r1;
+        if (tid < NS && sq == 0) bcol[1 - b][sj] = c1;
+        if (t1.first) {
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) av[kt] = an[kt];
+        }
+        t0 = t1;
+        t1 = t2;
+        t2 = st.next();
+        s1 = s2;
+        b = 1 - b;
     }
     __syncthreads();
     const int64_t f0 = (int64_t)g * GT;
@@ -1257,17 +1244,13 @@ __global__ __launch_bounds__(64 * M2L_WAVES) void k_fmm_m2l_mfma(
         const int tc = i / K, k = i % K;
         const int64_t f = f0 + tc;
         if (f >= nf) break;
-        float v = 0.0f;
-#pragma unroll
-        for (int w = 0; w < NC; ++w) v += acc[(w * GT + tc) * LS + k];
+        const float v = loc[tc * K + k];
         float* dst = &Lf[f * Kp + k];
         if (nsplit == 1) *dst = v;
         else if (v != 0.0f) atomicAdd(dst, v);
     }
 }
 
-// FP64 M2L of one directed pair per wavefront, any two levels:
-// L~_A,n += sum_m coef[n][m] (s_B / s_A)^|m| Q~_B,m a_{n+m}((c_A - c_B) / s_A)
 template <int P>
 __global__ __launch_bounds__(64) void k_fmm_m2l_fp64(const int2* __restrict__ pr, int64_t npr,
                                                      const int3* __restrict__ lat, int64_t nlat,
@@ -1742,7 +1725,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
                 tb.cf.as<float>(), tb.ix.as<uint16_t>(), tb.deg.as<int8_t>(), st.Qf.as<float>(),
                 st.Lf.as<float>());
         else
-            k_fmm_m2l_mfma<P><<<ngroups * nsplit, 64 * M2L_WAVES, 0, s>>>(
+            k_fmm_m2l6<P><<<ngroups * nsplit, 64 * (Kp / 16), 0, s>>>(
                 nf, tb.noff, nsplit, st.blist.as<int2>(), st.gnb.as<uint32_t>(), st.ghist.as<uint32_t>(),
                 st.lsrc.as<int>(), st.lcol.as<int16_t>(), tb.T.as<float>(), st.Qf.as<float>(),
                 st.Lf.as<float>());
