@@ -276,7 +276,7 @@ def run_fmm(args):
     workload = f"fmm_{head['profile']}_n{N}_p{head['order']}_theta{args.theta}"
     traffic = None   # HBM bytes per launch from the committed rocprofv3 --pmc summary
     # the matrix-core M2L kernel of this order
-    kname = "k_fmm_m2l4" if head["order"] == 4 else "k_fmm_m2l_mfma"
+    kname = "k_fmm_m2l4" if head["order"] == 4 else "k_fmm_m2l6"
     tf = os.path.join(REPO, "profiles", f"traffic_{kname}_{head['profile']}_n{N}_p{head['order']}.json")
     if os.path.exists(tf) and head["m2l"] == "mfma":
         rec = json.load(open(tf))
