@@ -103,7 +103,7 @@ struct FmmTables {
 struct FmmState {
     FmmTables tab;
     DevBuf flag, scan, fid, fnode, Qd, Qf, Ld, Lf, rmax, lvl;
-    DevBuf rec, code, fa, fb, cnt, lat, gen, p2p, m2p, blist, gnb, ghist, gpos, gpair, lsrc, lcol;
+    DevBuf rec, code, fa, fb, cnt, lat, gen, p2p, m2p, blist, gnb, ghist, gpos, gpair, lpk;
     DevBuf phist, ppos, plist, edges;
     int64_t nf = 0;
     std::vector<int> fls;  // FMM node range per depth
@@ -602,11 +602,10 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ r
 
 // ------------------------------------------------------------------ binning
 // Lattice pairs -> bins (target group, offset), without contended global
-// atomics: first by target group (the walk emits a target's pairs, and its
-// siblings', next to each other, so a run of equal groups within a
-// wavefront takes one atomic), then one workgroup per group sorts the
-// group's pairs by offset with an LDS histogram and writes the group's row
-// of bin starts.  Near pairs -> bins (target) the same way as the first step.
+// atomics: first by target group (counted and placed a tile at a time in
+// LDS), then one workgroup per group sorts the group's pairs by offset with
+// an LDS histogram and writes the group's row of bin starts.  Near pairs ->
+// bins (target) with one atomic per run of equal targets in a wavefront.
 // A run of equal keys within a wavefront: one atomic, made by its first lane.
 struct RunInfo {
     bool valid, head;
@@ -627,57 +626,130 @@ __device__ __forceinline__ RunInfo run_of(int key, int64_t base, int64_t n) {
     return RunInfo{valid, head, next - l, h};
 }
 
-// The binning kernels below take RUN_U wavefront-chunks of 64 entries per
-// loop round: the round's loads are issued before its atomics, so that each
-// thread has RUN_U loads in flight.
-constexpr int RUN_U = 4;
+// Lattice pairs by target group, a tile of BT_T * BT_U consecutive pairs at a
+// time.  The walk emits a split pair's children together and its pairs in
+// the order of their ancestors, so a tile's targets fall in a few groups: a
+// tile's runs of equal groups are counted in LDS (a window of BT_W groups
+// from the tile's smallest) and the window's non-zero counts reach global
+// memory with one atomic each.  A group outside the window takes a global
+// atomic per run.
+constexpr int BT_T = 256, BT_U = 8, BT_TILE = BT_T * BT_U, BT_W = 4096;
 
-__global__ __launch_bounds__(256) void k_fmm_lat_ghist(const int3* __restrict__ lat, int64_t n, int gt,
-                                                       uint32_t* __restrict__ ghist) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base0 < n;
-         base0 += stride * RUN_U) {
-        int key[RUN_U];
+// the smallest and largest valid key of a workgroup's tile, in every thread
+__device__ __forceinline__ int2 tile_key_range(const int (&key)[BT_U], int* __restrict__ red) {
+    int lo = INT_MAX, hi = -1;
 #pragma unroll
-        for (int u = 0; u < RUN_U; ++u) {
-            const int64_t i = base0 + u * stride + __lane_id();
+    for (int u = 0; u < BT_U; ++u)
+        if (key[u] >= 0) lo = min(lo, key[u]), hi = max(hi, key[u]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) red[2 * w] = lo, red[2 * w + 1] = hi;
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < BT_T / 64; ++v) lo = min(lo, red[2 * v]), hi = max(hi, red[2 * v + 1]);
+    __syncthreads();
+    return make_int2(lo, hi);
+}
+
+__global__ __launch_bounds__(BT_T) void k_fmm_lat_ghist(const int3* __restrict__ lat, int64_t n, int gt,
+                                                        uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t h[BT_W];
+    __shared__ int red[2 * BT_T / 64];
+    for (int j = threadIdx.x; j < BT_W; j += BT_T) h[j] = 0;
+    const int64_t ntile = (n + BT_TILE - 1) / BT_TILE;
+    const int wbase = threadIdx.x & ~63;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t t0 = tile * BT_TILE;
+        int key[BT_U];
+#pragma unroll
+        for (int u = 0; u < BT_U; ++u) {
+            const int64_t i = t0 + u * BT_T + threadIdx.x;
             key[u] = i < n ? lat[i].x / gt : -1;
         }
+        const int2 kr = tile_key_range(key, red);  // also orders the clears before the counts
 #pragma unroll
-        for (int u = 0; u < RUN_U; ++u) {
-            const int64_t base = base0 + u * stride;  // the same in the whole wavefront
+        for (int u = 0; u < BT_U; ++u) {
+            const int64_t base = t0 + u * BT_T + wbase;  // the same in the whole wavefront
             if (base >= n) break;
             const RunInfo r = run_of(key[u], base, n);
-            if (r.head) atomicAdd(&ghist[key[u]], (uint32_t)r.count);
+            if (r.head) {
+                const int w = key[u] - kr.x;
+                if (w < BT_W) atomicAdd(&h[w], (uint32_t)r.count);
+                else atomicAdd(&ghist[key[u]], (uint32_t)r.count);
+            }
+        }
+        __syncthreads();
+        const int span = min(BT_W, kr.y - kr.x + 1);
+        for (int j = threadIdx.x; j < span; j += BT_T) {
+            const uint32_t c = h[j];
+            if (c) {
+                atomicAdd(&ghist[kr.x + j], c);
+                h[j] = 0;
+            }
         }
     }
 }
 
-// pair -> (source, column << 24 | offset) at its group's next free place
-__global__ __launch_bounds__(256) void k_fmm_lat_gscatter(const int3* __restrict__ lat, int64_t n,
-                                                          int gt, uint32_t* __restrict__ gpos,
-                                                          int2* __restrict__ out) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base0 < n;
-         base0 += stride * RUN_U) {
-        int3 e[RUN_U];
+// pair -> (source, column << 24 | offset) at its group's next free place:
+// a tile's runs take their places within (tile, group) from LDS counters,
+// then each of the tile's groups reserves its places with one global atomic
+__global__ __launch_bounds__(BT_T) void k_fmm_lat_gscatter(const int3* __restrict__ lat, int64_t n,
+                                                           int gt, uint32_t* __restrict__ gpos,
+                                                           int2* __restrict__ out) {
+    __shared__ uint32_t h[BT_W];
+    __shared__ int red[2 * BT_T / 64];
+    for (int j = threadIdx.x; j < BT_W; j += BT_T) h[j] = 0;
+    const int64_t ntile = (n + BT_TILE - 1) / BT_TILE;
+    const int wbase = threadIdx.x & ~63;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int64_t t0 = tile * BT_TILE;
+        int3 e[BT_U];
+        int key[BT_U];
 #pragma unroll
-        for (int u = 0; u < RUN_U; ++u) {
-            const int64_t i = base0 + u * stride + __lane_id();
+        for (int u = 0; u < BT_U; ++u) {
+            const int64_t i = t0 + u * BT_T + threadIdx.x;
             e[u] = i < n ? lat[i] : make_int3(0, 0, 0);
+            key[u] = i < n ? e[u].x / gt : -1;
         }
+        const int2 kr = tile_key_range(key, red);
+        // place within (tile, group) for keys in the window, else the final place
+        uint32_t at[BT_U];
 #pragma unroll
-        for (int u = 0; u < RUN_U; ++u) {
-            const int64_t base = base0 + u * stride;
+        for (int u = 0; u < BT_U; ++u) at[u] = 0;
+#pragma unroll
+        for (int u = 0; u < BT_U; ++u) {
+            const int64_t base = t0 + u * BT_T + wbase;
             if (base >= n) break;
-            const int key = base + __lane_id() < n ? e[u].x / gt : -1;
-            const RunInfo r = run_of(key, base, n);
+            const RunInfo r = run_of(key[u], base, n);
             uint32_t o = 0;
-            if (r.head) o = atomicAdd(&gpos[key], (uint32_t)r.count);
-            o = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
-            if (r.valid)
-                out[o] = make_int2(e[u].y, (int)(((uint32_t)(e[u].x - key * gt) << 24) | (uint32_t)e[u].z));
+            if (r.head) {
+                const int w = key[u] - kr.x;
+                o = w < BT_W ? atomicAdd(&h[w], (uint32_t)r.count)
+                             : atomicAdd(&gpos[key[u]], (uint32_t)r.count);
+            }
+            at[u] = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
         }
+        __syncthreads();
+        const int span = min(BT_W, kr.y - kr.x + 1);
+        for (int j = threadIdx.x; j < span; j += BT_T) {
+            const uint32_t c = h[j];
+            if (c) h[j] = atomicAdd(&gpos[kr.x + j], c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < BT_U; ++u) {
+            if (key[u] < 0) continue;
+            const int w = key[u] - kr.x;
+            const uint32_t o = w < BT_W ? h[w] + at[u] : at[u];
+            out[o] = make_int2(e[u].y, (int)(((uint32_t)(e[u].x - key[u] * gt) << 24) | (uint32_t)e[u].z));
+        }
+        __syncthreads();
+        // clear the window for the next tile (an entry that stayed zero is zero)
+        for (int j = threadIdx.x; j < span; j += BT_T) h[j] = 0;
     }
 }
 
@@ -685,7 +757,8 @@ __global__ __launch_bounds__(256) void k_fmm_lat_gscatter(const int3* __restrict
 // by offset in LDS (LSORT_CH offsets per pass), the counts scanned, the
 // group's non-empty bins listed in offset order as (offset, first pair) at
 // blist[g * noff + r], r < gnb[g] (a bin ends where the next one starts, the
-// last at gstart[g+1]), and the pairs placed by offset.
+// last at gstart[g+1]), and the pairs placed by offset as one word each:
+// source | column << 24 (sources below 2^24, columns below 128).
 constexpr int LSORT_T = 1024;
 constexpr int LSORT_CH = 32768;
 constexpr int LSORT_U = 8;
@@ -694,8 +767,7 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
                                                            const uint32_t* __restrict__ gstart,
                                                            int noff, int2* __restrict__ blist,
                                                            uint32_t* __restrict__ gnb,
-                                                           int* __restrict__ src,
-                                                           int16_t* __restrict__ col) {
+                                                           int* __restrict__ pk) {
     __shared__ uint32_t h[LSORT_CH];
     __shared__ uint32_t ws[LSORT_T / 64];
     const int g = blockIdx.x, tid = threadIdx.x;
@@ -754,8 +826,7 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
                 const int o = ok ? (int)((uint32_t)v[u].y & 0xFFFFFFu) - c0 : -1;
                 if (o >= 0 && o < nc) {
                     const uint32_t p = s0 + atomicAdd(&h[o], 1u);
-                    src[p] = v[u].x;
-                    col[p] = (int16_t)((uint32_t)v[u].y >> 24);
+                    pk[p] = v[u].x | (int)((uint32_t)v[u].y & 0xFF000000u);
                 }
             }
         }
@@ -765,6 +836,11 @@ __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restric
     }
     if (tid == 0) gnb[g] = listed;
 }
+
+// The near-pair binning kernels take RUN_U wavefront-chunks of 64 entries
+// per loop round: the round's loads are issued before its atomics, so that
+// each thread has RUN_U loads in flight.
+constexpr int RUN_U = 4;
 
 __global__ __launch_bounds__(256) void k_fmm_p2p_hist(const int2* __restrict__ pr, int64_t n,
                                                       uint32_t* __restrict__ hist) {
@@ -906,7 +982,7 @@ template <int P>
 __global__ __launch_bounds__(64 * M2L4_WAVES) void k_fmm_m2l4(
     int64_t nf, int noff, int noff0, int noff1, int nsplit, const int2* __restrict__ blist,
     const uint32_t* __restrict__ gnb, const uint32_t* __restrict__ gstart,
-    const int* __restrict__ src, const int16_t* __restrict__ col, const float* __restrict__ A,
+    const int* __restrict__ pk, const float* __restrict__ A,
     const float* __restrict__ cf, const uint16_t* __restrict__ ix, const int8_t* __restrict__ deg,
     const float* __restrict__ Qf, float* __restrict__ Lf) {
     constexpr int K = ncoef(P), K2 = ncoef(2 * P), Kp = kpad(P), Kq = kq(P);
@@ -975,19 +1051,20 @@ __global__ __launch_bounds__(64 * M2L4_WAVES) void k_fmm_m2l4(
         else if constexpr (R == 1) return x1;
         else return x2;
     };
+    // sid: the tile's packed entry (source | column << 24)
     auto load_q = [&](const M2lTile& t, int sid, float (&q3)[KT], int& c) {
         const int64_t e = pair_of(t);
-        const float* q = Qf + (e >= 0 ? (int64_t)sid : nf) * Kq + lk * KT;
+        const float* q = Qf + (e >= 0 ? (int64_t)(sid & 0xFFFFFF) : nf) * Kq + lk * KT;
 #pragma unroll
         for (int kt = 0; kt < KT; ++kt) q3[kt] = q[kt];
-        c = col[e >= 0 ? e : 0];
+        c = sid >> 24;
     };
     auto load_a = [&](const M2lTile& t) -> float4 {
         return reinterpret_cast<const float4*>(A + (int64_t)(t.o >= 0 ? t.o : 0) * (4 * K2V))[lane < K2V ? lane : 0];
     };
     auto load_s = [&](const M2lTile& t) -> int {
         const int64_t e = pair_of(t);
-        return src[e >= 0 ? e : 0];
+        return pk[e >= 0 ? e : 0];
     };
     // the operator of tile t from its Taylor coefficients, staged in LDS: the
     // elements are read first (build_read), then scaled in place (build_scale)
@@ -1137,7 +1214,7 @@ template <int P>
 __global__ __launch_bounds__(64 * (kpad(P) / 16)) void k_fmm_m2l6(
     int64_t nf, int noff, int nsplit, const int2* __restrict__ blist,
     const uint32_t* __restrict__ gnb, const uint32_t* __restrict__ gstart,
-    const int* __restrict__ src, const int16_t* __restrict__ col, const float* __restrict__ T,
+    const int* __restrict__ pk, const float* __restrict__ T,
     const float* __restrict__ Qf, float* __restrict__ Lf) {
     constexpr int K = ncoef(P), Kp = kpad(P), Kq = kq(P), RT = Kp / 16, KT = Kq / 4;
     constexpr int NT = 64 * RT, GT = M2L6_GT, KQ4 = Kq / 4, NS = 16 * KQ4;
@@ -1169,17 +1246,16 @@ __global__ __launch_bounds__(64 * (kpad(P) / 16)) void k_fmm_m2l6(
         const uint32_t e = t.e0 + (uint32_t)sj;
         return (tid < NS && t.o >= 0 && e < t.b1) ? (int64_t)e : -1;
     };
-    // an empty column reads the zero record nf
+    // the packed entry (source | column << 24), -1 for an empty column,
+    // which reads the zero record nf and adds to no column
     auto load_src = [&](const M2lTile& t) -> int {
         const int64_t e = pair_of(t);
-        return e >= 0 ? src[e] : (int)nf;
+        return e >= 0 ? pk[e] : -1;
     };
+    auto src_of = [&](int p) -> int { return p >= 0 ? (p & 0xFFFFFF) : (int)nf; };
+    auto col_of = [](int p) -> int { return p >= 0 ? (p >> 24) : -1; };
     auto load_rec = [&](int sid) -> float4 {
         return reinterpret_cast<const float4*>(Qf + (int64_t)sid * Kq)[tid < NS ? sq : 0];
-    };
-    auto load_col = [&](const M2lTile& t) -> int {
-        const int64_t e = pair_of(t);
-        return e >= 0 ? (int)col[e] : -1;
     };
     // this wavefront's operator rows of offset o: lane (li, lk) holds
     // T[16 rt + li][lk KT + kt], kt < KT
@@ -1195,8 +1271,8 @@ __global__ __launch_bounds__(64 * (kpad(P) / 16)) void k_fmm_m2l6(
     if (t0.o >= 0) load_op(t0.o, av);
     {
         const int s0 = load_src(t0);
-        const int c0 = load_col(t0);
-        const float4 r0 = load_rec(s0);
+        const int c0 = col_of(s0);
+        const float4 r0 = load_rec(src_of(s0));
         if (tid < NS) bst4[0][tid] = r0;
         if (tid < NS && sq == 0) bcol[0][sj] = c0;
     }
@@ -1205,8 +1281,8 @@ __global__ __launch_bounds__(64 * (kpad(P) / 16)) void k_fmm_m2l6(
     while (t0.o >= 0) {
         // the next tile's records and column (and operator rows at a bin
         // start), the source ids of the tile after it
-        const float4 r1 = load_rec(s1);
-        const int c1 = load_col(t1);
+        const float4 r1 = load_rec(src_of(s1));
+        const int c1 = col_of(s1);
         const int s2 = load_src(t2);
         if (t1.o >= 0 && t1.first) load_op(t1.o, an);
         __syncthreads();  // buffer b holds tile t0; buffer 1-b is no longer read
@@ -1552,6 +1628,10 @@ __global__ __launch_bounds__(256) void k_fmm_roots(int nsys, const int* __restri
 // ------------------------------------------------------------------ driver
 // workgroups for a grid-stride loop over n items, 256 per group
 static inline int capped(int64_t n) { return (int)std::min<int64_t>(grid_for(std::max<int64_t>(n, 1), 256), 1 << 16); }
+// tiles of lattice pairs: eight workgroups per CU, each over every 2048th tile
+static inline int bin_grid(int64_t n) {
+    return (int)std::min<int64_t>((std::max<int64_t>(n, 1) + BT_TILE - 1) / BT_TILE, 2048);
+}
 
 template <int P>
 static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2, bool mfma,
@@ -1693,27 +1773,30 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
     if (mfma) {
         const int GT = group_of(P);
         static_assert(P != 4 || group_of(4) == M2L4_GT, "order-4 groups");
+        // a binned pair is one word: source | column << 24
+        static_assert(group_of(P) <= 128, "columns of 7 bits");
+        if (nf >= (1 << 24))
+            return fail(HQ_ERR_OVERFLOW, "FMM matrix-core M2L of %lld cells (at most 2^24)", (long long)nf);
         const int ngroups = grid_for(nf, GT);
         const int64_t nbin = (int64_t)ngroups * tb.noff;
         if ((rc = st.blist.ensure(std::max<int64_t>(nbin, 1) * sizeof(int2))) ||
             (rc = st.gnb.ensure((ngroups + 1) * 4)) || (rc = st.ghist.ensure((ngroups + 1) * 4)) ||
             (rc = st.gpos.ensure((ngroups + 1) * 4)) ||
             (rc = st.gpair.ensure(std::max<int64_t>(nlat, 1) * sizeof(int2))) ||
-            (rc = st.lsrc.ensure(std::max<int64_t>(nlat, 1) * 4)) ||
-            (rc = st.lcol.ensure(std::max<int64_t>(nlat, 1) * 2)))
+            (rc = st.lpk.ensure(std::max<int64_t>(nlat, 1) * 4)))
             return rc;
         HQ_CHECK_HIP(hipMemsetAsync(st.ghist.p, 0, (ngroups + 1) * 4, s));
         if (nlat)
-            k_fmm_lat_ghist<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, GT,
+            k_fmm_lat_ghist<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT,
                                                          st.ghist.as<uint32_t>());
         if ((rc = scan_excl_buf(st.ghist.as<uint32_t>(), ngroups, t->scanpart, s))) return rc;
         HQ_CHECK_HIP(hipMemcpyAsync(st.gpos.p, st.ghist.p, (ngroups + 1) * 4, hipMemcpyDeviceToDevice, s));
         if (nlat)
-            k_fmm_lat_gscatter<<<capped(nlat), 256, 0, s>>>(st.lat.as<int3>(), nlat, GT,
+            k_fmm_lat_gscatter<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT,
                                                             st.gpos.as<uint32_t>(), st.gpair.as<int2>());
         k_fmm_lat_lsort<<<ngroups, LSORT_T, 0, s>>>(st.gpair.as<int2>(), st.ghist.as<uint32_t>(),
                                                     tb.noff, st.blist.as<int2>(), st.gnb.as<uint32_t>(),
-                                                    st.lsrc.as<int>(), st.lcol.as<int16_t>());
+                                                    st.lpk.as<int>());
         // enough workgroups to fill the device: split the bins of a group
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
         if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
@@ -1721,13 +1804,13 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         if constexpr (P == 4)
             k_fmm_m2l4<P><<<ngroups * nsplit, 64 * M2L4_WAVES, 0, s>>>(
                 nf, tb.noff, tb.noff0, tb.noff1, nsplit, st.blist.as<int2>(), st.gnb.as<uint32_t>(),
-                st.ghist.as<uint32_t>(), st.lsrc.as<int>(), st.lcol.as<int16_t>(), tb.A.as<float>(),
+                st.ghist.as<uint32_t>(), st.lpk.as<int>(), tb.A.as<float>(),
                 tb.cf.as<float>(), tb.ix.as<uint16_t>(), tb.deg.as<int8_t>(), st.Qf.as<float>(),
                 st.Lf.as<float>());
         else
             k_fmm_m2l6<P><<<ngroups * nsplit, 64 * (Kp / 16), 0, s>>>(
                 nf, tb.noff, nsplit, st.blist.as<int2>(), st.gnb.as<uint32_t>(), st.ghist.as<uint32_t>(),
-                st.lsrc.as<int>(), st.lcol.as<int16_t>(), tb.T.as<float>(), st.Qf.as<float>(),
+                st.lpk.as<int>(), tb.T.as<float>(), st.Qf.as<float>(),
                 st.Lf.as<float>());
         HQ_CHECK_HIP(hipEventRecord(t->ev[7], s));
         if (ngen)
