@@ -382,16 +382,16 @@ __global__ void k_fmm_edges(const uint32_t* __restrict__ cnt, int nblk, uint32_t
 
 
 struct WalkPar {
-    double theta;
+    double theta2;
     int R, nd, nd1, noff0, noff1;
 };
 
 // One FMM cell as the walk reads it (64 bytes, four 16-byte loads): centre,
-// bounding radius, side, children's FMM ids [kid0, kid1) (the non-empty
-// children of an internal cell are FMM cells with consecutive ids), body
-// count, flag | depth << 2.
+// bounding radius, side and its reciprocal, children's FMM ids [kid0, kid1)
+// (the non-empty children of an internal cell are FMM cells with
+// consecutive ids), body count, flag | depth << 2.
 struct __align__(16) FRec {
-    double cx, cy, cz, rmax, side;
+    double cx, cy, cz, rmax, side, rside;
     int kid0, kid1, count;
     uint32_t info;
 };
@@ -409,6 +409,7 @@ __global__ __launch_bounds__(256) void k_fmm_records(int nf, FView v, const uint
     r.cz = v.cz[k];
     r.rmax = rmax[f];
     r.side = side_of(v, k);
+    r.rside = 1.0 / r.side;
     r.kid0 = flag == 2 ? (int)scan[v.cstart[k]] : 0;
     r.kid1 = flag == 2 ? (int)scan[v.cstart[k + 1]] : 0;
     r.count = v.count[k];
@@ -429,24 +430,25 @@ __device__ __forceinline__ uint32_t fmm_classify(const FRec& ra, const FRec& rb,
     const int da = (int)(ra.info >> 2), db = (int)(rb.info >> 2);
     if (a != b) {
         const double dx = rb.cx - ra.cx, dy = rb.cy - ra.cy, dz = rb.cz - ra.cz;
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        if (ra.rmax + rb.rmax <= w.theta * d) {
+        const double rs = ra.rmax + rb.rmax;
+        // rs <= theta d, squared
+        if (rs * rs <= w.theta2 * (dx * dx + dy * dy + dz * dz)) {
             // a leaf target much larger than its source: the local expansion
             // would carry the larger error, so the source acts on the bodies
             // directly (its multipole, or its bodies when it is a leaf)
             if (la && (ra.rmax > 2.0 * rb.rmax || (ra.rmax > rb.rmax && ra.count <= 8)))
                 return lb ? ACT_P2P : ACT_M2P;
             if (da == db) {
-                const double s = ra.side;
-                const int ox = (int)rint(dx / s), oy = (int)rint(dy / s), oz = (int)rint(dz / s);
+                // the cubes share a lattice: the offset is an integer of sides
+                const double rsd = ra.rside;
+                const int ox = (int)rint(dx * rsd), oy = (int)rint(dy * rsd), oz = (int)rint(dz * rsd);
                 if (abs(ox) <= w.R && abs(oy) <= w.R && abs(oz) <= w.R)
                     return ACT_LATTICE |
                            ((uint32_t)(((ox + w.R) * w.nd + (oy + w.R)) * w.nd + (oz + w.R)) << 3);
             } else if (da == db - 1 || da == db + 1) {
                 // half-integer offset in units of the finer cube
-                const double s = da > db ? ra.side : rb.side;
-                const int wx = (int)rint(2.0 * dx / s), wy = (int)rint(2.0 * dy / s),
-                          wz = (int)rint(2.0 * dz / s);
+                const double rsd = 2.0 * (da > db ? ra.rside : rb.rside);
+                const int wx = (int)rint(dx * rsd), wy = (int)rint(dy * rsd), wz = (int)rint(dz * rsd);
                 const int lim = 2 * w.R + 1;
                 if ((wx & 1) && (wy & 1) && (wz & 1) && abs(wx) <= lim && abs(wy) <= lim &&
                     abs(wz) <= lim) {
@@ -493,8 +495,12 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_count(const FRec* __restrict__ 
     }
     const int act = code_act(c);
     const uint32_t cv = code_count(c);
-    for (int q = 0; q < NACT; ++q) {
-        const uint32_t t = wave_sum(act == q ? cv : 0u);
+    // a split counts its child pairs, any other pair one entry of its list
+    const uint32_t te = wave_sum(act == ACT_EXPAND ? cv : 0u);
+    if (__lane_id() == 0) ws[ACT_EXPAND][threadIdx.x >> 6] = te;
+#pragma unroll
+    for (int q = 1; q < NACT; ++q) {
+        const uint32_t t = (uint32_t)__popcll(__ballot(act == q));
         if (__lane_id() == 0) ws[q][threadIdx.x >> 6] = t;
     }
     __syncthreads();
@@ -533,11 +539,18 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ r
     const int act = code_act(c);
     const uint32_t cv = code_count(c);
     uint32_t pre[NACT];
-    for (int q = 0; q < NACT; ++q) {
-        const uint32_t v = act == q ? cv : 0u;
+    {
+        const uint32_t v = act == ACT_EXPAND ? cv : 0u;
         const uint32_t incl = wave_incl_scan(v);
-        pre[q] = incl - v;
-        if (__lane_id() == 63) ws[q][threadIdx.x >> 6] = incl;
+        pre[ACT_EXPAND] = incl - v;
+        if (__lane_id() == 63) ws[ACT_EXPAND][threadIdx.x >> 6] = incl;
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int q = 1; q < NACT; ++q) {
+        const uint64_t m = __ballot(act == q);
+        pre[q] = (uint32_t)__popcll(m & lt);
+        if (__lane_id() == 0) ws[q][threadIdx.x >> 6] = (uint32_t)__popcll(m);
     }
     __syncthreads();
     const int wv = threadIdx.x >> 6;
@@ -1704,7 +1717,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         (int)nf, v, st.scan.as<uint32_t>(), st.rmax.as<double>(), st.rec.as<FRec>());
     HQ_CHECK_HIP(hipMemsetAsync(st.Ld.p, 0, nf * K * 8, s));
     // 3. dual walk: frontier ping-pong in fa / fb; lists appended
-    WalkPar w{theta, tb.R, tb.nd, tb.nd1, tb.noff0, tb.noff1};
+    WalkPar w{theta * theta, tb.R, tb.nd, tb.nd1, tb.noff0, tb.noff1};
     int64_t nfr = t->nsys, nlat = 0, ngen = 0, np2p = 0, nm2p = 0;
     // the lists keep their capacity from one evaluation to the next
     size_t lat_cap = st.lat.bytes / sizeof(int3), gen_cap = st.gen.bytes / sizeof(int2),
