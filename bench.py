@@ -288,7 +288,7 @@ def run_fmm(args):
         "steps": args.steps, "warmup": 1, "ms_per_step": head["ms_per_eval"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded uniform cube and Plummer sphere)",
-        "config": {"workload": workload, "N": N, "theta": args.theta, "leaf": 32},
+        "config": {"workload": workload, "N": N, "theta": args.theta, "leaf": 64},
         "rms_rel_err": head["rms_rel_err"], "max_rel_err": head["max_rel_err"],
         "roofline": {"bound": "mfma", "kernel": f"{kname}<{head['order']}>",
                      "achieved": head["m2l_tflops"], "peak": peak, "unit": "TFLOP/s",

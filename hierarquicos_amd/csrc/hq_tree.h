@@ -56,7 +56,7 @@ struct hq_tree {
     hq::DevBuf groups, gflag, bstart;
     int super_size = 256;  // bunching: sibling subtrees packed up to this many bodies
     int leaf = 0;          // bunched walks: cells of <= leaf bodies enter as their bodies (0: off)
-    int fmm_leaf = 32;     // FMM: cells of <= fmm_leaf bodies are leaves (1..64)
+    int fmm_leaf = 64;     // FMM: cells of <= fmm_leaf bodies are leaves (1..64)
     std::unique_ptr<hq::FmmState, hq::FmmFree> fmm;
     bool fused = true;     // mixed bunched walk as one fused kernel
     int split = 1;         // split evaluation of the fused walk: 0 never, 1 few bunches, 2 always

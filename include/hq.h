@@ -112,7 +112,7 @@ int hq_tree_destroy(hq_tree *t);
 #define HQ_TUNE_LEAF 4    /* 0 (default) .. 64: in bunched walks a cell of at most this
                              many bodies enters the lists as its bodies (exact P2P),
                              the leaf-bucket size of a bucketed tree */
-#define HQ_TUNE_FMM_LEAF 5 /* 1..64 (default 32): FMM cells of at most this many bodies are leaves */
+#define HQ_TUNE_FMM_LEAF 5 /* 1..64 (default 64): FMM cells of at most this many bodies are leaves */
 #define HQ_TUNE_GRAPH 6    /* 1 (default): hq_tree_integrate with HQ_FORCE_DIRECT, or with
                               HQ_FORCE_BUNCHED at HQ_MIXED in the fused walk, captures one step
                               in a HIP graph and replays it (same results); 0: launches every
