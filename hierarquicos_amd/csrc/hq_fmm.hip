@@ -1512,12 +1512,13 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
                 for (int b = ql; b < jc && jp + b < NB; b += 16) {
                     const int jj = jf + b;
                     const double4 r = sb[jj];  // the sorted body as one (x, y, z, m) record
-                    if constexpr (F32)
+                    if constexpr (F32) {
                         sbuf[jp + b] = make_float4((float)(r.x - c[0]), (float)(r.y - c[1]),
                                                    (float)(r.z - c[2]), (float)r.w);
-                    else
+                    } else {
                         sbuf[jp + b] = r;
-                    sidx[jp + b] = jj;
+                        sidx[jp + b] = jj;
+                    }
                 }
         }
         __syncthreads();
@@ -1528,8 +1529,11 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
                 const float4 sq = sbuf[j];
                 const float dx = sq.x - tx, dy = sq.y - ty, dz = sq.z - tz;
                 const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, e2)));
-                const float ri = __frsqrt_rn(d2);
-                const float w = sidx[j] == i ? 0.0f : sq.w * ri * ri * ri;
+                // the target itself (staged by the same recentring, so dx = dy =
+                // dz = 0 exactly) adds nothing: with eps2 > 0 through the zero
+                // differences, with eps2 = 0 through ri = 0
+                const float ri = d2 > 0.0f ? __frsqrt_rn(d2) : 0.0f;
+                const float w = sq.w * ri * ri * ri;
                 p[0] = fmaf(w, dx, p[0]);
                 p[1] = fmaf(w, dy, p[1]);
                 p[2] = fmaf(w, dz, p[2]);
