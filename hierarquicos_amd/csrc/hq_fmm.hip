@@ -1525,6 +1525,7 @@ __global__ __launch_bounds__(64) void k_fmm_leaf(int nf, FView v, const double* 
         if constexpr (F32) {
             float p[3] = {0.f, 0.f, 0.f};
             const float e2 = (float)eps2;
+#pragma unroll 4
             for (int j = grp; j < fill; j += NG) {
                 const float4 sq = sbuf[j];
                 const float dx = sq.x - tx, dy = sq.y - ty, dz = sq.z - tz;
