@@ -772,23 +772,27 @@ __global__ __launch_bounds__(BT_T) void k_fmm_lat_gscatter(const int3* __restric
 // blist[g * noff + r], r < gnb[g] (a bin ends where the next one starts, the
 // last at gstart[g+1]), and the pairs placed by offset as one word each:
 // source | column << 24 (sources below 2^24, columns below 128).
+// The offset counts live in dynamic LDS, min(noff, LSORT_CH) entries: at
+// theta = 0.5 the 16,577 offsets take 66 KB, so two workgroups share a CU.
 constexpr int LSORT_T = 1024;
 constexpr int LSORT_CH = 32768;
 constexpr int LSORT_U = 8;
+__host__ __device__ inline int lsort_chunk(int noff) { return noff < LSORT_CH ? noff : LSORT_CH; }
 
 __global__ __launch_bounds__(LSORT_T) void k_fmm_lat_lsort(const int2* __restrict__ gp,
                                                            const uint32_t* __restrict__ gstart,
                                                            int noff, int2* __restrict__ blist,
                                                            uint32_t* __restrict__ gnb,
                                                            int* __restrict__ pk) {
-    __shared__ uint32_t h[LSORT_CH];
+    extern __shared__ uint32_t h[];  // lsort_chunk(noff) counts
     __shared__ uint32_t ws[LSORT_T / 64];
+    const int ch = lsort_chunk(noff);
     const int g = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = gstart[g], s1 = gstart[g + 1];
     int2* row = blist + (int64_t)g * noff;
     uint32_t done = 0, listed = 0;  // pairs and non-empty bins of the earlier passes
-    for (int c0 = 0; c0 < noff; c0 += LSORT_CH) {
-        const int nc = min(LSORT_CH, noff - c0);
+    for (int c0 = 0; c0 < noff; c0 += ch) {
+        const int nc = min(ch, noff - c0);
         for (int o = tid; o < nc; o += LSORT_T) h[o] = 0;
         __syncthreads();
         // LSORT_U pairs per thread in flight: the loads of a round are
@@ -1812,9 +1816,13 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         if (nlat)
             k_fmm_lat_gscatter<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT,
                                                             st.gpos.as<uint32_t>(), st.gpair.as<int2>());
-        k_fmm_lat_lsort<<<ngroups, LSORT_T, 0, s>>>(st.gpair.as<int2>(), st.ghist.as<uint32_t>(),
+        const int lsh = lsort_chunk(tb.noff) * (int)sizeof(uint32_t);
+        HQ_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fmm_lat_lsort),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, lsh));
+        k_fmm_lat_lsort<<<ngroups, LSORT_T, lsh, s>>>(st.gpair.as<int2>(), st.ghist.as<uint32_t>(),
                                                     tb.noff, st.blist.as<int2>(), st.gnb.as<uint32_t>(),
                                                     st.lpk.as<int>());
+        HQ_CHECK_HIP(hipGetLastError());
         // enough workgroups to fill the device: split the bins of a group
         const int nsplit = std::max(1, std::min(64, 2048 / ngroups));
         if (nsplit > 1) HQ_CHECK_HIP(hipMemsetAsync(st.Lf.p, 0, nf * Kp * 4, s));
