@@ -479,36 +479,58 @@ __device__ __forceinline__ uint32_t code_count(uint32_t c) {
 
 constexpr int WB = 256;
 
-// Classifies every frontier pair once, keeps its code, and stores per block
-// the counts of next-frontier pairs, lattice, generic, near and M2P pairs.
+// Classifies every frontier pair once, keeps its code, and stores per chunk
+// of WB pairs (the emit kernel's workgroups) the counts of next-frontier
+// pairs, lattice, generic, near and M2P pairs.  A workgroup takes WALK_U
+// consecutive chunks, a thread one pair of each, with the loads of all its
+// pairs in flight together (the pass is bound by the latency of the pair and
+// cell-record loads).
+constexpr int WALK_U = 2;
+
 __global__ __launch_bounds__(WB) void k_fmm_walk_count(const FRec* __restrict__ rec, WalkPar w,
                                                        const int2* __restrict__ fr, int64_t nfr,
                                                        uint32_t* __restrict__ code,
                                                        uint32_t* __restrict__ cnt, int nblk) {
-    __shared__ uint32_t ws[NACT][WB / 64];
-    const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
-    uint32_t c = ACT_NONE;
-    if (i < nfr) {
-        const int a = fr[i].x, b = fr[i].y;
-        c = a < 0 ? (uint32_t)ACT_NONE : fmm_classify(rec[a], rec[b], w, a, b);
-        code[i] = c;
-    }
-    const int act = code_act(c);
-    const uint32_t cv = code_count(c);
-    // a split counts its child pairs, any other pair one entry of its list
-    const uint32_t te = wave_sum(act == ACT_EXPAND ? cv : 0u);
-    if (__lane_id() == 0) ws[ACT_EXPAND][threadIdx.x >> 6] = te;
+    __shared__ uint32_t ws[WALK_U][NACT][WB / 64];
+    const int64_t i0 = blockIdx.x * (int64_t)(WALK_U * WB) + threadIdx.x;
+    int2 pr[WALK_U];
 #pragma unroll
-    for (int q = 1; q < NACT; ++q) {
-        const uint32_t t = (uint32_t)__popcll(__ballot(act == q));
-        if (__lane_id() == 0) ws[q][threadIdx.x >> 6] = t;
+    for (int u = 0; u < WALK_U; ++u) {
+        const int64_t i = i0 + u * WB;
+        pr[u] = i < nfr ? fr[i] : make_int2(-1, 0);
+    }
+    FRec ra[WALK_U], rb[WALK_U];
+#pragma unroll
+    for (int u = 0; u < WALK_U; ++u)
+        if (pr[u].x >= 0) {
+            ra[u] = rec[pr[u].x];
+            rb[u] = rec[pr[u].y];
+        }
+#pragma unroll
+    for (int u = 0; u < WALK_U; ++u) {
+        const int64_t i = i0 + u * WB;
+        uint32_t c = ACT_NONE;
+        if (pr[u].x >= 0) c = fmm_classify(ra[u], rb[u], w, pr[u].x, pr[u].y);
+        if (i < nfr) code[i] = c;
+        const int act = code_act(c);
+        const uint32_t cv = code_count(c);
+        // a split counts its child pairs, any other pair one entry of its list
+        const uint32_t te = wave_sum(act == ACT_EXPAND ? cv : 0u);
+        if (__lane_id() == 0) ws[u][ACT_EXPAND][threadIdx.x >> 6] = te;
+#pragma unroll
+        for (int q = 1; q < NACT; ++q) {
+            const uint32_t t = (uint32_t)__popcll(__ballot(act == q));
+            if (__lane_id() == 0) ws[u][q][threadIdx.x >> 6] = t;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < NACT) {
+    if (threadIdx.x < WALK_U * NACT) {
+        const int u = threadIdx.x / NACT, q = threadIdx.x % NACT;
+        const int64_t blk = (int64_t)blockIdx.x * WALK_U + u;
         uint32_t t = 0;
-        for (int k = 0; k < WB / 64; ++k) t += ws[threadIdx.x][k];
-        cnt[(int64_t)threadIdx.x * (nblk + 1) + blockIdx.x] = t;
-        if (blockIdx.x == 0) cnt[(int64_t)threadIdx.x * (nblk + 1) + nblk] = 0;
+        for (int k = 0; k < WB / 64; ++k) t += ws[u][q][k];
+        if (blk < nblk) cnt[(int64_t)q * (nblk + 1) + blk] = t;
+        if (blk == 0) cnt[(int64_t)q * (nblk + 1) + nblk] = 0;
     }
 }
 
@@ -1758,7 +1780,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
         if ((rc = st.cnt.ensure(((size_t)NACT * (nblk + 1) + 1) * 4)) || (rc = st.code.ensure(nfr * 4)))
             return rc;
         uint32_t* cnt = st.cnt.as<uint32_t>();
-        k_fmm_walk_count<<<nblk, WB, 0, s>>>(st.rec.as<FRec>(), w, cur->as<int2>(), nfr,
+        k_fmm_walk_count<<<(nblk + WALK_U - 1) / WALK_U, WB, 0, s>>>(st.rec.as<FRec>(), w, cur->as<int2>(), nfr,
                                              st.code.as<uint32_t>(), cnt, nblk);
         // one scan over [category][block]: category q's slice starts at its base
         if ((rc = scan_excl_buf(cnt, (int64_t)NACT * (nblk + 1), t->scanpart, s))) return rc;
