@@ -534,10 +534,11 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_count(const FRec* __restrict__ 
     }
 }
 
-// Writes each pair's output at its place (block prefix from the counts,
-// scanned as one array [category][block], then the wavefront's prefix): a
+// Writes each pair's output at its place (chunk prefix from the counts,
+// scanned as one array [category][chunk], then the wavefront's prefix): a
 // split's child pairs to the next frontier, an accept or near pair to its
-// list.
+// list.  A workgroup takes WALK_U consecutive chunks of WB pairs, as the
+// classify pass does, with the loads of all of them issued first.
 __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ rec,
                                                       const int2* __restrict__ fr, int64_t nfr,
                                                       const uint32_t* __restrict__ code,
@@ -547,64 +548,83 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ r
                                                       int2* __restrict__ gen, int64_t gen0,
                                                       int2* __restrict__ p2p, int64_t p2p0,
                                                       int2* __restrict__ m2p, int64_t m2p0) {
-    __shared__ uint32_t ws[NACT][WB / 64];
+    __shared__ uint32_t ws[WALK_U][NACT][WB / 64];
     __shared__ uint32_t xstart[WB / 64][64];
     __shared__ int4 xpar[WB / 64][64];
-    const int64_t i = blockIdx.x * (int64_t)WB + threadIdx.x;
-    uint32_t c = ACT_NONE;
-    int a = 0, b = 0;
-    if (i < nfr) {
-        c = code[i];
-        a = fr[i].x;
-        b = fr[i].y;
-    }
-    const int act = code_act(c);
-    const uint32_t cv = code_count(c);
-    uint32_t pre[NACT];
-    {
-        const uint32_t v = act == ACT_EXPAND ? cv : 0u;
-        const uint32_t incl = wave_incl_scan(v);
-        pre[ACT_EXPAND] = incl - v;
-        if (__lane_id() == 63) ws[ACT_EXPAND][threadIdx.x >> 6] = incl;
-    }
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
+    const int64_t i0 = blockIdx.x * (int64_t)(WALK_U * WB) + threadIdx.x;
+    uint32_t c[WALK_U];
+    int a[WALK_U], b[WALK_U];
 #pragma unroll
-    for (int q = 1; q < NACT; ++q) {
-        const uint64_t m = __ballot(act == q);
-        pre[q] = (uint32_t)__popcll(m & lt);
-        if (__lane_id() == 0) ws[q][threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    for (int u = 0; u < WALK_U; ++u) {
+        const int64_t i = i0 + u * WB;
+        c[u] = ACT_NONE;
+        a[u] = b[u] = 0;
+        if (i < nfr) {
+            c[u] = code[i];
+            const int2 pr = fr[i];
+            a[u] = pr.x;
+            b[u] = pr.y;
+        }
     }
-    __syncthreads();
-    const int wv = threadIdx.x >> 6;
-    for (int q = 0; q < NACT; ++q) {
-        uint32_t base = cnt[(int64_t)q * (nblk + 1) + blockIdx.x] - cnt[(int64_t)q * (nblk + 1)];
-        for (int k = 0; k < wv; ++k) base += ws[q][k];
-        pre[q] += base;
-    }
-    // a split's child pairs: the wavefront writes them together, consecutive
-    // lanes at consecutive places (a pair may have up to 64 children): lane
-    // j's pairs are [ex[j], ex[j] + cv) of the wavefront's run, which starts
-    // at pre[0] of its first lane
-    {
-        const int lane = __lane_id();
-        int a0 = a, b0 = b, nbk = 1;
-        if (act == ACT_EXPAND) {
-            const int sp = (int)((c >> 3) & 3u);
-            if (sp & 1) a0 = rec[a].kid0;
+    // a split's first child ids and the source's child count
+    int a0[WALK_U], b0[WALK_U], nbk[WALK_U];
+#pragma unroll
+    for (int u = 0; u < WALK_U; ++u) {
+        a0[u] = a[u];
+        b0[u] = b[u];
+        nbk[u] = 1;
+        if (code_act(c[u]) == ACT_EXPAND) {
+            const int sp = (int)((c[u] >> 3) & 3u);
+            if (sp & 1) a0[u] = rec[a[u]].kid0;
             if (sp & 2) {
-                b0 = rec[b].kid0;
-                nbk = rec[b].kid1 - b0;
+                b0[u] = rec[b[u]].kid0;
+                nbk[u] = rec[b[u]].kid1 - b0[u];
             }
         }
+    }
+    uint32_t pre[WALK_U][NACT];
+#pragma unroll
+    for (int u = 0; u < WALK_U; ++u) {
+        const int act = code_act(c[u]);
+        const uint32_t v = act == ACT_EXPAND ? code_count(c[u]) : 0u;
+        const uint32_t incl = wave_incl_scan(v);
+        pre[u][ACT_EXPAND] = incl - v;
+        if (lane == 63) ws[u][ACT_EXPAND][wv] = incl;
+#pragma unroll
+        for (int q = 1; q < NACT; ++q) {
+            const uint64_t m = __ballot(act == q);
+            pre[u][q] = (uint32_t)__popcll(m & lt);
+            if (lane == 0) ws[u][q][wv] = (uint32_t)__popcll(m);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < WALK_U; ++u) {
+        const int64_t blk = (int64_t)blockIdx.x * WALK_U + u;
+        if (blk >= nblk) break;  // the same in the whole workgroup
+        for (int q = 0; q < NACT; ++q) {
+            uint32_t base = cnt[(int64_t)q * (nblk + 1) + blk] - cnt[(int64_t)q * (nblk + 1)];
+            for (int k = 0; k < wv; ++k) base += ws[u][q][k];
+            pre[u][q] += base;
+        }
+        const int act = code_act(c[u]);
+        const uint32_t cv = code_count(c[u]);
+        // a split's child pairs: the wavefront writes them together,
+        // consecutive lanes at consecutive places (a pair may have up to 64
+        // children): lane j's pairs are [ex[j], ex[j] + cv) of the
+        // wavefront's run, which starts at pre[0] of its first lane
         const uint32_t ne = act == ACT_EXPAND ? cv : 0u;
         const uint32_t incl = wave_incl_scan(ne);
         const uint32_t tot = __shfl(incl, 63, 64);
         if (tot > 0) {
             uint32_t* xs = &xstart[wv][0];
             int4* xp = &xpar[wv][0];
+            __builtin_amdgcn_wave_barrier();  // the previous chunk's reads are done
             xs[lane] = incl - ne;
-            xp[lane] = make_int4(a0, b0, nbk, (int)ne);
-            const int64_t o0 = (int64_t)__shfl((int)(pre[0] - (incl - ne)), 0, 64);
+            xp[lane] = make_int4(a0[u], b0[u], nbk[u], (int)ne);
+            const int64_t o0 = (int64_t)__shfl((int)(pre[u][0] - (incl - ne)), 0, 64);
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -623,15 +643,15 @@ __global__ __launch_bounds__(WB) void k_fmm_walk_emit(const FRec* __restrict__ r
                 next[o0 + k] = make_int2(q.x + r / q.z, q.y + r % q.z);
             }
         }
-    }
-    if (act == ACT_LATTICE) {
-        lat[lat0 + pre[1]] = make_int3(a, b, (int)(c >> 3));
-    } else if (act == ACT_GENERIC) {
-        gen[gen0 + pre[2]] = make_int2(a, b);
-    } else if (act == ACT_P2P) {
-        p2p[p2p0 + pre[3]] = make_int2(a, b);
-    } else if (act == ACT_M2P) {
-        m2p[m2p0 + pre[4]] = make_int2(a, b);
+        if (act == ACT_LATTICE) {
+            lat[lat0 + pre[u][1]] = make_int3(a[u], b[u], (int)(c[u] >> 3));
+        } else if (act == ACT_GENERIC) {
+            gen[gen0 + pre[u][2]] = make_int2(a[u], b[u]);
+        } else if (act == ACT_P2P) {
+            p2p[p2p0 + pre[u][3]] = make_int2(a[u], b[u]);
+        } else if (act == ACT_M2P) {
+            m2p[m2p0 + pre[u][4]] = make_int2(a[u], b[u]);
+        }
     }
 }
 
@@ -1799,7 +1819,7 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
             (rc = grow(st.p2p, p2p_cap, np2p, np2p + tot[3] + 1, sizeof(int2))) ||
             (rc = grow(st.m2p, m2p_cap, nm2p, nm2p + tot[4] + 1, sizeof(int2))))
             return rc;
-        k_fmm_walk_emit<<<nblk, WB, 0, s>>>(st.rec.as<FRec>(), cur->as<int2>(), nfr,
+        k_fmm_walk_emit<<<(nblk + WALK_U - 1) / WALK_U, WB, 0, s>>>(st.rec.as<FRec>(), cur->as<int2>(), nfr,
                                             st.code.as<uint32_t>(), cnt, nblk, nxt->as<int2>(),
                                             st.lat.as<int3>(), nlat, st.gen.as<int2>(), ngen,
                                             st.p2p.as<int2>(), np2p, st.m2p.as<int2>(), nm2p);
