@@ -139,7 +139,9 @@ static int build_tables(FmmTables& tb, int p, double theta) {
             mcoef[n * K + m] = sgn * binom(a.x + b.x, a.x) * binom(a.y + b.y, a.y) * binom(a.z + b.z, a.z);
         }
     // M2M / L2L for the child with digit dg (bits 4x + 2y + z: 1 = upper half),
-    // child centre = parent centre + delta s_parent / 4
+    // child centre = parent centre + delta s_parent / 4.  Both are stored by
+    // input coefficient, then output coefficient, so that the lanes (one per
+    // output coefficient) of a step over the inputs read consecutive words.
     std::vector<double> m2m(8 * K * K, 0.0), l2l(8 * K * K, 0.0);
     for (int dg = 0; dg < 8; ++dg) {
         const double del[3] = {(dg & 4) ? 1.0 : -1.0, (dg & 2) ? 1.0 : -1.0, (dg & 1) ? 1.0 : -1.0};
@@ -151,9 +153,9 @@ static int build_tables(FmmTables& tb, int p, double theta) {
                     double c = binom(a.x, b.x) * binom(a.y, b.y) * binom(a.z, b.z) *
                                pow(del[0] / 4, a.x - b.x) * pow(del[1] / 4, a.y - b.y) *
                                pow(del[2] / 4, a.z - b.z);
-                    m2m[(dg * K + n) * K + m] = c * ldexp(1.0, -(b.x + b.y + b.z));
+                    m2m[(dg * K + m) * K + n] = c * ldexp(1.0, -(b.x + b.y + b.z));
                     // L~_c,m += C(n,m) (delta/4)^(n-m) 2^-(|m|+1) L~_p,n
-                    l2l[(dg * K + m) * K + n] = c * ldexp(1.0, -(b.x + b.y + b.z) - 1);
+                    l2l[(dg * K + n) * K + m] = c * ldexp(1.0, -(b.x + b.y + b.z) - 1);
                 }
             }
     }
@@ -341,12 +343,13 @@ __global__ __launch_bounds__(64) void k_fmm_m2m(int f0, int f1, FView v, const d
         const int fc = v.fid[c];
         const double* q = Qd + (int64_t)fc * K;
         const double* t = m2m + (int64_t)v.digit[c] * K * K;
-        for (int h = 0; h < 2; ++h) {
-            const int n = lane + 64 * h;
-            if (n < K) {
-                double a = acc[h];
-                for (int m = 0; m <= n; ++m) a += t[n * K + m] * q[m];
-                acc[h] = a;
+        // input m to output n (zero unless m <= n), in the order of m
+        for (int m = 0; m < K; ++m) {
+            const double qm = q[m];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int n = lane + 64 * h;
+                if (n < K) acc[h] += t[m * K + n] * qm;
             }
         }
         r = fmax(r, rmax[fc] + 0.4330127018922193 * s);  // |c_child - c| = sqrt(3) s / 4
@@ -1463,10 +1466,20 @@ __global__ __launch_bounds__(64) void k_fmm_l2l(int f0, int f1, FView v, const i
     if (fp < 0) return;
     const double* t = l2l + (int64_t)v.digit[k] * K * K;
     const double* lp = Ld + (int64_t)fp * K;
-    for (int m = threadIdx.x; m < K; m += 64) {
-        double acc = 0.0;
-        for (int n = 0; n < K; ++n) acc += t[m * K + n] * lp[n];
-        Ld[(int64_t)f * K + m] += acc;
+    // parent coefficient n to child coefficient m, in the order of n
+    double acc[2] = {0.0, 0.0};
+    for (int n = 0; n < K; ++n) {
+        const double ln = lp[n];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int m = threadIdx.x + 64 * h;
+            if (m < K) acc[h] += t[n * K + m] * ln;
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int m = threadIdx.x + 64 * h;
+        if (m < K) Ld[(int64_t)f * K + m] += acc[h];
     }
 }
 
