@@ -592,12 +592,23 @@ def main():
                     f"order 5 beyond {args.far:g})")
     else:
         far_text = f"far field beyond {args.far:g} bunch radii (local expansion of order 5)"
+    headline = (args.profile == "plummer" and N == 1 << 20 and args.method == 12
+                and args.theta == 0.5)
+    if headline:
+        metric = "particles/s per force eval at N=1M, rms rel err <=1e-4 vs FP64 direct"
+    else:
+        nl = f"{N >> 20}M" if N % (1 << 20) == 0 else str(N)
+        metric = (f"particles/s per force eval at N={nl}, {args.profile}, "
+                  f"{ {10: 'monopole', 11: 'quadrupole', 12: 'octupole'}[args.method]}, "
+                  f"theta={args.theta:g} (rms rel err vs FP64 direct reported)")
+    shape = {"plummer": "Plummer sphere", "hernquist": "Hernquist sphere",
+             "cube": "uniform cube"}[args.profile]
     line = {
-        "metric": "particles/s per force eval at N=1M, rms rel err <=1e-4 vs FP64 direct",
+        "metric": metric,
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded Plummer sphere)",
+        "data": f"synthetic (seeded {shape})",
         "config": {"workload": workload, "profile": args.profile,
                    "N": N, "seed": args.seed, "theta": args.theta, "eps": args.eps,
                    "order": order,
