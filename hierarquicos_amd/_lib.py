@@ -22,7 +22,7 @@ HQ_FP64, HQ_MIXED = 0, 1
 HQ_LAYOUT_3N, HQ_LAYOUT_N3 = 0, 1
 HQ_TUNE_BUNCH, HQ_TUNE_SUPER, HQ_TUNE_FUSED, HQ_TUNE_LEAF, HQ_TUNE_FMM_LEAF = 1, 2, 3, 4, 5
 HQ_TUNE_GRAPH, HQ_TUNE_SPLIT, HQ_TUNE_FAR, HQ_TUNE_FAR_INNER = 6, 7, 8, 9
-HQ_TUNE_GRAPH_GRID, HQ_TUNE_MAC = 10, 11
+HQ_TUNE_GRAPH_GRID, HQ_TUNE_MAC, HQ_TUNE_FMM_BIN_WINDOW = 10, 11, 12
 HQ_FMM_MFMA, HQ_FMM_FP64 = 0, 1
 HQ_SCHEME_VERLET, HQ_SCHEME_EULER = 0, 1
 HQ_FORCE_TREE, HQ_FORCE_BUNCHED, HQ_FORCE_DIRECT = 0, 1, 2

@@ -197,6 +197,9 @@ int hq_tree_tune(hq_tree* t, int key, int64_t value) {
     } else if (key == HQ_TUNE_MAC) {
         if (value < 0 || value > 1) return fail(HQ_ERR_ARGS, "acceptance test %lld outside 0..1", (long long)value);
         t->mac = (int)value;
+    } else if (key == HQ_TUNE_FMM_BIN_WINDOW) {
+        if (value < 1 || value > 4096) return fail(HQ_ERR_ARGS, "FMM bin window %lld outside 1..4096", (long long)value);
+        t->fmm_binw = (int)value;
     } else if (key == HQ_TUNE_GRAPH_GRID) {
         if (value < 0 || value > ((int64_t)1 << 30)) return fail(HQ_ERR_ARGS, "graph grid %lld out of range", (long long)value);
         t->graph_grid = value;
@@ -284,6 +287,7 @@ int hq_tree_get_tune(const hq_tree* t, int key, int64_t* value) {
         case HQ_TUNE_FAR_INNER: *value = t->far_inner; break;
         case HQ_TUNE_GRAPH_GRID: *value = t->graph_grid; break;
         case HQ_TUNE_MAC: *value = t->mac; break;
+        case HQ_TUNE_FMM_BIN_WINDOW: *value = t->fmm_binw; break;
         default: return fail(HQ_ERR_ARGS, "unknown tuning key %d", key);
     }
     return HQ_OK;

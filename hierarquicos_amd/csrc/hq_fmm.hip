@@ -687,10 +687,10 @@ __device__ __forceinline__ RunInfo run_of(int key, int64_t base, int64_t n) {
 // Lattice pairs by target group, a tile of BT_T * BT_U consecutive pairs at a
 // time.  The walk emits a split pair's children together and its pairs in
 // the order of their ancestors, so a tile's targets fall in a few groups: a
-// tile's runs of equal groups are counted in LDS (a window of BT_W groups
-// from the tile's smallest) and the window's non-zero counts reach global
-// memory with one atomic each.  A group outside the window takes a global
-// atomic per run.
+// tile's runs of equal groups are counted in LDS (a window of win <= BT_W
+// groups from the tile's smallest, HQ_TUNE_FMM_BIN_WINDOW) and the window's
+// non-zero counts reach global memory with one atomic each.  A group outside
+// the window takes a global atomic per run.
 constexpr int BT_T = 256, BT_U = 8, BT_TILE = BT_T * BT_U, BT_W = 4096;
 
 // the smallest and largest valid key of a workgroup's tile, in every thread
@@ -714,7 +714,7 @@ __device__ __forceinline__ int2 tile_key_range(const int (&key)[BT_U], int* __re
 }
 
 __global__ __launch_bounds__(BT_T) void k_fmm_lat_ghist(const int3* __restrict__ lat, int64_t n, int gt,
-                                                        uint32_t* __restrict__ ghist) {
+                                                        int win, uint32_t* __restrict__ ghist) {
     __shared__ uint32_t h[BT_W];
     __shared__ int red[2 * BT_T / 64];
     for (int j = threadIdx.x; j < BT_W; j += BT_T) h[j] = 0;
@@ -736,12 +736,12 @@ __global__ __launch_bounds__(BT_T) void k_fmm_lat_ghist(const int3* __restrict__
             const RunInfo r = run_of(key[u], base, n);
             if (r.head) {
                 const int w = key[u] - kr.x;
-                if (w < BT_W) atomicAdd(&h[w], (uint32_t)r.count);
+                if (w < win) atomicAdd(&h[w], (uint32_t)r.count);
                 else atomicAdd(&ghist[key[u]], (uint32_t)r.count);
             }
         }
         __syncthreads();
-        const int span = min(BT_W, kr.y - kr.x + 1);
+        const int span = min(win, kr.y - kr.x + 1);
         for (int j = threadIdx.x; j < span; j += BT_T) {
             const uint32_t c = h[j];
             if (c) {
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(BT_T) void k_fmm_lat_ghist(const int3* __restrict__
 // a tile's runs take their places within (tile, group) from LDS counters,
 // then each of the tile's groups reserves its places with one global atomic
 __global__ __launch_bounds__(BT_T) void k_fmm_lat_gscatter(const int3* __restrict__ lat, int64_t n,
-                                                           int gt, uint32_t* __restrict__ gpos,
+                                                           int gt, int win, uint32_t* __restrict__ gpos,
                                                            int2* __restrict__ out) {
     __shared__ uint32_t h[BT_W];
     __shared__ int red[2 * BT_T / 64];
@@ -786,13 +786,13 @@ __global__ __launch_bounds__(BT_T) void k_fmm_lat_gscatter(const int3* __restric
             uint32_t o = 0;
             if (r.head) {
                 const int w = key[u] - kr.x;
-                o = w < BT_W ? atomicAdd(&h[w], (uint32_t)r.count)
+                o = w < win ? atomicAdd(&h[w], (uint32_t)r.count)
                              : atomicAdd(&gpos[key[u]], (uint32_t)r.count);
             }
             at[u] = __shfl(o, r.hlane, 64) + (uint32_t)(__lane_id() - r.hlane);
         }
         __syncthreads();
-        const int span = min(BT_W, kr.y - kr.x + 1);
+        const int span = min(win, kr.y - kr.x + 1);
         for (int j = threadIdx.x; j < span; j += BT_T) {
             const uint32_t c = h[j];
             if (c) h[j] = atomicAdd(&gpos[kr.x + j], c);
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(BT_T) void k_fmm_lat_gscatter(const int3* __restric
         for (int u = 0; u < BT_U; ++u) {
             if (key[u] < 0) continue;
             const int w = key[u] - kr.x;
-            const uint32_t o = w < BT_W ? h[w] + at[u] : at[u];
+            const uint32_t o = w < win ? h[w] + at[u] : at[u];
             out[o] = make_int2(e[u].y, (int)(((uint32_t)(e[u].x - key[u] * gt) << 24) | (uint32_t)e[u].z));
         }
         __syncthreads();
@@ -1863,13 +1863,14 @@ static int fmm_run(hq_tree* t, FmmState& st, double theta, double G, double eps2
             (rc = st.lpk.ensure(std::max<int64_t>(nlat, 1) * 4)))
             return rc;
         HQ_CHECK_HIP(hipMemsetAsync(st.ghist.p, 0, (ngroups + 1) * 4, s));
+        const int win = std::min(BT_W, std::max(1, t->fmm_binw));  // groups counted in LDS
         if (nlat)
-            k_fmm_lat_ghist<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT,
+            k_fmm_lat_ghist<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT, win,
                                                          st.ghist.as<uint32_t>());
         if ((rc = scan_excl_buf(st.ghist.as<uint32_t>(), ngroups, t->scanpart, s))) return rc;
         HQ_CHECK_HIP(hipMemcpyAsync(st.gpos.p, st.ghist.p, (ngroups + 1) * 4, hipMemcpyDeviceToDevice, s));
         if (nlat)
-            k_fmm_lat_gscatter<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT,
+            k_fmm_lat_gscatter<<<bin_grid(nlat), BT_T, 0, s>>>(st.lat.as<int3>(), nlat, GT, win,
                                                             st.gpos.as<uint32_t>(), st.gpair.as<int2>());
         const int lsh = lsort_chunk(tb.noff) * (int)sizeof(uint32_t);
         HQ_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fmm_lat_lsort),

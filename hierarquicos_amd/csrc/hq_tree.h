@@ -100,6 +100,7 @@ struct hq_tree {
     uint64_t gcache_clock = 0;
     int64_t graph_grid = 0;  // HQ_TUNE_GRAPH_GRID: that grid if > 0, else 2 x bunches + 64
     int mac = 1;             // HQ_TUNE_MAC of the bunched walks: 0 cell side, 1 side + offset
+    int fmm_binw = 4096;     // HQ_TUNE_FMM_BIN_WINDOW: target groups counted in LDS per tile
     unsigned tune_user = 0;  // HQ_TUNE_* keys the caller set (bit per key): the table keeps off them
     int table_tag = -1;      // angle and precision of the last setting taken from the tuning table
     hq::DevBuf tune_buf, tune_tmp;  // autotuner: sample slots, direct forces, error; direct scratch

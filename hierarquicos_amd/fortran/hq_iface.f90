@@ -22,7 +22,8 @@ MODULE hq_iface
     INTEGER(C_INT), PARAMETER :: HQ_TUNE_BUNCH = 1, HQ_TUNE_SUPER = 2, HQ_TUNE_FUSED = 3
     INTEGER(C_INT), PARAMETER :: HQ_TUNE_LEAF = 4, HQ_TUNE_FMM_LEAF = 5, HQ_TUNE_GRAPH = 6, &
                                    HQ_TUNE_SPLIT = 7, HQ_TUNE_FAR = 8, HQ_TUNE_FAR_INNER = 9, &
-                                   HQ_TUNE_GRAPH_GRID = 10, HQ_TUNE_MAC = 11
+                                   HQ_TUNE_GRAPH_GRID = 10, HQ_TUNE_MAC = 11, &
+                                   HQ_TUNE_FMM_BIN_WINDOW = 12
     ! FMM modes
     INTEGER(C_INT), PARAMETER :: HQ_FMM_MFMA = 0, HQ_FMM_FP64 = 1
     ! integration

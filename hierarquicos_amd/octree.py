@@ -146,7 +146,8 @@ class OctreeType:
         return self._run(fn, out, L.HQ_LAYOUT_3N, theta2, G, eps2)
 
     def tune(self, bunch=None, super_size=None, fused=None, leaf=None, fmm_leaf=None,
-             graph=None, split=None, far=None, far_inner=None, graph_grid=None, mac=None):
+             graph=None, split=None, far=None, far_inner=None, graph_grid=None, mac=None,
+             fmm_bin_window=None):
         """Bunch size (<= 64), super-group size, fused/two-kernel mode and leaf
         bucket (cells of <= leaf bodies enter as their bodies) of the bunched
         walk; FMM leaf size; graph: whether integrate() replays one captured
@@ -158,7 +159,10 @@ class OctreeType:
         enter an expansion of higher order; graph_grid: the replayed walk's grid
         in bunches (0, default: twice the first step's bunch count plus 64);
         mac: the bunched walks' acceptance test, "side" (the grouped test of
-        octree_default.f90:1927-1937) or "offset" (side + theta |com - centre|)."""
+        octree_default.f90:1927-1937) or "offset" (side + theta |com - centre|);
+        fmm_bin_window: target groups whose lattice-pair counts a tile keeps in
+        LDS (1..4096, default 4096; fewer send more pairs through global
+        atomics, which changes only the order of the FMM's sums)."""
         lib = L.lib()
         if mac is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_MAC, {"side": 0, "offset": 1}[mac]))
@@ -175,6 +179,8 @@ class OctreeType:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_GRAPH, int(bool(graph))))
         if fmm_leaf is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FMM_LEAF, int(fmm_leaf)))
+        if fmm_bin_window is not None:
+            L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_FMM_BIN_WINDOW, int(fmm_bin_window)))
         if leaf is not None:
             L.check(lib.hq_tree_tune(self._h, L.HQ_TUNE_LEAF, int(leaf)))
         if fused is not None:
@@ -246,7 +252,8 @@ class OctreeType:
                 "split": self.get_tune("split"), "ms": ms.value}
 
     _TUNE_KEYS = {"bunch": 1, "super_size": 2, "fused": 3, "leaf": 4, "fmm_leaf": 5, "graph": 6,
-                  "split": 7, "far": 8, "far_inner": 9, "graph_grid": 10, "mac": 11}
+                  "split": 7, "far": 8, "far_inner": 9, "graph_grid": 10, "mac": 11,
+                  "fmm_bin_window": 12}
 
     def get_tune(self, key):
         """Current value of a tuning knob (the names of tune(); far and

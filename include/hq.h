@@ -145,6 +145,10 @@ int hq_tree_destroy(hq_tree *t);
                               opens cells whose centre of mass sits near a face (at 1M Plummer
                               bodies, octupole, theta 0.5: 18% lower RMS error for 4% more
                               time).  Methods 30-32 keep the reference's test */
+#define HQ_TUNE_FMM_BIN_WINDOW 12 /* matrix-core FMM: target groups whose pair counts a tile
+                              of lattice pairs keeps in LDS, from the tile's smallest group
+                              (1..4096, default 4096); the pairs of groups beyond it take one
+                              global atomic per run.  Changes only the order of the sums */
 int hq_tree_tune(hq_tree *t, int key, int64_t value);
 
 /* Cartesian FMM of order 4 or 6 on the built tree: Dehnen's method carried to

@@ -84,3 +84,22 @@ def test_fmm_rejects_bad_arguments():
         t.fmm(0.25, 1.0, 0.0, 5)
     with pytest.raises(HQError):
         t.fmm(1.0, 1.0, 0.0, 4)
+
+
+@pytest.mark.parametrize("window", [1, 3])
+def test_fmm_binning_outside_the_lds_window(window):
+    """Lattice pairs whose target group lies beyond the tile's LDS window of
+    group counts take global atomics; with a window of one or three groups
+    most of them do.  The bins then hold the same pairs in another order, so
+    the forces equal the default's to FP32 round-off of the M2L sums."""
+    m, q, Fd, t = _case("plummer", 20000, seed=9)
+    F0 = t.fmm(0.25, 1.0, 0.0, 4, "mfma").T
+    assert t.get_tune("fmm_bin_window") == 4096
+    t.tune(fmm_bin_window=window)
+    assert t.get_tune("fmm_bin_window") == window
+    F1 = t.fmm(0.25, 1.0, 0.0, 4, "mfma").T
+    diff = np.linalg.norm(F1 - F0, axis=1) / np.linalg.norm(F0, axis=1)
+    assert float(np.sqrt(np.mean(diff ** 2))) < 1e-6 and float(diff.max()) < 1e-4, float(diff.max())
+    assert abs(O.rel_errors(F1, Fd)[0] - O.rel_errors(F0, Fd)[0]) < 1e-6
+    with pytest.raises(HQError):
+        t.tune(fmm_bin_window=0)
